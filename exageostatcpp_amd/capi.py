@@ -1,0 +1,272 @@
+"""ctypes binding of the C ABI in include/exageostat_amd.h.
+
+The library must be built (``__graft_entry__.build()`` or
+``python exageostatcpp_amd/build.py``); loading fails loudly otherwise.  There
+is no CPU fallback: every numerical result comes from the gfx950 kernels.
+"""
+import ctypes
+import ctypes.util
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libexageostat_amd.so")
+
+SUCCESS = 0
+ERR_INVALID_ARG = 1
+ERR_HIP = 2
+ERR_NOT_SPD = 3
+ERR_OUT_OF_MEMORY = 4
+ERR_UNSUPPORTED = 5
+ERR_NO_DEVICE = 6
+ERR_STATE = 7
+
+KERNEL_UNIVARIATE = 0
+KERNEL_BIVARIATE = 1
+KERNELS = {"UnivariateMaternStationary": 0, "univariate_matern_stationary": 0,
+           "BivariateMaternParsimonious": 1, "bivariate_matern_parsimonious": 1}
+PRECISION_FP64 = 0
+PRECISION_MIXED = 1
+PROFILE_UPDATE = 0
+PROFILE_COVARIANCE = 1
+PROFILE_PANEL_TRSM = 2
+PROFILE_DIAGONAL = 3
+PROFILE_SOLVE = 4
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_i64 = ctypes.c_int64
+_int = ctypes.c_int
+_vp = ctypes.c_void_p
+_lib = None
+
+
+class EGSError(RuntimeError):
+    def __init__(self, code, message):
+        super().__init__(message)
+        self.code = code
+
+
+class NotPositiveDefinite(EGSError):
+    pass
+
+
+EXPORTS = {
+    "egs_last_error": ([], ctypes.c_char_p),
+    "egs_version": ([], _int),
+    "egs_init": ([_int, ctypes.POINTER(_vp)], _int),
+    "egs_finalize": ([_vp], _int),
+    "egs_synchronize": ([_vp], _int),
+    "egs_covariance_tile": ([_vp, _int, _dp, _int, _int, _int, _int, _dp, _dp, _dp, _i64, _dp, _dp, _dp, _i64, _dp,
+                             _int], _int),
+    "egs_problem_create": ([_vp, _int, _i64, _dp, _dp, _dp, _int, _int, ctypes.POINTER(_vp)], _int),
+    "egs_problem_destroy": ([_vp], _int),
+    "egs_problem_device_bytes": ([_vp, ctypes.POINTER(_i64)], _int),
+    "egs_problem_set_precision": ([_vp, _int, _int, _int], _int),
+    "egs_problem_set_observations": ([_vp, _dp, _i64], _int),
+    "egs_problem_get_observations": ([_vp, _dp, _i64], _int),
+    "egs_loglik": ([_vp, _dp, _dp, _dp, _dp], _int),
+    "egs_covariance": ([_vp, _dp], _int),
+    "egs_potrf": ([_vp], _int),
+    "egs_solve_logdet_quad": ([_vp, _dp, _dp], _int),
+    "egs_get_matrix": ([_vp, _dp, _i64], _int),
+    "egs_last_timings": ([_vp, _dp, _dp, _dp], _int),
+    "egs_problem_profile": ([_vp, _int], _int),
+    "egs_problem_kernel_stats": ([_vp, _int, ctypes.POINTER(_i64), _dp, _dp], _int),
+    "egs_generate_locations": ([_i64, _int, _dp, _dp, _dp], _int),
+    "egs_generate_observations": ([_vp, _dp, _int, _dp], _int),
+    "egs_predict": ([_vp, _dp, _i64, _dp, _dp, _dp, _dp, _dp, _dp], _int),
+}
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("libexageostat_amd.so is not built (run __graft_entry__.build()); "
+                               "there is no CPU fallback")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (args, res) in EXPORTS.items():
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = res
+        _lib = L
+    return _lib
+
+
+def _check(code):
+    if code != SUCCESS:
+        msg = lib().egs_last_error().decode()
+        if code == ERR_NOT_SPD:
+            raise NotPositiveDefinite(code, msg)
+        raise EGSError(code, msg)
+
+
+def _arr(a):
+    if a is None:
+        return None, None
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    return a, a.ctypes.data_as(_dp)
+
+
+def _kid(kernel):
+    return KERNELS[kernel] if isinstance(kernel, str) else int(kernel)
+
+
+class Context:
+    """One GPU: ExaGeoStatHardware's role for this library."""
+
+    def __init__(self, device=0):
+        h = _vp()
+        _check(lib().egs_init(device, ctypes.byref(h)))
+        self.handle = h
+        self.device = device
+
+    def synchronize(self):
+        _check(lib().egs_synchronize(self.handle))
+
+    def close(self):
+        if self.handle:
+            lib().egs_finalize(self.handle)
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def covariance_tile(self, kernel, rows, cols, row_off, col_off, loc1, loc2, theta, metric=0):
+        """Kernel<T>::GenerateCovarianceMatrix on the GPU; returns a (rows, cols) array."""
+        out = np.zeros(rows * cols)
+        x1, px1 = _arr(loc1[0])
+        y1, py1 = _arr(loc1[1])
+        z1, pz1 = _arr(loc1[2] if len(loc1) > 2 else None)
+        x2, px2 = _arr(loc2[0])
+        y2, py2 = _arr(loc2[1])
+        z2, pz2 = _arr(loc2[2] if len(loc2) > 2 else None)
+        th, pth = _arr(theta)
+        _check(lib().egs_covariance_tile(self.handle, _kid(kernel), out.ctypes.data_as(_dp), rows, cols, row_off,
+                                         col_off, px1, py1, pz1, len(x1), px2, py2, pz2, len(x2), pth, metric))
+        return out.reshape(cols, rows).T.copy()
+
+
+class Problem:
+    """A location set with its device-resident covariance tiles (InitiateDescriptors)."""
+
+    def __init__(self, ctx, kernel, x, y, z=None, metric=0, dts=960):
+        self.ctx = ctx
+        self.kernel = _kid(kernel)
+        self.p = 2 if self.kernel == KERNEL_BIVARIATE else 1
+        self.x, px = _arr(x)
+        self.y, py = _arr(y)
+        self.z, pz = _arr(z)
+        self.n_loc = len(self.x)
+        self.n = self.n_loc * self.p
+        self.dts = dts
+        h = _vp()
+        _check(lib().egs_problem_create(ctx.handle, self.kernel, self.n_loc, px, py, pz, metric, dts,
+                                        ctypes.byref(h)))
+        self.handle = h
+
+    def close(self):
+        if self.handle:
+            lib().egs_problem_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def device_bytes(self):
+        b = _i64()
+        _check(lib().egs_problem_device_bytes(self.handle, ctypes.byref(b)))
+        return b.value
+
+    def set_precision(self, mode, band_fp64=2, band_fp32=10):
+        _check(lib().egs_problem_set_precision(self.handle, mode, band_fp64, band_fp32))
+
+    def set_observations(self, z):
+        z, pz = _arr(z)
+        _check(lib().egs_problem_set_observations(self.handle, pz, len(z)))
+
+    def get_observations(self):
+        out = np.empty(self.n)
+        _check(lib().egs_problem_get_observations(self.handle, out.ctypes.data_as(_dp), self.n))
+        return out
+
+    def loglik(self, theta):
+        th, pth = _arr(theta)
+        llh, ld, q = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        _check(lib().egs_loglik(self.handle, pth, ctypes.byref(llh), ctypes.byref(ld), ctypes.byref(q)))
+        return llh.value, ld.value, q.value
+
+    def covariance(self, theta):
+        th, pth = _arr(theta)
+        _check(lib().egs_covariance(self.handle, pth))
+
+    def potrf(self):
+        _check(lib().egs_potrf(self.handle))
+
+    def solve_logdet_quad(self):
+        ld, q = ctypes.c_double(), ctypes.c_double()
+        _check(lib().egs_solve_logdet_quad(self.handle, ctypes.byref(ld), ctypes.byref(q)))
+        return ld.value, q.value
+
+    def get_matrix(self):
+        """Lower tiles of the current device matrix as an (n, n) array (upper part zero)."""
+        out = np.zeros(self.n * self.n)
+        _check(lib().egs_get_matrix(self.handle, out.ctypes.data_as(_dp), self.n))
+        return out.reshape(self.n, self.n).T.copy()
+
+    def timings(self):
+        a, b, c = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        _check(lib().egs_last_timings(self.handle, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)))
+        return {"covariance_ms": a.value, "factor_ms": b.value, "solve_ms": c.value}
+
+    def profile(self, enable=True):
+        _check(lib().egs_problem_profile(self.handle, 1 if enable else 0))
+
+    def kernel_stats(self, kernel):
+        n, ms, fl = _i64(), ctypes.c_double(), ctypes.c_double()
+        _check(lib().egs_problem_kernel_stats(self.handle, kernel, ctypes.byref(n), ctypes.byref(ms),
+                                              ctypes.byref(fl)))
+        return {"launches": n.value, "ms": ms.value, "flops": fl.value}
+
+    def generate_observations(self, theta, seed=0):
+        th, pth = _arr(theta)
+        out = np.empty(self.n)
+        _check(lib().egs_generate_observations(self.handle, pth, seed, out.ctypes.data_as(_dp)))
+        return out
+
+    def predict(self, theta, xm, ym, zm=None, z_actual=None):
+        th, pth = _arr(theta)
+        xm, pxm = _arr(xm)
+        ym, pym = _arr(ym)
+        zm, pzm = _arr(zm)
+        za, pza = _arr(z_actual)
+        out = np.empty(len(xm) * self.p)
+        mspe = ctypes.c_double()
+        _check(lib().egs_predict(self.handle, pth, len(xm), pxm, pym, pzm, out.ctypes.data_as(_dp), pza,
+                                 ctypes.byref(mspe)))
+        return out, mspe.value
+
+
+def srand(seed):
+    """Seeds the C library's rand(), as ExaGeoStat<T>::ExaGeoStatLoadData does (ExaGeoStat.cpp:76)."""
+    libc = ctypes.CDLL(ctypes.util.find_library("c") or "libc.so.6")
+    libc.srand(ctypes.c_uint(seed))
+
+
+def generate_locations(n, dim=2, seed=None):
+    """LocationGenerator<T>::GenerateLocations through the library (rand() after srand(seed))."""
+    if seed is not None:
+        srand(seed)
+    x = np.empty(n)
+    y = np.empty(n)
+    z = np.empty(n) if dim == 3 else None
+    _check(lib().egs_generate_locations(n, dim, x.ctypes.data_as(_dp), y.ctypes.data_as(_dp),
+                                        None if z is None else z.ctypes.data_as(_dp)))
+    return (x, y) if dim != 3 else (x, y, z)

@@ -1,0 +1,865 @@
+// C ABI of the gfx950 maximum-likelihood path (include/exageostat_amd.h).
+//
+// Host orchestration only: shape checks, HBM layout of a problem, the order of
+// kernel launches on the context's stream.  All arithmetic on Sigma runs in the
+// HIP kernels of egs_covgen.hip, egs_chol.hip and egs_solve.hip; the host adds
+// up nt per-tile partials in tile order and evaluates the closing formula of
+// ChameleonImplementation.cpp:257-265.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/exageostat_amd.h"
+#include "egs_common.hpp"
+#include "egs_kernels.hpp"
+#include "egs_matern.hpp"
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const char *fmt, ...) {
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_last_error = buf;
+    return code;
+}
+
+#define EGS_HIP(expr)                                                                              \
+    do {                                                                                           \
+        hipError_t e_ = (expr);                                                                    \
+        if (e_ != hipSuccess) return fail(EGS_ERR_HIP, "%s: %s", #expr, hipGetErrorString(e_));   \
+    } while (0)
+
+constexpr double kPi = 3.141592653589793;   // Definitions.hpp:45
+constexpr int kMaxTileSize = 2048;
+
+}  // namespace
+
+struct egs_problem_s;
+
+struct egs_context_s {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::vector<egs_problem_s *> problems;   // live problems, released by egs_finalize
+};
+
+// Per-kernel timing with HIP events on the library stream (enabled by
+// egs_problem_profile); collected after the stream synchronizes.
+struct KernelStat {
+    int64_t launches = 0;
+    double ms = 0.0;
+    double flops = 0.0;
+};
+
+struct Profiler {
+    bool on = false;
+    std::vector<hipEvent_t> ev;            // pairs: start, stop
+    std::vector<std::pair<int, double>> pending;   // (kernel id, flops) per used pair
+    KernelStat stats[EGS_PROFILE_KERNELS];
+};
+
+struct egs_problem_s {
+    egs_context_s *ctx = nullptr;
+    int kernel = 0, p = 1, metric = 0, dim = 2;
+    int dts = 0, ts = 0, nt = 0, nsb = 0;
+    int64_t n_loc = 0, n = 0, npad = 0;
+    int mode = EGS_PRECISION_FP64, band64 = 2, band32 = 10;
+
+    double *d_x = nullptr, *d_y = nullptr, *d_z = nullptr;
+    char *d_arena = nullptr;
+    size_t arena_bytes = 0;
+    int64_t *d_off = nullptr;
+    uint8_t *d_store = nullptr;
+    int2 *d_pairs = nullptr;
+    double *d_winv = nullptr;
+    double *d_obs = nullptr, *d_work = nullptr, *d_aux = nullptr, *d_partials = nullptr;
+    int *d_info = nullptr;
+
+    std::vector<int64_t> h_off;
+    std::vector<uint8_t> h_store;
+    std::vector<int2> h_pairs;
+    std::vector<double> h_partials;
+
+    bool has_obs = false;
+    bool factored = false;
+    double factored_theta[6] = {0, 0, 0, 0, 0, 0};
+    double cov_theta[6] = {0, 0, 0, 0, 0, 0};
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    double t_cov = 0, t_fact = 0, t_solve = 0;
+    Profiler prof;
+
+    egs::TileMap map() const {
+        egs::TileMap m;
+        m.base = d_arena;
+        m.off = d_off;
+        m.store = d_store;
+        m.nt = nt;
+        m.ts = ts;
+        return m;
+    }
+    int ntheta() const { return kernel == 1 ? 6 : 3; }
+    double *winv(int k) const { return d_winv + (int64_t) k * nsb * egs::kBlk * egs::kBlk; }
+};
+
+namespace {
+
+int check_problem(egs_problem_t p) {
+    if (!p) return fail(EGS_ERR_INVALID_ARG, "null problem");
+    if (!p->ctx) return fail(EGS_ERR_STATE, "the problem's context was finalized");
+    if (hipSetDevice(p->ctx->device) != hipSuccess) return fail(EGS_ERR_HIP, "hipSetDevice failed");
+    return EGS_SUCCESS;
+}
+
+constexpr double kMaxNu = 100.0;
+
+bool valid_nu(double nu) { return nu > 0.0 && nu <= kMaxNu; }
+
+int make_params(int kernel, int metric, const double *th, egs::MaternParams &prm) {
+    if (!th) return fail(EGS_ERR_INVALID_ARG, "theta is null");
+    const int nth = kernel == EGS_KERNEL_BIVARIATE_MATERN_PARSIMONIOUS ? 6 : 3;
+    for (int q = 0; q < nth; ++q)
+        if (!std::isfinite(th[q])) return fail(EGS_ERR_INVALID_ARG, "theta[%d] is not finite", q);
+    const double range = kernel == EGS_KERNEL_BIVARIATE_MATERN_PARSIMONIOUS ? th[2] : th[1];
+    if (!(range > 0.0)) return fail(EGS_ERR_INVALID_ARG, "the range parameter must be positive");
+    if (kernel == EGS_KERNEL_BIVARIATE_MATERN_PARSIMONIOUS ? !(valid_nu(th[3]) && valid_nu(th[4])) : !valid_nu(th[2]))
+        return fail(EGS_ERR_INVALID_ARG, "smoothness must lie in (0, %g]", kMaxNu);
+    prm.kernel = kernel;
+    prm.metric = metric;
+    if (kernel == EGS_KERNEL_UNIVARIATE_MATERN_STATIONARY) {
+        // UnivariateMaternStationary.cpp:47-49
+        const double sigma2 = th[0], nu = th[2];
+        prm.range = th[1];
+        for (int q = 0; q < 4; ++q) {
+            prm.sig[q] = sigma2;
+            prm.nu[q] = nu;
+            prm.con[q] = sigma2 * (1.0 / (std::pow(2.0, nu - 1) * std::tgamma(nu)));
+        }
+        return EGS_SUCCESS;
+    }
+    if (kernel == EGS_KERNEL_BIVARIATE_MATERN_PARSIMONIOUS) {
+        // BivariateMaternParsimonious.cpp:50-66
+        const double con1 = th[0] * (1.0 / (std::pow(2.0, th[3] - 1) * std::tgamma(th[3])));
+        const double con2 = th[1] * (1.0 / (std::pow(2.0, th[4] - 1) * std::tgamma(th[4])));
+        const double nu12 = 0.5 * (th[3] + th[4]);
+        const double rho = th[5] *
+                           std::sqrt((std::tgamma(th[3] + 1) * std::tgamma(th[4] + 1)) /
+                                     (std::tgamma(th[3]) * std::tgamma(th[4]))) *
+                           std::tgamma(nu12) / std::tgamma(nu12 + 1);
+        const double con12 = rho * std::sqrt(th[0] * th[1]) * (1.0 / (std::pow(2.0, nu12 - 1) * std::tgamma(nu12)));
+        prm.range = th[2];
+        prm.sig[0] = th[0];
+        prm.sig[1] = prm.sig[2] = rho * std::sqrt(th[0] * th[1]);
+        prm.sig[3] = th[1];
+        prm.nu[0] = th[3];
+        prm.nu[1] = prm.nu[2] = nu12;
+        prm.nu[3] = th[4];
+        prm.con[0] = con1;
+        prm.con[1] = prm.con[2] = con12;
+        prm.con[3] = con2;
+        return EGS_SUCCESS;
+    }
+    return fail(EGS_ERR_INVALID_ARG, "unknown kernel %d", kernel);
+}
+
+void free_problem_buffers(egs_problem_s *p) {
+    if (p->ctx) {
+        auto &v = p->ctx->problems;
+        v.erase(std::remove(v.begin(), v.end(), p), v.end());
+    }
+    hipFree(p->d_x);
+    hipFree(p->d_y);
+    hipFree(p->d_z);
+    hipFree(p->d_arena);
+    hipFree(p->d_off);
+    hipFree(p->d_store);
+    hipFree(p->d_pairs);
+    hipFree(p->d_winv);
+    hipFree(p->d_obs);
+    hipFree(p->d_work);
+    hipFree(p->d_aux);
+    hipFree(p->d_partials);
+    hipFree(p->d_info);
+    for (auto &e : p->ev)
+        if (e) hipEventDestroy(e);
+    for (auto &e : p->prof.ev)
+        if (e) hipEventDestroy(e);
+    p->prof.ev.clear();
+    p->prof.pending.clear();
+    p->d_x = p->d_y = p->d_z = nullptr;
+    p->d_arena = nullptr;
+    p->d_off = nullptr;
+    p->d_store = nullptr;
+    p->d_pairs = nullptr;
+    p->d_winv = p->d_obs = p->d_work = p->d_aux = p->d_partials = nullptr;
+    p->d_info = nullptr;
+    for (auto &e : p->ev) e = nullptr;
+}
+
+// Opens a timed region for kernel `id` on the problem's stream; returns the
+// event pair index or -1 when profiling is off.
+int prof_begin(egs_problem_s *p, int id, double flops) {
+    if (!p->prof.on) return -1;
+    Profiler &pr = p->prof;
+    const size_t idx = pr.pending.size();
+    if (2 * idx + 2 > pr.ev.size()) {
+        hipEvent_t a = nullptr, b = nullptr;
+        if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) return -1;
+        pr.ev.push_back(a);
+        pr.ev.push_back(b);
+    }
+    if (hipEventRecord(pr.ev[2 * idx], p->ctx->stream) != hipSuccess) return -1;
+    pr.pending.emplace_back(id, flops);
+    return (int) idx;
+}
+
+void prof_end(egs_problem_s *p, int idx) {
+    if (idx < 0) return;
+    hipEventRecord(p->prof.ev[2 * (size_t) idx + 1], p->ctx->stream);
+}
+
+// Call after the stream has synchronized.
+void prof_collect(egs_problem_s *p) {
+    Profiler &pr = p->prof;
+    for (size_t i = 0; i < pr.pending.size(); ++i) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, pr.ev[2 * i], pr.ev[2 * i + 1]) != hipSuccess) continue;
+        KernelStat &ks = pr.stats[pr.pending[i].first];
+        ks.launches += 1;
+        ks.ms += ms;
+        ks.flops += pr.pending[i].second;
+    }
+    pr.pending.clear();
+}
+
+// Rows of tile i that belong to the matrix (the last tile row is short).
+double real_rows(const egs_problem_s *p, int i) {
+    const int64_t r = p->n - (int64_t) i * p->ts;
+    return (double) std::min<int64_t>(p->ts, std::max<int64_t>(0, r));
+}
+
+// Storage precision of tile (i, j) under the problem's plan.
+uint8_t tile_store(const egs_problem_s *p, int i, int j) {
+    if (p->mode == EGS_PRECISION_FP64) return egs::kStoreF64;
+    return (i - j) <= p->band64 ? egs::kStoreF64 : egs::kStoreF32;
+}
+
+// (Re)build the tile layout in HBM for the current plan.
+int layout_tiles(egs_problem_s *p) {
+    const int nt = p->nt;
+    p->h_off.assign((size_t) nt * nt, -1);
+    p->h_store.assign((size_t) nt * nt, 0);
+    size_t off = 0;
+    const size_t tile_elems = (size_t) p->ts * p->ts;
+    for (int j = 0; j < nt; ++j) {
+        for (int i = j; i < nt; ++i) {
+            const uint8_t st = tile_store(p, i, j);
+            p->h_store[(size_t) i * nt + j] = st;
+            p->h_off[(size_t) i * nt + j] = (int64_t) off;
+            off += tile_elems * (st == egs::kStoreF64 ? 8 : 4);
+            off = (off + 255) & ~(size_t) 255;
+        }
+    }
+    if (p->d_arena && p->arena_bytes != off) {
+        hipFree(p->d_arena);
+        p->d_arena = nullptr;
+    }
+    if (!p->d_arena) {
+        if (hipMalloc(&p->d_arena, off) != hipSuccess) {
+            p->d_arena = nullptr;
+            return fail(EGS_ERR_OUT_OF_MEMORY, "cannot allocate %zu bytes for the covariance tiles", off);
+        }
+        p->arena_bytes = off;
+    }
+    EGS_HIP(hipMemcpy(p->d_off, p->h_off.data(), sizeof(int64_t) * p->h_off.size(), hipMemcpyHostToDevice));
+    EGS_HIP(hipMemcpy(p->d_store, p->h_store.data(), p->h_store.size(), hipMemcpyHostToDevice));
+    p->factored = false;
+    return EGS_SUCCESS;
+}
+
+int run_covariance(egs_problem_s *p, const double *theta) {
+    egs::MaternParams prm;
+    int st = make_params(p->kernel, p->metric, theta, prm);
+    if (st) return st;
+    const double *dz = (p->kernel == EGS_KERNEL_UNIVARIATE_MATERN_STATIONARY) ? p->d_z : nullptr;
+    const int pi = prof_begin(p, EGS_PROFILE_COVARIANCE, 0.0);
+    egs::launch_covgen_tiles(p->map(), p->d_pairs, (int) p->h_pairs.size(), prm, p->d_x, p->d_y, dz, p->n,
+                             p->ctx->stream);
+    prof_end(p, pi);
+    EGS_HIP(hipGetLastError());
+    p->factored = false;
+    std::memcpy(p->cov_theta, theta, sizeof(double) * p->ntheta());
+    return EGS_SUCCESS;
+}
+
+// Enqueue the tiled Cholesky; the caller checks d_info after synchronizing.
+int run_potrf(egs_problem_s *p) {
+    hipStream_t s = p->ctx->stream;
+    const egs::TileMap m = p->map();
+    const int nt = p->nt, nsb = p->nsb;
+    EGS_HIP(hipMemsetAsync(p->d_info, 0, sizeof(int), s));
+    const double ts = (double) p->ts;
+    for (int k = 0; k < nt; ++k) {
+        double *wk = p->winv(k);
+        const double kb = real_rows(p, k);
+        int pi = prof_begin(p, EGS_PROFILE_DIAGONAL, kb * kb * kb / 3.0);
+        for (int jb = 0; jb < nsb; ++jb) {
+            egs::launch_potrf_block(m, k, jb, wk, p->d_info, p->n, s);
+            if (jb + 1 < nsb)
+                egs::launch_trsm_rows(m, k, (int64_t) k * nsb + jb + 1, nsb - jb - 1, jb, jb + 1, wk, s);
+        }
+        prof_end(p, pi);
+        if (k + 1 < nt) {
+            const int below = nt - k - 1;
+            double rows_below = 0.0, upd = 0.0;
+            for (int i = k + 1; i < nt; ++i) {
+                const double ri = real_rows(p, i);
+                rows_below += ri;
+                upd += ri * (ri + 1.0) * ts;                       // SYRK on the diagonal tile
+                for (int j = k + 1; j < i; ++j) upd += 2.0 * ri * real_rows(p, j) * ts;   // GEMM
+            }
+            pi = prof_begin(p, EGS_PROFILE_PANEL_TRSM, rows_below * ts * ts);
+            egs::launch_trsm_rows(m, k, (int64_t) (k + 1) * nsb, below * nsb, 0, nsb, wk, s);
+            prof_end(p, pi);
+            pi = prof_begin(p, EGS_PROFILE_UPDATE, upd);
+            egs::launch_update(m, p->d_pairs, below * (below + 1) / 2, k, s);
+            prof_end(p, pi);
+        }
+    }
+    EGS_HIP(hipGetLastError());
+    return EGS_SUCCESS;
+}
+
+int read_info(egs_problem_s *p) {
+    prof_collect(p);
+    int info = 0;
+    EGS_HIP(hipMemcpy(&info, p->d_info, sizeof(int), hipMemcpyDeviceToHost));
+    if (info != 0) {
+        p->factored = false;
+        return fail(EGS_ERR_NOT_SPD, "dpotrf failed, Matrix is not positive definite (leading minor %d)", info);
+    }
+    return EGS_SUCCESS;
+}
+
+// Forward solve of d_work in place with the per-tile partials.
+void run_forward(egs_problem_s *p) {
+    hipStream_t s = p->ctx->stream;
+    const egs::TileMap m = p->map();
+    const int pi = prof_begin(p, EGS_PROFILE_SOLVE, (double) p->n * (double) p->n);
+    for (int k = 0; k < p->nt; ++k) {
+        egs::launch_trsv_diag(m, k, p->winv(k), p->d_work, p->d_partials, s);
+        egs::launch_trsv_panel(m, k, p->d_work, s);
+    }
+    prof_end(p, pi);
+}
+
+void run_backward(egs_problem_s *p) {
+    hipStream_t s = p->ctx->stream;
+    const egs::TileMap m = p->map();
+    for (int k = p->nt - 1; k >= 0; --k) {
+        egs::launch_trsv_t_partial(m, k, p->d_work, p->d_aux, s);
+        egs::launch_trsv_t_diag(m, k, p->winv(k), p->d_work, p->d_aux, s);
+    }
+}
+
+int read_partials(egs_problem_s *p, double &logdet, double &quad) {
+    p->h_partials.resize((size_t) 2 * p->nt);
+    EGS_HIP(hipMemcpy(p->h_partials.data(), p->d_partials, sizeof(double) * 2 * p->nt, hipMemcpyDeviceToHost));
+    double ld = 0.0, q = 0.0;
+    for (int k = 0; k < p->nt; ++k) {
+        ld += p->h_partials[2 * k];
+        q += p->h_partials[2 * k + 1];
+    }
+    logdet = 2.0 * ld;
+    quad = q;
+    return EGS_SUCCESS;
+}
+
+uint64_t morton_spread(uint64_t v) {
+    uint64_t out = 0;
+    for (int b = 0; b < 16; ++b) out |= ((v >> b) & 1ull) << (4 * b);
+    return out;
+}
+
+uint64_t morton_compact(uint64_t v) {
+    uint64_t out = 0;
+    for (int b = 0; b < 16; ++b) out |= ((v >> (4 * b)) & 1ull) << b;
+    return out;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *egs_last_error(void) { return g_last_error.c_str(); }
+
+int egs_version(void) { return 100; }
+
+int egs_init(int device, egs_context_t *ctx) {
+    if (!ctx) return fail(EGS_ERR_INVALID_ARG, "ctx is null");
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count <= 0)
+        return fail(EGS_ERR_NO_DEVICE, "no HIP device is visible");
+    if (device < 0 || device >= count) return fail(EGS_ERR_INVALID_ARG, "device %d out of range [0, %d)", device, count);
+    EGS_HIP(hipSetDevice(device));
+    auto *c = new egs_context_s();
+    c->device = device;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return fail(EGS_ERR_HIP, "hipStreamCreate failed");
+    }
+    *ctx = c;
+    return EGS_SUCCESS;
+}
+
+int egs_finalize(egs_context_t ctx) {
+    if (!ctx) return EGS_SUCCESS;
+    hipSetDevice(ctx->device);
+    hipStreamSynchronize(ctx->stream);
+    // problems still alive lose their device memory; destroying them later is safe
+    while (!ctx->problems.empty()) {
+        egs_problem_s *p = ctx->problems.back();
+        free_problem_buffers(p);
+        p->ctx = nullptr;
+    }
+    hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return EGS_SUCCESS;
+}
+
+int egs_synchronize(egs_context_t ctx) {
+    if (!ctx) return fail(EGS_ERR_INVALID_ARG, "ctx is null");
+    EGS_HIP(hipSetDevice(ctx->device));
+    EGS_HIP(hipStreamSynchronize(ctx->stream));
+    return EGS_SUCCESS;
+}
+
+int egs_covariance_tile(egs_context_t ctx, int kernel, double *A, int rows, int cols, int row_off, int col_off,
+                        const double *x1, const double *y1, const double *z1, int64_t n1, const double *x2,
+                        const double *y2, const double *z2, int64_t n2, const double *theta, int metric) {
+    if (!ctx || !A || !x1 || !y1 || !x2 || !y2) return fail(EGS_ERR_INVALID_ARG, "null argument");
+    if (rows < 0 || cols < 0 || row_off < 0 || col_off < 0) return fail(EGS_ERR_INVALID_ARG, "negative size");
+    if (metric != 0 && metric != 1) return fail(EGS_ERR_INVALID_ARG, "unknown distance metric %d", metric);
+    const int pvars = kernel == EGS_KERNEL_BIVARIATE_MATERN_PARSIMONIOUS ? 2 : 1;
+    if (kernel == EGS_KERNEL_BIVARIATE_MATERN_PARSIMONIOUS && ((row_off | col_off) & 1))
+        return fail(EGS_ERR_INVALID_ARG, "bivariate tiles need even row and column offsets");
+    if ((int64_t) row_off + rows > n1 * pvars || (int64_t) col_off + cols > n2 * pvars)
+        return fail(EGS_ERR_INVALID_ARG, "tile exceeds the location sets");
+    const bool three_d = (kernel == EGS_KERNEL_UNIVARIATE_MATERN_STATIONARY) && z1 && z2;
+    if (three_d && metric == 1) return fail(EGS_ERR_INVALID_ARG, "Great Circle (GC) distance is only valid for 2D!");
+    egs::MaternParams prm;
+    int st = make_params(kernel, metric, theta, prm);
+    if (st) return st;
+    if (rows == 0 || cols == 0) return EGS_SUCCESS;
+    EGS_HIP(hipSetDevice(ctx->device));
+    double *buf = nullptr;
+    const size_t nb1 = sizeof(double) * (size_t) n1, nb2 = sizeof(double) * (size_t) n2;
+    const size_t out_bytes = sizeof(double) * (size_t) rows * cols;
+    EGS_HIP(hipMalloc(&buf, out_bytes + 3 * nb1 + 3 * nb2));
+    double *d_out = buf;
+    double *dx1 = buf + (size_t) rows * cols, *dy1 = dx1 + n1, *dz1 = dy1 + n1;
+    double *dx2 = dz1 + n1, *dy2 = dx2 + n2, *dz2 = dy2 + n2;
+    hipStream_t s = ctx->stream;
+    hipMemcpyAsync(dx1, x1, nb1, hipMemcpyHostToDevice, s);
+    hipMemcpyAsync(dy1, y1, nb1, hipMemcpyHostToDevice, s);
+    hipMemcpyAsync(dx2, x2, nb2, hipMemcpyHostToDevice, s);
+    hipMemcpyAsync(dy2, y2, nb2, hipMemcpyHostToDevice, s);
+    if (three_d) {
+        hipMemcpyAsync(dz1, z1, nb1, hipMemcpyHostToDevice, s);
+        hipMemcpyAsync(dz2, z2, nb2, hipMemcpyHostToDevice, s);
+    }
+    egs::launch_covgen_rect(d_out, rows, cols, row_off, col_off, prm, dx1, dy1, three_d ? dz1 : nullptr, dx2, dy2,
+                            three_d ? dz2 : nullptr, s);
+    hipError_t e = hipMemcpyAsync(A, d_out, out_bytes, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    hipFree(buf);
+    if (e != hipSuccess) return fail(EGS_ERR_HIP, "covariance tile: %s", hipGetErrorString(e));
+    return EGS_SUCCESS;
+}
+
+int egs_problem_create(egs_context_t ctx, int kernel, int64_t n_loc, const double *x, const double *y,
+                       const double *z, int metric, int dts, egs_problem_t *problem) {
+    if (!ctx || !problem || !x || !y) return fail(EGS_ERR_INVALID_ARG, "null argument");
+    if (kernel != EGS_KERNEL_UNIVARIATE_MATERN_STATIONARY && kernel != EGS_KERNEL_BIVARIATE_MATERN_PARSIMONIOUS)
+        return fail(EGS_ERR_INVALID_ARG, "unknown kernel %d", kernel);
+    if (n_loc <= 0) return fail(EGS_ERR_INVALID_ARG, "n_loc must be positive");
+    if (dts <= 0) return fail(EGS_ERR_INVALID_ARG, "dense tile size must be positive");
+    if (metric != 0 && metric != 1) return fail(EGS_ERR_INVALID_ARG, "unknown distance metric %d", metric);
+    if (kernel == EGS_KERNEL_UNIVARIATE_MATERN_STATIONARY && z && metric == 1)
+        return fail(EGS_ERR_INVALID_ARG, "Great Circle (GC) distance is only valid for 2D!");
+    EGS_HIP(hipSetDevice(ctx->device));
+    auto *p = new egs_problem_s();
+    p->ctx = ctx;
+    p->kernel = kernel;
+    p->p = kernel == EGS_KERNEL_BIVARIATE_MATERN_PARSIMONIOUS ? 2 : 1;
+    p->metric = metric;
+    p->dim = (z && kernel == EGS_KERNEL_UNIVARIATE_MATERN_STATIONARY) ? 3 : 2;
+    p->dts = dts;
+    p->ts = std::min(((dts + egs::kBlk - 1) / egs::kBlk) * egs::kBlk, kMaxTileSize);
+    p->nsb = p->ts / egs::kBlk;
+    p->n_loc = n_loc;
+    p->n = n_loc * p->p;
+    p->nt = (int) ((p->n + p->ts - 1) / p->ts);
+    p->npad = (int64_t) p->nt * p->ts;
+    const int nt = p->nt;
+    for (int j = nt - 1; j >= 0; --j)
+        for (int i = j; i < nt; ++i) p->h_pairs.push_back(int2{i, j});
+
+    auto cleanup = [&](int code) {
+        free_problem_buffers(p);
+        delete p;
+        return code;
+    };
+    const size_t nb = sizeof(double) * (size_t) n_loc;
+    const size_t vb = sizeof(double) * (size_t) p->npad;
+    bool ok = hipMalloc(&p->d_x, nb) == hipSuccess && hipMalloc(&p->d_y, nb) == hipSuccess &&
+              (p->dim != 3 || hipMalloc(&p->d_z, nb) == hipSuccess) &&
+              hipMalloc(&p->d_off, sizeof(int64_t) * (size_t) nt * nt) == hipSuccess &&
+              hipMalloc(&p->d_store, (size_t) nt * nt) == hipSuccess &&
+              hipMalloc(&p->d_pairs, sizeof(int2) * p->h_pairs.size()) == hipSuccess &&
+              hipMalloc(&p->d_winv, sizeof(double) * (size_t) nt * p->nsb * egs::kBlk * egs::kBlk) == hipSuccess &&
+              hipMalloc(&p->d_obs, vb) == hipSuccess && hipMalloc(&p->d_work, vb) == hipSuccess &&
+              hipMalloc(&p->d_aux, vb) == hipSuccess &&
+              hipMalloc(&p->d_partials, sizeof(double) * 2 * (size_t) nt) == hipSuccess &&
+              hipMalloc(&p->d_info, sizeof(int)) == hipSuccess;
+    if (!ok) return cleanup(fail(EGS_ERR_OUT_OF_MEMORY, "cannot allocate the problem's device buffers"));
+    for (auto &e : p->ev)
+        if (hipEventCreate(&e) != hipSuccess) return cleanup(fail(EGS_ERR_HIP, "hipEventCreate failed"));
+    if (hipMemcpy(p->d_x, x, nb, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(p->d_y, y, nb, hipMemcpyHostToDevice) != hipSuccess ||
+        (p->dim == 3 && hipMemcpy(p->d_z, z, nb, hipMemcpyHostToDevice) != hipSuccess) ||
+        hipMemcpy(p->d_pairs, p->h_pairs.data(), sizeof(int2) * p->h_pairs.size(), hipMemcpyHostToDevice) !=
+            hipSuccess ||
+        hipMemset(p->d_obs, 0, vb) != hipSuccess)
+        return cleanup(fail(EGS_ERR_HIP, "copying the locations to the device failed"));
+    int st = layout_tiles(p);
+    if (st) return cleanup(st);
+    ctx->problems.push_back(p);
+    *problem = p;
+    return EGS_SUCCESS;
+}
+
+int egs_problem_destroy(egs_problem_t p) {
+    if (!p) return EGS_SUCCESS;
+    if (p->ctx) {
+        hipSetDevice(p->ctx->device);
+        hipStreamSynchronize(p->ctx->stream);
+        free_problem_buffers(p);
+    }
+    delete p;
+    return EGS_SUCCESS;
+}
+
+int egs_problem_device_bytes(egs_problem_t p, int64_t *bytes) {
+    int st = check_problem(p);
+    if (st) return st;
+    if (!bytes) return fail(EGS_ERR_INVALID_ARG, "bytes is null");
+    const int64_t vec = (int64_t) sizeof(double) * p->npad;
+    *bytes = (int64_t) p->arena_bytes + 3 * vec +
+             (int64_t) sizeof(double) * p->nt * p->nsb * egs::kBlk * egs::kBlk +
+             (int64_t) sizeof(double) * p->n_loc * (p->dim == 3 ? 3 : 2);
+    return EGS_SUCCESS;
+}
+
+int egs_problem_set_precision(egs_problem_t p, int mode, int band_fp64, int band_fp32) {
+    int st = check_problem(p);
+    if (st) return st;
+    if (mode != EGS_PRECISION_FP64 && mode != EGS_PRECISION_MIXED)
+        return fail(EGS_ERR_INVALID_ARG, "unknown precision mode %d", mode);
+    if (mode == EGS_PRECISION_MIXED)
+        return fail(EGS_ERR_UNSUPPORTED, "mixed precision is not available in this build");
+    if (band_fp64 < 0 || band_fp32 < band_fp64) return fail(EGS_ERR_INVALID_ARG, "bands must satisfy 0 <= fp64 <= fp32");
+    p->mode = mode;
+    p->band64 = band_fp64;
+    p->band32 = band_fp32;
+    return layout_tiles(p);
+}
+
+int egs_problem_set_observations(egs_problem_t p, const double *z, int64_t n) {
+    int st = check_problem(p);
+    if (st) return st;
+    if (!z || n != p->n) return fail(EGS_ERR_INVALID_ARG, "observations must have n_loc * P = %lld entries", (long long) p->n);
+    EGS_HIP(hipMemcpy(p->d_obs, z, sizeof(double) * (size_t) n, hipMemcpyHostToDevice));
+    p->has_obs = true;
+    return EGS_SUCCESS;
+}
+
+int egs_problem_get_observations(egs_problem_t p, double *z, int64_t n) {
+    int st = check_problem(p);
+    if (st) return st;
+    if (!z || n != p->n) return fail(EGS_ERR_INVALID_ARG, "observations have n_loc * P = %lld entries", (long long) p->n);
+    if (!p->has_obs) return fail(EGS_ERR_STATE, "the problem has no observations");
+    EGS_HIP(hipMemcpy(z, p->d_obs, sizeof(double) * (size_t) n, hipMemcpyDeviceToHost));
+    return EGS_SUCCESS;
+}
+
+int egs_covariance(egs_problem_t p, const double *theta) {
+    int st = check_problem(p);
+    if (st) return st;
+    st = run_covariance(p, theta);
+    if (st) return st;
+    EGS_HIP(hipStreamSynchronize(p->ctx->stream));
+    return EGS_SUCCESS;
+}
+
+int egs_potrf(egs_problem_t p) {
+    int st = check_problem(p);
+    if (st) return st;
+    st = run_potrf(p);
+    if (st) return st;
+    EGS_HIP(hipStreamSynchronize(p->ctx->stream));
+    st = read_info(p);
+    if (st) return st;
+    p->factored = true;
+    std::memcpy(p->factored_theta, p->cov_theta, sizeof(p->cov_theta));
+    return EGS_SUCCESS;
+}
+
+int egs_solve_logdet_quad(egs_problem_t p, double *logdet, double *quad) {
+    int st = check_problem(p);
+    if (st) return st;
+    if (!p->has_obs) return fail(EGS_ERR_STATE, "the problem has no observations");
+    hipStream_t s = p->ctx->stream;
+    egs::launch_copy_pad(p->d_work, p->d_obs, p->n, p->npad, s);
+    run_forward(p);
+    EGS_HIP(hipGetLastError());
+    EGS_HIP(hipStreamSynchronize(s));
+    double ld, q;
+    st = read_partials(p, ld, q);
+    if (st) return st;
+    if (logdet) *logdet = ld;
+    if (quad) *quad = q;
+    return EGS_SUCCESS;
+}
+
+int egs_loglik(egs_problem_t p, const double *theta, double *loglik, double *logdet, double *quad) {
+    int st = check_problem(p);
+    if (st) return st;
+    if (!loglik) return fail(EGS_ERR_INVALID_ARG, "loglik is null");
+    if (!p->has_obs) return fail(EGS_ERR_STATE, "the problem has no observations");
+    hipStream_t s = p->ctx->stream;
+    EGS_HIP(hipEventRecord(p->ev[0], s));
+    egs::launch_copy_pad(p->d_work, p->d_obs, p->n, p->npad, s);
+    st = run_covariance(p, theta);
+    if (st) return st;
+    EGS_HIP(hipEventRecord(p->ev[1], s));
+    st = run_potrf(p);
+    if (st) return st;
+    EGS_HIP(hipEventRecord(p->ev[2], s));
+    run_forward(p);
+    EGS_HIP(hipGetLastError());
+    EGS_HIP(hipEventRecord(p->ev[3], s));
+    EGS_HIP(hipStreamSynchronize(s));
+    st = read_info(p);
+    if (st) return st;
+    p->factored = true;
+    std::memcpy(p->factored_theta, theta, sizeof(double) * p->ntheta());
+    float ms = 0;
+    hipEventElapsedTime(&ms, p->ev[0], p->ev[1]);
+    p->t_cov = ms;
+    hipEventElapsedTime(&ms, p->ev[1], p->ev[2]);
+    p->t_fact = ms;
+    hipEventElapsedTime(&ms, p->ev[2], p->ev[3]);
+    p->t_solve = ms;
+    double ld, q;
+    st = read_partials(p, ld, q);
+    if (st) return st;
+    const double n = (double) p->n;
+    *loglik = -0.5 * q - 0.5 * ld - (double) (n / 2.0) * std::log(2.0 * kPi);
+    if (logdet) *logdet = ld;
+    if (quad) *quad = q;
+    return EGS_SUCCESS;
+}
+
+int egs_problem_profile(egs_problem_t p, int enable) {
+    int st = check_problem(p);
+    if (st) return st;
+    EGS_HIP(hipStreamSynchronize(p->ctx->stream));
+    prof_collect(p);
+    p->prof.on = enable != 0;
+    for (auto &k : p->prof.stats) k = KernelStat();
+    return EGS_SUCCESS;
+}
+
+int egs_problem_kernel_stats(egs_problem_t p, int kernel, int64_t *launches, double *total_ms, double *flops) {
+    int st = check_problem(p);
+    if (st) return st;
+    if (kernel < 0 || kernel >= EGS_PROFILE_KERNELS) return fail(EGS_ERR_INVALID_ARG, "unknown kernel id %d", kernel);
+    const KernelStat &k = p->prof.stats[kernel];
+    if (launches) *launches = k.launches;
+    if (total_ms) *total_ms = k.ms;
+    if (flops) *flops = k.flops;
+    return EGS_SUCCESS;
+}
+
+int egs_last_timings(egs_problem_t p, double *t_cov, double *t_fact, double *t_solve) {
+    int st = check_problem(p);
+    if (st) return st;
+    if (t_cov) *t_cov = p->t_cov;
+    if (t_fact) *t_fact = p->t_fact;
+    if (t_solve) *t_solve = p->t_solve;
+    return EGS_SUCCESS;
+}
+
+int egs_get_matrix(egs_problem_t p, double *out, int64_t ld) {
+    int st = check_problem(p);
+    if (st) return st;
+    if (!out || ld < p->n) return fail(EGS_ERR_INVALID_ARG, "output must be n x n with ld >= n");
+    EGS_HIP(hipStreamSynchronize(p->ctx->stream));
+    const int ts = p->ts;
+    std::vector<char> buf((size_t) ts * ts * 8);
+    for (int j = 0; j < p->nt; ++j) {
+        for (int i = j; i < p->nt; ++i) {
+            const size_t idx = (size_t) i * p->nt + j;
+            const bool f64 = p->h_store[idx] == egs::kStoreF64;
+            EGS_HIP(hipMemcpy(buf.data(), p->d_arena + p->h_off[idx], (size_t) ts * ts * (f64 ? 8 : 4),
+                              hipMemcpyDeviceToHost));
+            for (int c = 0; c < ts; ++c) {
+                const int64_t gc = (int64_t) j * ts + c;
+                if (gc >= p->n) break;
+                for (int r = 0; r < ts; ++r) {
+                    const int64_t gr = (int64_t) i * ts + r;
+                    if (gr >= p->n) break;
+                    if (gr < gc) continue;
+                    const size_t e = (size_t) r + (size_t) c * ts;
+                    out[gr + gc * ld] = f64 ? reinterpret_cast<double *>(buf.data())[e]
+                                            : (double) reinterpret_cast<float *>(buf.data())[e];
+                }
+            }
+        }
+    }
+    return EGS_SUCCESS;
+}
+
+int egs_generate_locations(int64_t n, int dim, double *x, double *y, double *z) {
+    // LocationGenerator.cpp:25-123
+    if (n <= 0 || !x || !y || (dim != 2 && dim != 3) || (dim == 3 && !z))
+        return fail(EGS_ERR_INVALID_ARG, "invalid location request");
+    const int root = dim == 3 ? (int) std::ceil(std::cbrt((double) n)) : (int) std::ceil(std::sqrt((double) n));
+    auto uni = []() { return ((double) std::rand() / (1.0 + RAND_MAX)) * 0.8 - 0.4; };
+    int64_t idx = 0;
+    for (int i = 0; i < root && idx < n; ++i) {
+        for (int j = 0; j < root && idx < n; ++j) {
+            if (dim == 3) {
+                for (int k = 0; k < root && idx < n; ++k) {
+                    x[idx] = ((i + 1) - 0.5 + uni()) / root;
+                    y[idx] = ((j + 1) - 0.5 + uni()) / root;
+                    z[idx] = ((k + 1) - 0.5 + uni()) / root;
+                    ++idx;
+                }
+            } else {
+                x[idx] = ((i + 1) - 0.5 + uni()) / root;
+                y[idx] = ((j + 1) - 0.5 + uni()) / root;
+                ++idx;
+            }
+        }
+    }
+    std::vector<uint64_t> code((size_t) n);
+    for (int64_t i = 0; i < n; ++i) {
+        const uint16_t qx = (uint16_t) (x[i] * 65535.0 + .5);
+        const uint16_t qy = (uint16_t) (y[i] * 65535.0 + .5);
+        const uint16_t qz = dim == 3 ? (uint16_t) (z[i] * 65535.0 + .5) : 0;
+        code[i] = (morton_spread(qz) << 2) + (morton_spread(qy) << 1) + morton_spread(qx);
+    }
+    std::sort(code.begin(), code.end());
+    for (int64_t i = 0; i < n; ++i) {
+        x[i] = (double) morton_compact(code[i]) / 65535.0;
+        y[i] = (double) morton_compact(code[i] >> 1) / 65535.0;
+        if (dim == 3) z[i] = (double) morton_compact(code[i] >> 2) / 65535.0;
+    }
+    return EGS_SUCCESS;
+}
+
+int egs_generate_observations(egs_problem_t p, const double *theta, int seed, double *z_out) {
+    int st = check_problem(p);
+    if (st) return st;
+    hipStream_t s = p->ctx->stream;
+    // iseed = {seed, seed, seed, 1} (LinearAlgebraMethods.cpp:348) as one 48-bit word
+    const uint64_t sd = (uint64_t) (uint32_t) seed;
+    const uint64_t seed48 = (((sd * 4096u + sd) * 4096u + sd) * 4096u + 1u) & ((1ull << 48) - 1);
+    egs::launch_dlarnv_normal(p->d_aux, p->n, seed48, s);
+    egs::launch_copy_pad(p->d_aux, p->d_aux, p->n, p->npad, s);
+    st = run_covariance(p, theta);
+    if (st) return st;
+    st = run_potrf(p);
+    if (st) return st;
+    egs::launch_trmv(p->map(), p->d_aux, p->d_work, s);
+    EGS_HIP(hipGetLastError());
+    EGS_HIP(hipStreamSynchronize(s));
+    st = read_info(p);
+    if (st) return st;
+    p->factored = true;
+    std::memcpy(p->factored_theta, theta, sizeof(double) * p->ntheta());
+    EGS_HIP(hipMemcpy(p->d_obs, p->d_work, sizeof(double) * (size_t) p->n, hipMemcpyDeviceToDevice));
+    p->has_obs = true;
+    if (z_out) EGS_HIP(hipMemcpy(z_out, p->d_obs, sizeof(double) * (size_t) p->n, hipMemcpyDeviceToHost));
+    return EGS_SUCCESS;
+}
+
+int egs_predict(egs_problem_t p, const double *theta, int64_t n_miss, const double *xm, const double *ym,
+                const double *zm, double *zmiss, const double *z_actual, double *mspe) {
+    int st = check_problem(p);
+    if (st) return st;
+    if (!theta || !xm || !ym || !zmiss || n_miss <= 0) return fail(EGS_ERR_INVALID_ARG, "invalid prediction request");
+    if (!p->has_obs) return fail(EGS_ERR_STATE, "the problem has no observations");
+    const bool three_d = p->dim == 3;
+    if (three_d && !zm) return fail(EGS_ERR_INVALID_ARG, "3D problem needs z coordinates of the targets");
+    hipStream_t s = p->ctx->stream;
+    const bool reuse = p->factored && std::memcmp(p->factored_theta, theta, sizeof(double) * p->ntheta()) == 0;
+    if (!reuse) {
+        st = run_covariance(p, theta);
+        if (st) return st;
+        st = run_potrf(p);
+        if (st) return st;
+    }
+    egs::launch_copy_pad(p->d_work, p->d_obs, p->n, p->npad, s);
+    run_forward(p);
+    run_backward(p);
+    egs::MaternParams prm;
+    st = make_params(p->kernel, p->metric, theta, prm);
+    if (st) return st;
+    const int64_t rows = n_miss * p->p;
+    double *buf = nullptr;
+    EGS_HIP(hipMalloc(&buf, sizeof(double) * (size_t) (rows + 3 * n_miss)));
+    double *d_out = buf, *dxm = buf + rows, *dym = dxm + n_miss, *dzm = dym + n_miss;
+    hipError_t e = hipMemcpyAsync(dxm, xm, sizeof(double) * n_miss, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(dym, ym, sizeof(double) * n_miss, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess && three_d) e = hipMemcpyAsync(dzm, zm, sizeof(double) * n_miss, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) {
+        egs::launch_cov_gemv(d_out, rows, p->n, prm, dxm, dym, three_d ? dzm : nullptr, p->d_x, p->d_y,
+                             three_d ? p->d_z : nullptr, p->d_work, s);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(zmiss, d_out, sizeof(double) * (size_t) rows, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    hipFree(buf);
+    if (e != hipSuccess) return fail(EGS_ERR_HIP, "prediction: %s", hipGetErrorString(e));
+    if (!reuse) {
+        st = read_info(p);
+        if (st) return st;
+        p->factored = true;
+        std::memcpy(p->factored_theta, theta, sizeof(double) * p->ntheta());
+    }
+    if (mspe) {
+        if (z_actual) {
+            double acc = 0.0;
+            for (int64_t i = 0; i < rows; ++i) acc += (z_actual[i] - zmiss[i]) * (z_actual[i] - zmiss[i]);
+            *mspe = acc / (double) rows;
+        } else {
+            *mspe = -1.0;
+        }
+    }
+    return EGS_SUCCESS;
+}
+
+}  // extern "C"

@@ -1,0 +1,54 @@
+// Shared device/host definitions for the gfx950 kernels of exageostatcpp_amd.
+//
+// Matrix storage: the covariance matrix of dimension n (= locations x variables)
+// is held as lower tiles of ts x ts elements (ts = the dense tile size rounded up
+// to a multiple of 64).  Each tile is column-major with leading dimension ts and
+// owns its own block of one HBM arena, so every tile can carry its own storage
+// precision.  The matrix is padded to npad = nt * ts with the identity, which
+// keeps it SPD, leaves log|Sigma| unchanged and lets every kernel work on full
+// 64-row blocks without bounds checks.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace egs {
+
+constexpr int kBlk = 64;        // sub-block edge used by every factorization kernel
+constexpr int kThreads = 256;   // 4 waves of 64 lanes per workgroup
+
+enum StoragePrecision : uint8_t { kStoreF64 = 0, kStoreF32 = 1 };
+enum ComputeClass : uint8_t { kComputeF64 = 0, kComputeF32 = 1, kComputeBF16 = 2 };
+
+// Location of every lower tile in the arena.  off[i * nt + j] is the byte offset
+// of tile (i, j), i >= j; store[i * nt + j] its storage precision.
+struct TileMap {
+    char *base;
+    const int64_t *off;
+    const uint8_t *store;
+    int nt;
+    int ts;
+};
+
+__device__ __forceinline__ double *tile_f64(const TileMap &m, int i, int j) {
+    return reinterpret_cast<double *>(m.base + m.off[i * m.nt + j]);
+}
+__device__ __forceinline__ float *tile_f32(const TileMap &m, int i, int j) {
+    return reinterpret_cast<float *>(m.base + m.off[i * m.nt + j]);
+}
+
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+// D[i][j] += sum_k X[i][k] Y[k][j] on a 16x16x4 tile.  Lane l supplies X[l%16][l/16]
+// and Y[l/16][l%16].  Result element v (0..3) of lane l is D[l/16 + 4*v][l%16] for
+// the FP64 instruction and D[4*(l/16) + v][l%16] for the FP32 one (both measured
+// on gfx950).
+__device__ __forceinline__ f64x4 mfma_f64(double x, double y, f64x4 acc) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(x, y, acc, 0, 0, 0);
+}
+__device__ __forceinline__ f32x4 mfma_f32(float x, float y, f32x4 acc) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(x, y, acc, 0, 0, 0);
+}
+
+}  // namespace egs

@@ -1,0 +1,57 @@
+// Host-side launchers of the gfx950 kernels.  All launches are asynchronous on
+// the given stream; shapes are validated by the caller (egs_capi.cpp) before
+// any launch.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "egs_common.hpp"
+#include "egs_matern.hpp"
+
+namespace egs {
+
+// ---- covariance generation (egs_covgen.hip)
+void launch_covgen_tiles(const TileMap &map, const int2 *pairs, int npairs, const MaternParams &prm,
+                         const double *x, const double *y, const double *z, int64_t n, hipStream_t s);
+void launch_covgen_rect(double *out, int rows, int cols, int64_t row_off, int64_t col_off, const MaternParams &prm,
+                        const double *x1, const double *y1, const double *z1, const double *x2, const double *y2,
+                        const double *z2, hipStream_t s);
+
+// ---- tiled Cholesky (egs_chol.hip)
+// Factor 64 x 64 diagonal block jb of diagonal tile k (after its left-looking
+// update) and write its inverse to winv_k + jb * 4096.  The first failing
+// global column (1-based) is recorded in *info.
+void launch_potrf_block(const TileMap &map, int k, int jb, double *winv_k, int *info, int64_t n, hipStream_t s);
+// Triangular solve X L_kk^T = B for row blocks [g0, g0 + count) of tile column k
+// (global 64-row block g lives in tile g / nsb at row (g % nsb) * 64), column
+// blocks [jb0, jb1).
+void launch_trsm_rows(const TileMap &map, int k, int64_t g0, int count, int jb0, int jb1, const double *winv_k,
+                      hipStream_t s);
+// Trailing update C_ij -= L_ik L_jk^T for the first npairs lower tiles of the
+// update order (all (i, j) with i >= j > k).
+void launch_update(const TileMap &map, const int2 *pairs, int npairs, int k, hipStream_t s);
+
+// ---- triangular solves and reductions (egs_solve.hip)
+// Forward solve of tile row k: y_k = L_kk^{-1} z_k in place, with per-tile
+// partials[2k] = sum log L_ii (L_ii > 0), partials[2k+1] = y_k . y_k.
+void launch_trsv_diag(const TileMap &map, int k, const double *winv_k, double *z, double *partials, hipStream_t s);
+// z_i -= L_ik y_k for every tile row i > k.
+void launch_trsv_panel(const TileMap &map, int k, double *z, hipStream_t s);
+// Backward solve with L^T of tile row k (after rows > k are solved):
+// partial sums of L_ik^T y_i into scratch, then the diagonal solve.
+void launch_trsv_t_partial(const TileMap &map, int k, const double *z, double *scratch, hipStream_t s);
+void launch_trsv_t_diag(const TileMap &map, int k, const double *winv_k, double *z, const double *scratch,
+                        hipStream_t s);
+// out = L e (lower-triangular product over the tiles).
+void launch_trmv(const TileMap &map, const double *e, double *out, hipStream_t s);
+// out[r] = sum_c Sigma(l1_r, l2_c) x[c] for r < rows, c < cols (kriging mean).
+void launch_cov_gemv(double *out, int64_t rows, int64_t cols, const MaternParams &prm, const double *x1,
+                     const double *y1, const double *z1, const double *x2, const double *y2, const double *z2,
+                     const double *xv, hipStream_t s);
+// e = LAPACK dlarnv(3, iseed) N(0,1) values (48-bit LCG stream, Box-Muller).
+void launch_dlarnv_normal(double *e, int64_t n, uint64_t seed48, hipStream_t s);
+// dst = src (n doubles), padding [n, npad) with zeros.
+void launch_copy_pad(double *dst, const double *src, int64_t n, int64_t npad, hipStream_t s);
+
+}  // namespace egs

@@ -1,0 +1,305 @@
+// Triangular solves, log-determinant / quadratic-form partials, kriging product
+// and the synthetic-data helpers on gfx950.
+//
+// The forward solve replaces CHAMELEON_dtrsm_Tile(Left, Lower, NoTrans) on Z
+// (ChameleonImplementation.cpp:216-217, 337-347) and fuses the per-tile
+// reductions the reference runs as dmdet (dmdet-codelet.cpp:70-80) and ddotp
+// (ddotp-codelet.cpp:62-72) tasks: the workgroup that solves diagonal tile k
+// also writes sum(log L_ii) over L_ii > 0 and y_k . y_k for that tile.  The host
+// sums the per-tile partials in tile order, so results do not depend on
+// scheduling.  All reductions inside a workgroup use a fixed order.
+#include "egs_common.hpp"
+#include "egs_kernels.hpp"
+#include "egs_matern.hpp"
+
+namespace egs {
+
+constexpr int kMaxTs = 2048;
+
+// Fixed-order sum of one value per thread over the workgroup.
+__device__ double wg_sum(double v, double *red) {
+    const int t = threadIdx.x;
+    red[t] = v;
+    __syncthreads();
+    for (int s = kThreads / 2; s > 0; s >>= 1) {
+        if (t < s) red[t] += red[t + s];
+        __syncthreads();
+    }
+    const double r = red[0];
+    __syncthreads();
+    return r;
+}
+
+// ------------------------------------------------------------------------
+// Forward solve of diagonal tile k, with the log-det and quad partials.
+// ------------------------------------------------------------------------
+__global__ void __launch_bounds__(kThreads) trsv_diag_kernel(TileMap map, int k, const double *winv_k, double *z,
+                                                             double *partials) {
+    __shared__ double sy[kMaxTs];
+    __shared__ double sp[4][kBlk];
+    __shared__ double st[kBlk];
+    __shared__ double red[kThreads];
+    const int ts = map.ts;
+    const int nsb = ts / kBlk;
+    const int t = threadIdx.x;
+    const int r = t & 63, g = t >> 6;
+    const double *L = tile_f64(map, k, k);
+    double *zk = z + (int64_t) k * ts;
+    for (int e = t; e < ts; e += kThreads) sy[e] = zk[e];
+    __syncthreads();
+    for (int jb = 0; jb < nsb; ++jb) {
+        // t = z_jb - L[jb, 0:jb] y[0:jb]
+        const int row = jb * kBlk + r;
+        double s = 0.0;
+        for (int c = g; c < jb * kBlk; c += 4) s += L[row + (int64_t) c * ts] * sy[c];
+        sp[g][r] = s;
+        __syncthreads();
+        if (t < kBlk) st[t] = sy[jb * kBlk + t] - (((sp[0][t] + sp[1][t]) + sp[2][t]) + sp[3][t]);
+        __syncthreads();
+        // y_jb = W_jb t  (W lower triangular)
+        const double *W = winv_k + (int64_t) jb * kBlk * kBlk;
+        double u = 0.0;
+        for (int c = g; c <= r; c += 4) u += W[r + c * kBlk] * st[c];
+        sp[g][r] = u;
+        __syncthreads();
+        if (t < kBlk) sy[jb * kBlk + t] = ((sp[0][t] + sp[1][t]) + sp[2][t]) + sp[3][t];
+        __syncthreads();
+    }
+    double ld = 0.0, q = 0.0;
+    for (int e = t; e < ts; e += kThreads) {
+        const double d = L[e + (int64_t) e * ts];
+        if (d > 0.0) ld += log(d);
+        q += sy[e] * sy[e];
+        zk[e] = sy[e];
+    }
+    ld = wg_sum(ld, red);
+    q = wg_sum(q, red);
+    if (t == 0) {
+        partials[2 * k] = ld;
+        partials[2 * k + 1] = q;
+    }
+}
+
+// z_i -= L_ik y_k for the tile rows below k, one workgroup per 64 rows.
+__global__ void __launch_bounds__(kThreads) trsv_panel_kernel(TileMap map, int k, double *z) {
+    __shared__ double sy[kMaxTs];
+    __shared__ double sp[4][kBlk];
+    const int ts = map.ts;
+    const int nsb = ts / kBlk;
+    const int t = threadIdx.x;
+    const int r = t & 63, g = t >> 6;
+    const int i = k + 1 + (int) (blockIdx.x / nsb);
+    const int rb = (int) (blockIdx.x % nsb);
+    const double *yk = z + (int64_t) k * ts;
+    for (int e = t; e < ts; e += kThreads) sy[e] = yk[e];
+    __syncthreads();
+    const double *A = tile_f64(map, i, k) + rb * kBlk;
+    double s = 0.0;
+    for (int c = g; c < ts; c += 4) s += A[r + (int64_t) c * ts] * sy[c];
+    sp[g][r] = s;
+    __syncthreads();
+    if (t < kBlk) {
+        double *zi = z + (int64_t) i * ts + rb * kBlk;
+        zi[t] -= ((sp[0][t] + sp[1][t]) + sp[2][t]) + sp[3][t];
+    }
+}
+
+// scratch[(i - k - 1) * ts + c] = sum_r L_ik[r][c] y_i[r] for every tile row i > k.
+__global__ void __launch_bounds__(kThreads) trsv_t_partial_kernel(TileMap map, int k, const double *z,
+                                                                  double *scratch) {
+    __shared__ double sy[kMaxTs];
+    const int ts = map.ts;
+    const int nsb = ts / kBlk;
+    const int i = k + 1 + (int) (blockIdx.x / nsb);
+    const int cb = (int) (blockIdx.x % nsb);
+    const int t = threadIdx.x;
+    const int lane = t & 63, w = t >> 6;
+    const double *yi = z + (int64_t) i * ts;
+    for (int e = t; e < ts; e += kThreads) sy[e] = yi[e];
+    __syncthreads();
+    const double *A = tile_f64(map, i, k);
+    // wave w reduces columns cb*64 + w*16 .. +15, lanes stride the rows
+    for (int cc = 0; cc < 16; ++cc) {
+        const int c = cb * kBlk + w * 16 + cc;
+        double s = 0.0;
+        for (int r0 = lane; r0 < ts; r0 += 64) s += A[r0 + (int64_t) c * ts] * sy[r0];
+        // fixed-order butterfly over the 64 lanes
+        for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+        if (lane == 0) scratch[(int64_t) (i - k - 1) * ts + c] = s;
+    }
+}
+
+// Backward solve of diagonal tile k: y_k = L_kk^{-T} (z_k - sum_i partials_i).
+__global__ void __launch_bounds__(kThreads) trsv_t_diag_kernel(TileMap map, int k, const double *winv_k, double *z,
+                                                               const double *scratch) {
+    __shared__ double sy[kMaxTs];
+    __shared__ double sp[4][kBlk];
+    __shared__ double st[kBlk];
+    const int ts = map.ts;
+    const int nsb = ts / kBlk;
+    const int nt = map.nt;
+    const int t = threadIdx.x;
+    const int c = t & 63, g = t >> 6;
+    const double *L = tile_f64(map, k, k);
+    double *zk = z + (int64_t) k * ts;
+    for (int e = t; e < ts; e += kThreads) {
+        double v = zk[e];
+        for (int i = k + 1; i < nt; ++i) v -= scratch[(int64_t) (i - k - 1) * ts + e];
+        sy[e] = v;
+    }
+    __syncthreads();
+    for (int jb = nsb - 1; jb >= 0; --jb) {
+        // t[c] = sy[jb*64 + c] - sum_{r >= (jb+1)*64} L[r][jb*64 + c] y[r]
+        const int col = jb * kBlk + c;
+        double s = 0.0;
+        for (int rr = (jb + 1) * kBlk + g; rr < ts; rr += 4) s += L[rr + (int64_t) col * ts] * sy[rr];
+        sp[g][c] = s;
+        __syncthreads();
+        if (t < kBlk) st[t] = sy[jb * kBlk + t] - (((sp[0][t] + sp[1][t]) + sp[2][t]) + sp[3][t]);
+        __syncthreads();
+        // y[c] = sum_{r >= c} W[r][c] t[r]
+        const double *W = winv_k + (int64_t) jb * kBlk * kBlk;
+        double u = 0.0;
+        for (int rr = c + g; rr < kBlk; rr += 4) {
+            if (rr >= c) u += W[rr + c * kBlk] * st[rr];
+        }
+        sp[g][c] = u;
+        __syncthreads();
+        if (t < kBlk) sy[jb * kBlk + t] = ((sp[0][t] + sp[1][t]) + sp[2][t]) + sp[3][t];
+        __syncthreads();
+    }
+    for (int e = t; e < ts; e += kThreads) zk[e] = sy[e];
+}
+
+// out = L e, one workgroup per 64 rows.
+__global__ void __launch_bounds__(kThreads) trmv_kernel(TileMap map, const double *e, double *out) {
+    __shared__ double sp[4][kBlk];
+    const int ts = map.ts;
+    const int nsb = ts / kBlk;
+    const int64_t gblk = blockIdx.x;
+    const int i = (int) (gblk / nsb);
+    const int rb = (int) (gblk % nsb);
+    const int t = threadIdx.x;
+    const int r = t & 63, g = t >> 6;
+    double s = 0.0;
+    for (int k = 0; k <= i; ++k) {
+        const double *A = tile_f64(map, i, k) + rb * kBlk;
+        const double *ek = e + (int64_t) k * ts;
+        const int cend = (k < i) ? ts : (rb + 1) * kBlk;
+        for (int c = g; c < cend; c += 4) {
+            const int row_in_tile = rb * kBlk + r;
+            if (k == i && c > row_in_tile) continue;
+            s += A[r + (int64_t) c * ts] * ek[c];
+        }
+    }
+    sp[g][r] = s;
+    __syncthreads();
+    if (t < kBlk) out[gblk * kBlk + t] = ((sp[0][t] + sp[1][t]) + sp[2][t]) + sp[3][t];
+}
+
+// Kriging mean: out[r] = sum_c Sigma(l1_r, l2_c) x[c].
+struct CovGemvArgs {
+    double *out;
+    int64_t rows, cols;
+    MaternParams prm;
+    Locs l1, l2;
+    const double *x;
+    int three_d;
+};
+
+__global__ void __launch_bounds__(kThreads) cov_gemv_kernel(CovGemvArgs a) {
+    __shared__ double red[kThreads];
+    const int64_t r = blockIdx.x;
+    double s = 0.0;
+    for (int64_t c = threadIdx.x; c < a.cols; c += kThreads)
+        s += covariance_entry(a.prm, a.l1, a.l2, r, c, a.three_d != 0) * a.x[c];
+    s = wg_sum(s, red);
+    if (threadIdx.x == 0) a.out[r] = s;
+}
+
+// dlarnv(idist = 3): value m uses the uniforms number 2m and 2m+1 of the
+// stream u_j = (s0 * A^(j+1) mod 2^48) / 2^48 with A = 33952834046453.
+__device__ __forceinline__ uint64_t lcg_pow(uint64_t e) {
+    const uint64_t mask = (1ull << 48) - 1;
+    uint64_t base = 33952834046453ull, r = 1;
+    while (e) {
+        if (e & 1) r = (r * base) & mask;
+        base = (base * base) & mask;
+        e >>= 1;
+    }
+    return r;
+}
+
+__global__ void __launch_bounds__(kThreads) dlarnv_normal_kernel(double *e, int64_t n, uint64_t seed48) {
+    const int64_t m = (int64_t) blockIdx.x * kThreads + threadIdx.x;
+    if (m >= n) return;
+    const uint64_t mask = (1ull << 48) - 1;
+    const uint64_t s1 = (seed48 * lcg_pow((uint64_t) (2 * m + 1))) & mask;
+    const uint64_t s2 = (s1 * 33952834046453ull) & mask;
+    const double u1 = (double) s1 * 0x1p-48;
+    const double u2 = (double) s2 * 0x1p-48;
+    e[m] = sqrt(-2.0 * log(u1)) * cos(6.2831853071795864769252867663 * u2);
+}
+
+__global__ void __launch_bounds__(kThreads) copy_pad_kernel(double *dst, const double *src, int64_t n, int64_t npad) {
+    const int64_t i = (int64_t) blockIdx.x * kThreads + threadIdx.x;
+    if (i < npad) dst[i] = (i < n) ? src[i] : 0.0;
+}
+
+void launch_trsv_diag(const TileMap &map, int k, const double *winv_k, double *z, double *partials, hipStream_t s) {
+    hipLaunchKernelGGL(trsv_diag_kernel, dim3(1), dim3(kThreads), 0, s, map, k, winv_k, z, partials);
+}
+
+void launch_trsv_panel(const TileMap &map, int k, double *z, hipStream_t s) {
+    const int nsb = map.ts / kBlk;
+    const int rows_below = map.nt - k - 1;
+    if (rows_below <= 0) return;
+    hipLaunchKernelGGL(trsv_panel_kernel, dim3(rows_below * nsb), dim3(kThreads), 0, s, map, k, z);
+}
+
+void launch_trsv_t_partial(const TileMap &map, int k, const double *z, double *scratch, hipStream_t s) {
+    const int nsb = map.ts / kBlk;
+    const int rows_below = map.nt - k - 1;
+    if (rows_below <= 0) return;
+    hipLaunchKernelGGL(trsv_t_partial_kernel, dim3(rows_below * nsb), dim3(kThreads), 0, s, map, k, z, scratch);
+}
+
+void launch_trsv_t_diag(const TileMap &map, int k, const double *winv_k, double *z, const double *scratch,
+                        hipStream_t s) {
+    hipLaunchKernelGGL(trsv_t_diag_kernel, dim3(1), dim3(kThreads), 0, s, map, k, winv_k, z, scratch);
+}
+
+void launch_trmv(const TileMap &map, const double *e, double *out, hipStream_t s) {
+    const int nsb = map.ts / kBlk;
+    hipLaunchKernelGGL(trmv_kernel, dim3(map.nt * nsb), dim3(kThreads), 0, s, map, e, out);
+}
+
+void launch_cov_gemv(double *out, int64_t rows, int64_t cols, const MaternParams &prm, const double *x1,
+                     const double *y1, const double *z1, const double *x2, const double *y2, const double *z2,
+                     const double *xv, hipStream_t s) {
+    if (rows <= 0) return;
+    CovGemvArgs a;
+    a.out = out;
+    a.rows = rows;
+    a.cols = cols;
+    a.prm = prm;
+    a.l1 = Locs{x1, y1, z1};
+    a.l2 = Locs{x2, y2, z2};
+    a.x = xv;
+    a.three_d = (z1 != nullptr && z2 != nullptr);
+    hipLaunchKernelGGL(cov_gemv_kernel, dim3((unsigned) rows), dim3(kThreads), 0, s, a);
+}
+
+void launch_dlarnv_normal(double *e, int64_t n, uint64_t seed48, hipStream_t s) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(dlarnv_normal_kernel, dim3((unsigned) ((n + kThreads - 1) / kThreads)), dim3(kThreads), 0, s,
+                       e, n, seed48);
+}
+
+void launch_copy_pad(double *dst, const double *src, int64_t n, int64_t npad, hipStream_t s) {
+    if (npad <= 0) return;
+    hipLaunchKernelGGL(copy_pad_kernel, dim3((unsigned) ((npad + kThreads - 1) / kThreads)), dim3(kThreads), 0, s,
+                       dst, src, n, npad);
+}
+
+}  // namespace egs

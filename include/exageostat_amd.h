@@ -1,0 +1,165 @@
+/*
+ * exageostat_amd.h -- C ABI of the MI355X (gfx950) maximum-likelihood path.
+ *
+ * This is the drop-in boundary below ExaGeoStatCPP's dense linear-algebra
+ * solver: every entry point names the reference interface it replaces
+ * (paths relative to the ExaGeoStatCPP source tree).  Plain pointers and
+ * sizes only; host arrays are owned by the caller and copied in; every device
+ * buffer is owned by the library.  Functions return EGS_SUCCESS (0) or an
+ * error code, and egs_last_error() gives the message of the last failure on
+ * the calling thread.  A context is used by one host thread at a time; calls
+ * are blocking unless stated otherwise.
+ */
+#ifndef EXAGEOSTAT_AMD_H
+#define EXAGEOSTAT_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define EGS_SUCCESS 0
+#define EGS_ERR_INVALID_ARG 1
+#define EGS_ERR_HIP 2
+#define EGS_ERR_NOT_SPD 3
+#define EGS_ERR_OUT_OF_MEMORY 4
+#define EGS_ERR_UNSUPPORTED 5
+#define EGS_ERR_NO_DEVICE 6
+#define EGS_ERR_STATE 7
+
+/* Covariance kernels (the PluginRegistry names of
+ * src/kernels/concrete/UnivariateMaternStationary.cpp:35-36 and
+ * src/kernels/concrete/BivariateMaternParsimonious.cpp:35-36). */
+#define EGS_KERNEL_UNIVARIATE_MATERN_STATIONARY 0
+#define EGS_KERNEL_BIVARIATE_MATERN_PARSIMONIOUS 1
+
+/* Distance metrics (DistanceCalculationHelpers.cpp:38-67): 0 Euclidean,
+ * 1 great-circle (haversine, km, x = latitude, y = longitude in degrees). */
+#define EGS_METRIC_EUCLIDEAN 0
+#define EGS_METRIC_GREAT_CIRCLE 1
+
+/* Precision plans for the factorization. */
+#define EGS_PRECISION_FP64 0
+#define EGS_PRECISION_MIXED 1
+
+typedef struct egs_context_s *egs_context_t;
+typedef struct egs_problem_s *egs_problem_t;
+
+/* Message of the last failure on this thread ("" if none). */
+const char *egs_last_error(void);
+/* Library version as 10000 * major + 100 * minor + patch. */
+int egs_version(void);
+
+/* ---- hardware ------------------------------------------------------------
+ * Replaces ExaGeoStatHardware::InitHardware / the destructor
+ * (src/hardware/ExaGeoStatHardware.cpp:126-212): selects HIP device `device`
+ * and creates the library's streams. */
+int egs_init(int device, egs_context_t *ctx);
+int egs_finalize(egs_context_t ctx);
+/* Waits for all work of the context. */
+int egs_synchronize(egs_context_t ctx);
+
+/* ---- kernel plugin ---------------------------------------------------------
+ * Kernel<T>::GenerateCovarianceMatrix (inst/include/kernels/Kernel.hpp:93-97,
+ * e.g. src/kernels/concrete/UnivariateMaternStationary.cpp:39-66): writes the
+ * rows x cols column-major block (ld = rows) of Sigma(loc1, loc2) whose first
+ * element is (row_off, col_off), evaluated on the GPU.  loc*_z may be NULL
+ * (2D).  theta has 3 (univariate) or 6 (bivariate) entries. */
+int egs_covariance_tile(egs_context_t ctx, int kernel, double *A, int rows, int cols, int row_off, int col_off,
+                        const double *x1, const double *y1, const double *z1, int64_t n1,
+                        const double *x2, const double *y2, const double *z2, int64_t n2,
+                        const double *theta, int metric);
+
+/* ---- problems (a set of locations and its device-resident Sigma) ---------
+ * Replaces LinearAlgebraMethods<T>::InitiateDescriptors
+ * (src/linear-algebra-solvers/LinearAlgebraMethods.cpp:35-118): the covariance
+ * of n_loc locations (dimension n_loc * P with P = 1 or 2 variables) in lower
+ * tiles of `dts` (rounded up to a multiple of 64 on the device).
+ * z may be NULL for 2D.  metric is used for every covariance of the problem. */
+int egs_problem_create(egs_context_t ctx, int kernel, int64_t n_loc, const double *x, const double *y,
+                       const double *z, int metric, int dts, egs_problem_t *problem);
+int egs_problem_destroy(egs_problem_t problem);
+/* Device bytes the problem holds. */
+int egs_problem_device_bytes(egs_problem_t problem, int64_t *bytes);
+
+/* Precision plan: EGS_PRECISION_FP64, or EGS_PRECISION_MIXED with tiles whose
+ * tile distance |i - j| <= band_fp64 in FP64, <= band_fp32 in FP32 and the
+ * rest in BF16 products accumulated in FP32. */
+int egs_problem_set_precision(egs_problem_t problem, int mode, int band_fp64, int band_fp32);
+
+/* Observations Z (length n_loc * P): the measurements descriptor Z and its
+ * restore copy Z_COPY of ChameleonImplementation.cpp:113-125. */
+int egs_problem_set_observations(egs_problem_t problem, const double *z, int64_t n);
+int egs_problem_get_observations(egs_problem_t problem, double *z, int64_t n);
+
+/* ---- the likelihood ------------------------------------------------------
+ * ChameleonImplementation<T>::ExaGeoStatMLETile
+ * (src/linear-algebra-solvers/concrete/chameleon/ChameleonImplementation.cpp:35-327):
+ * Sigma(theta) -> Cholesky -> log|Sigma| -> L^{-1} z -> z' z ->
+ * loglik = -0.5 z'Sigma^{-1}z - 0.5 log|Sigma| - (n/2) log(2 pi).
+ * logdet and quad may be NULL.  Returns EGS_ERR_NOT_SPD when Sigma(theta) is
+ * not positive definite (ChameleonDense.cpp:29-32). */
+int egs_loglik(egs_problem_t problem, const double *theta, double *loglik, double *logdet, double *quad);
+
+/* The phases of egs_loglik, for callers that drive them one by one:
+ *   RuntimeFunctions<T>::CovarianceMatrix (src/runtime/starpu/StarPuFunctions.cpp:24-45),
+ *   ExaGeoStatPotrfTile (ChameleonDense.cpp:25-37),
+ *   ExaGeoStatMeasureDetTileAsync + ExaGeoStatTrsmTile + ExaGeoStatDoubleDotProduct
+ *   (StarPuFunctions.cpp:119-135, 189-204; ChameleonImplementation.cpp:337-347). */
+int egs_covariance(egs_problem_t problem, const double *theta);
+int egs_potrf(egs_problem_t problem);
+int egs_solve_logdet_quad(egs_problem_t problem, double *logdet, double *quad);
+
+/* Copies the lower tiles of the current device matrix (Sigma after
+ * egs_covariance, L after egs_potrf) into a column-major n x n host array
+ * (ExaGeoStatDesc2Lap, LinearAlgebraMethods.cpp:1373-1380).  The strictly
+ * upper part of `out` is left untouched. */
+int egs_get_matrix(egs_problem_t problem, double *out, int64_t ld);
+
+/* Phase times of the last egs_loglik, milliseconds: covariance, factorization,
+ * solve (log-det, triangular solve and dot product). */
+int egs_last_timings(egs_problem_t problem, double *t_covariance, double *t_factor, double *t_solve);
+
+/* ---- measurement -----------------------------------------------------------
+ * The reference times phases with START_TIMING/STOP_TIMING
+ * (inst/include/utilities/Logger.hpp:153-160; ChameleonImplementation.cpp:129-219).
+ * egs_problem_profile(p, 1) resets the counters and brackets every launch of the
+ * kernels below with HIP events on the library stream; egs_problem_kernel_stats
+ * returns, per kernel, the number of timed launches, their summed duration and
+ * their summed algorithmic flops (FP64 real rows only, padding excluded). */
+#define EGS_PROFILE_UPDATE 0        /* trailing SYRK/GEMM update, one launch per step */
+#define EGS_PROFILE_COVARIANCE 1    /* covariance generation of all lower tiles */
+#define EGS_PROFILE_PANEL_TRSM 2    /* panel triangular solve, one launch per step */
+#define EGS_PROFILE_DIAGONAL 3      /* diagonal-tile factorization (all its launches) */
+#define EGS_PROFILE_SOLVE 4         /* forward solve with log-det and dot product */
+#define EGS_PROFILE_KERNELS 5
+int egs_problem_profile(egs_problem_t problem, int enable);
+int egs_problem_kernel_stats(egs_problem_t problem, int kernel, int64_t *launches, double *total_ms, double *flops);
+
+/* ---- synthetic data ------------------------------------------------------
+ * LocationGenerator<T>::GenerateLocations (src/data-generators/LocationGenerator.cpp:25-123):
+ * jittered grid from the C library's rand() (seed it with srand first, as
+ * ExaGeoStat.cpp:76 does), then Morton order on a 16-bit grid.
+ * dim: 2 or 3 (z must be non-NULL for 3). */
+int egs_generate_locations(int64_t n, int dim, double *x, double *y, double *z);
+
+/* LinearAlgebraMethods<T>::GenerateObservationsVector
+ * (LinearAlgebraMethods.cpp:333-449): z = L(theta) e with e = dlarnv(3,
+ * {seed, seed, seed, 1}); the result is written to z_out (length n_loc * P, may
+ * be NULL) and becomes the problem's observations. */
+int egs_generate_observations(egs_problem_t problem, const double *theta, int seed, double *z_out);
+
+/* ---- kriging ---------------------------------------------------------------
+ * LinearAlgebraMethods<T>::ExaGeoStatMLEPredictTile (LinearAlgebraMethods.cpp:451-605):
+ * zmiss = Sigma(miss, obs) Sigma(obs, obs)^{-1} zobs for the problem's
+ * locations and observations.  The factor of Sigma(obs, obs) at theta is
+ * reused when the problem already holds it.  z_actual may be NULL; otherwise
+ * *mspe receives mean((z_actual - zmiss)^2) (dmse-codelet.cpp:40-73), else -1. */
+int egs_predict(egs_problem_t problem, const double *theta, int64_t n_miss, const double *xm, const double *ym,
+                const double *zm, double *zmiss, const double *z_actual, double *mspe);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,211 @@
+"""GPU parity of the HIP path against the CPU oracle, through the C ABI.
+
+Sizes keep the oracle within seconds; tolerances are written per test.  The
+reference's fixtures (tests/golden/reference_fixtures.json) are checked on the
+GPU path as well.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import exageostatcpp_amd as egs
+from oracle import orc
+
+pytestmark = pytest.mark.gpu
+
+
+def log(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = egs.Context(0)
+    yield c
+    c.close()
+
+
+def _rng_locs(n, seed):
+    rng = np.random.default_rng(seed)
+    return rng.random(n), rng.random(n)
+
+
+# ------------------------------------------------------------------ covariance
+@pytest.mark.parametrize("theta", [[1.0, 0.1, 0.5], [0.7, 0.05, 1.0], [1.3, 0.2, 1.234], [2.0, 0.08, 2.5],
+                                   [1.0, 0.3, 0.3], [1.1, 0.02, 4.2]])
+def test_covariance_tile_univariate(ctx, theta):
+    log("covariance tile univariate theta=%s" % theta)
+    x, y = _rng_locs(300, 1)
+    G = ctx.covariance_tile(0, 70, 90, 13, 201, (x, y), (x, y), theta)
+    O = orc.covariance_tile(0, 70, 90, 13, 201, (x, y), (x, y), theta)
+    assert np.max(np.abs(G - O) / np.abs(O)) < 1e-13
+
+
+def test_covariance_tile_bivariate_3d_gcd(ctx):
+    log("covariance tile bivariate / 3D / great-circle")
+    x, y = _rng_locs(100, 2)
+    th = [1.0, 1.5, 0.1, 0.5, 1.0, 0.5]
+    G = ctx.covariance_tile(1, 40, 60, 20, 100, (x, y), (x, y), th)
+    O = orc.covariance_tile(1, 40, 60, 20, 100, (x, y), (x, y), th)
+    assert np.max(np.abs(G - O)) < 1e-13
+    z = np.random.default_rng(3).random(100)
+    G = ctx.covariance_tile(0, 50, 50, 0, 50, (x, y, z), (x, y, z), [1.0, 0.1, 1.7])
+    O = orc.covariance_tile(0, 50, 50, 0, 50, (x, y, z), (x, y, z), [1.0, 0.1, 1.7])
+    assert np.max(np.abs(G - O) / np.abs(O)) < 1e-13
+    lat, lon = 180 * x - 90, 360 * y - 180
+    G = ctx.covariance_tile(0, 60, 60, 0, 40, (lat, lon), (lat, lon), [1.0, 637.1, 0.5], metric=1)
+    O = orc.covariance_tile(0, 60, 60, 0, 40, (lat, lon), (lat, lon), [1.0, 637.1, 0.5], metric=1)
+    assert np.max(np.abs(G - O) / np.abs(O)) < 1e-12
+    with pytest.raises(egs.EGSError, match="Great Circle"):
+        ctx.covariance_tile(0, 5, 5, 0, 0, (x, y, z), (x, y, z), [1.0, 0.1, 0.5], metric=1)
+
+
+@pytest.mark.parametrize("n,dts,kernel", [(300, 64, 0), (515, 100, 0), (130, 64, 1)])
+def test_problem_covariance_matches_oracle(ctx, n, dts, kernel):
+    log("problem covariance n=%d dts=%d kernel=%d" % (n, dts, kernel))
+    x, y = _rng_locs(n, n)
+    th = [1.0, 0.1, 0.9] if kernel == 0 else [1.0, 0.8, 0.1, 0.5, 1.2, 0.3]
+    p = egs.Problem(ctx, kernel, x, y, dts=dts)
+    p.covariance(th)
+    G = p.get_matrix()
+    O = np.tril(orc.covariance_matrix(kernel, (x, y), (x, y), th, dts=dts, lower=False))
+    assert np.max(np.abs(G - O)) < 1e-14 * max(th[0], th[1])
+
+
+# ------------------------------------------------------------------ factorization
+@pytest.mark.parametrize("n,dts", [(64, 64), (200, 64), (700, 128), (1000, 320), (2100, 960)])
+def test_potrf_matches_oracle(ctx, n, dts):
+    log("potrf n=%d dts=%d" % (n, dts))
+    x, y = orc.generate_locations(n, seed=5)
+    th = [1.0, 0.1, 0.5]
+    p = egs.Problem(ctx, 0, x, y, dts=dts)
+    p.covariance(th)
+    p.potrf()
+    Lg = p.get_matrix()
+    A = orc.covariance_matrix(0, (x, y), (x, y), th, dts=dts, lower=False)
+    Lo = np.tril(orc.potrf(A, dts))
+    err = np.max(np.abs(Lg - Lo))
+    log("  max |L_gpu - L_oracle| = %.3e" % err)
+    assert err < 1e-11
+    assert np.max(np.abs(Lg @ Lg.T - A)) < 1e-12
+
+
+def test_not_positive_definite_reports(ctx):
+    log("not-SPD error path")
+    x, y = _rng_locs(100, 7)
+    p = egs.Problem(ctx, 0, x, y, dts=64)
+    p.set_observations(np.ones(100))
+    with pytest.raises(egs.NotPositiveDefinite, match="not positive definite"):
+        p.loglik([-1.0, 0.1, 0.5])
+    # the problem stays usable
+    llh, _, _ = p.loglik([1.0, 0.1, 0.5])
+    assert np.isfinite(llh)
+
+
+# ------------------------------------------------------------------ likelihood
+@pytest.mark.parametrize("n,dts,theta", [(1600, 320, [1.0, 0.1, 0.5]), (900, 64, [1.0, 0.1, 1.0]),
+                                         (1500, 960, [0.8, 0.03, 1.3]), (333, 128, [1.2, 0.2, 2.5])])
+def test_loglik_matches_oracle(ctx, n, dts, theta):
+    log("loglik n=%d dts=%d theta=%s" % (n, dts, theta))
+    x, y = orc.generate_locations(n, seed=0)
+    z = orc.generate_observations(0, (x, y), theta, dts, seed=0)
+    ref = orc.loglik(0, (x, y), theta, z, dts)
+    p = egs.Problem(ctx, 0, x, y, dts=dts)
+    p.set_observations(z)
+    llh, ld, q = p.loglik(theta)
+    log("  llh gpu %.15e oracle %.15e" % (llh, ref["llh"]))
+    assert abs(llh - ref["llh"]) <= 1e-10 * abs(ref["llh"])
+    assert abs(ld - ref["logdet"]) <= 1e-10 * max(1.0, abs(ref["logdet"]))
+    assert abs(q - ref["quad"]) <= 1e-10 * abs(ref["quad"])
+
+
+def test_loglik_bivariate_matches_oracle(ctx):
+    log("loglik bivariate")
+    x, y = orc.generate_locations(400, seed=2)
+    th = [1.0, 1.0, 0.1, 0.5, 1.0, 0.5]
+    z = orc.generate_observations(1, (x, y), th, 128, seed=0)
+    ref = orc.loglik(1, (x, y), th, z, 128)
+    p = egs.Problem(ctx, 1, x, y, dts=128)
+    p.set_observations(z)
+    llh, _, _ = p.loglik(th)
+    assert abs(llh - ref["llh"]) <= 1e-10 * abs(ref["llh"])
+
+
+def test_loglik_fixture_first_bobyqa_points(ctx, fixtures):
+    log("loglik fixture -24.026")
+    f = fixtures["modeling_exact"]
+    L16 = fixtures["locations16"]
+    p = egs.Problem(ctx, 0, L16["x"], L16["y"], dts=f["dts"])
+    p.set_observations(L16["z"])
+    lb = np.array(f["lb"])
+    step = 0.25 * (np.array(f["ub"]) - lb)
+    vals = [p.loglik(t)[0] for t in (lb, lb + step * [1, 0, 0], lb + step * [0, 1, 0])]
+    assert abs(max(vals) - f["loglik"]) < f["tol"]
+
+
+# ------------------------------------------------------------------ synthetic data
+@pytest.mark.parametrize("key,kernel", [("z_uni_n9", 0), ("z_uni_n16", 0), ("z_biv_n16", 1)])
+def test_generate_observations_fixture(ctx, fixtures, key, kernel):
+    log("generate observations %s" % key)
+    f = fixtures[key]
+    x, y = egs.generate_locations(f["n"], seed=f["seed"])
+    p = egs.Problem(ctx, kernel, x, y, dts=f["dts"])
+    z = p.generate_observations(f["theta"], seed=f["seed"])
+    assert np.max(np.abs(z[:len(f["z"])] - f["z"])) < f["tol"]
+    zo = orc.generate_observations(kernel, (x, y), f["theta"], f["dts"], seed=f["seed"])
+    assert np.max(np.abs(z - zo)) < 1e-13
+
+
+def test_generate_observations_large(ctx):
+    log("generate observations n=3000")
+    x, y = egs.generate_locations(3000, seed=4)
+    th = [1.0, 0.1, 0.5]
+    p = egs.Problem(ctx, 0, x, y, dts=320)
+    z = p.generate_observations(th, seed=3)
+    zo = orc.generate_observations(0, (x, y), th, 320, seed=3)
+    assert np.max(np.abs(z - zo)) < 1e-11
+
+
+# ------------------------------------------------------------------ kriging
+def test_kriging_fixture(ctx, fixtures):
+    log("kriging fixture")
+    f = fixtures["kriging_r"]
+    L16 = fixtures["locations16"]
+    x, y, z = np.array(L16["x"]), np.array(L16["y"]), np.array(L16["z"])
+    p = egs.Problem(ctx, 0, x, y, dts=f["dts"])
+    p.set_observations(z)
+    zm, _ = p.predict(f["theta"], x[:2], y[:2])
+    assert np.max(np.abs(zm - z[:2])) < f["interp_tol"]
+    keep = len(x) - len(f["test_x"])
+    p2 = egs.Problem(ctx, 0, x[:keep], y[:keep], dts=f["dts"])
+    p2.set_observations(z[:keep])
+    zm, _ = p2.predict(f["theta"], f["test_x"], f["test_y"])
+    assert np.max(np.abs(zm - f["expected"])) < f["tol"]
+
+
+def test_mspe_fixture(ctx, fixtures):
+    log("mspe fixture")
+    f = fixtures["mspe"]
+    L16 = fixtures["locations16"]
+    z, x, y = orc.shuffle(orc.GlibcRand(1), L16["z"], L16["x"], L16["y"])
+    k = f["zmiss"]
+    p = egs.Problem(ctx, 0, x[k:], y[k:], dts=f["dts"])
+    p.set_observations(z[k:])
+    _, mspe = p.predict(f["theta"], x[:k], y[:k], z_actual=z[:k])
+    assert abs(mspe - f["mspe"]) < f["rel_tol"] * f["mspe"]
+
+
+@pytest.mark.parametrize("n_obs,dts,theta", [(2000, 320, [1.0, 0.1, 1.0]), (1100, 960, [0.9, 0.05, 0.5])])
+def test_kriging_matches_oracle(ctx, n_obs, dts, theta):
+    log("kriging n_obs=%d" % n_obs)
+    x, y = orc.generate_locations(n_obs, seed=9)
+    z = orc.generate_observations(0, (x, y), theta, dts, seed=1)
+    rng = np.random.default_rng(0)
+    xm, ym = rng.random(50), rng.random(50)
+    p = egs.Problem(ctx, 0, x, y, dts=dts)
+    p.set_observations(z)
+    zm, _ = p.predict(theta, xm, ym)
+    zo, _ = orc.krige(0, (x, y), (xm, ym), theta, z, dts)
+    assert np.max(np.abs(zm - zo)) < 1e-8
