@@ -97,6 +97,11 @@ def main():
     elapsed = time.perf_counter() - t0
     upd = prob.kernel_stats(egs.capi.PROFILE_UPDATE)
     cov = prob.kernel_stats(egs.capi.PROFILE_COVARIANCE)
+    kstats = {name: prob.kernel_stats(kid) for name, kid in (("update", egs.capi.PROFILE_UPDATE),
+                                                            ("covariance", egs.capi.PROFILE_COVARIANCE),
+                                                            ("panel_trsm", egs.capi.PROFILE_PANEL_TRSM),
+                                                            ("diagonal", egs.capi.PROFILE_DIAGONAL),
+                                                            ("solve", egs.capi.PROFILE_SOLVE))}
     prob.profile(False)
 
     ms_per_step = 1000.0 * elapsed / args.steps
@@ -130,6 +135,9 @@ def main():
             "avg_launch_ms": upd["ms"] / max(1, upd["launches"]),
         },
         "phases_ms_per_step": {k: v / args.steps for k, v in phases.items()},
+        "kernels_per_step": {k: {"ms": v["ms"] / args.steps, "launches": v["launches"] // args.steps,
+                                 "tflops": (v["flops"] / (v["ms"] * 1e-3) / 1e12) if v["ms"] > 0 and v["flops"] > 0
+                                 else None} for k, v in kstats.items()},
         "covariance_gbps": None,
         "loglik": llh,
     }

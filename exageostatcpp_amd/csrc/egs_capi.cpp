@@ -50,7 +50,8 @@ struct egs_problem_s;
 
 struct egs_context_s {
     int device = 0;
-    hipStream_t stream = nullptr;
+    hipStream_t stream = nullptr;     // critical path: next panel, solves, copies
+    hipStream_t bulk = nullptr;       // trailing updates that the next panel does not wait for
     std::vector<egs_problem_s *> problems;   // live problems, released by egs_finalize
 };
 
@@ -96,6 +97,7 @@ struct egs_problem_s {
     double factored_theta[6] = {0, 0, 0, 0, 0, 0};
     double cov_theta[6] = {0, 0, 0, 0, 0, 0};
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    std::vector<hipEvent_t> ev_panel, ev_bulk;   // per step: panel solved / bulk update done
     double t_cov = 0, t_fact = 0, t_solve = 0;
     Profiler prof;
 
@@ -195,6 +197,12 @@ void free_problem_buffers(egs_problem_s *p) {
     for (auto &e : p->prof.ev)
         if (e) hipEventDestroy(e);
     p->prof.ev.clear();
+    for (auto &e : p->ev_panel)
+        if (e) hipEventDestroy(e);
+    for (auto &e : p->ev_bulk)
+        if (e) hipEventDestroy(e);
+    p->ev_panel.clear();
+    p->ev_bulk.clear();
     p->prof.pending.clear();
     p->d_x = p->d_y = p->d_z = nullptr;
     p->d_arena = nullptr;
@@ -208,8 +216,9 @@ void free_problem_buffers(egs_problem_s *p) {
 
 // Opens a timed region for kernel `id` on the problem's stream; returns the
 // event pair index or -1 when profiling is off.
-int prof_begin(egs_problem_s *p, int id, double flops) {
+int prof_begin(egs_problem_s *p, int id, double flops, hipStream_t stream = nullptr) {
     if (!p->prof.on) return -1;
+    if (!stream) stream = p->ctx->stream;
     Profiler &pr = p->prof;
     const size_t idx = pr.pending.size();
     if (2 * idx + 2 > pr.ev.size()) {
@@ -218,14 +227,14 @@ int prof_begin(egs_problem_s *p, int id, double flops) {
         pr.ev.push_back(a);
         pr.ev.push_back(b);
     }
-    if (hipEventRecord(pr.ev[2 * idx], p->ctx->stream) != hipSuccess) return -1;
+    if (hipEventRecord(pr.ev[2 * idx], stream) != hipSuccess) return -1;
     pr.pending.emplace_back(id, flops);
     return (int) idx;
 }
 
-void prof_end(egs_problem_s *p, int idx) {
+void prof_end(egs_problem_s *p, int idx, hipStream_t stream = nullptr) {
     if (idx < 0) return;
-    hipEventRecord(p->prof.ev[2 * (size_t) idx + 1], p->ctx->stream);
+    hipEventRecord(p->prof.ev[2 * (size_t) idx + 1], stream ? stream : p->ctx->stream);
 }
 
 // Call after the stream has synchronized.
@@ -302,40 +311,80 @@ int run_covariance(egs_problem_s *p, const double *theta) {
     return EGS_SUCCESS;
 }
 
-// Enqueue the tiled Cholesky; the caller checks d_info after synchronizing.
-int run_potrf(egs_problem_s *p) {
+// Algorithmic flops of updating tile columns [j0, j1) with one panel (lower part).
+double update_flops(const egs_problem_s *p, int j0, int j1) {
+    const double ts = (double) p->ts;
+    double f = 0.0;
+    for (int j = j0; j < j1; ++j) {
+        const double rj = real_rows(p, j);
+        f += rj * (rj + 1.0) * ts;                                                  // SYRK on tile (j, j)
+        for (int i = j + 1; i < p->nt; ++i) f += 2.0 * real_rows(p, i) * rj * ts;   // GEMM on tile (i, j)
+    }
+    return f;
+}
+
+// Diagonal tile k: 64 x 64 blocks left-looking, then the panel below it.
+void factor_panel(egs_problem_s *p, int k) {
     hipStream_t s = p->ctx->stream;
     const egs::TileMap m = p->map();
     const int nt = p->nt, nsb = p->nsb;
-    EGS_HIP(hipMemsetAsync(p->d_info, 0, sizeof(int), s));
     const double ts = (double) p->ts;
-    for (int k = 0; k < nt; ++k) {
-        double *wk = p->winv(k);
-        const double kb = real_rows(p, k);
-        int pi = prof_begin(p, EGS_PROFILE_DIAGONAL, kb * kb * kb / 3.0);
-        for (int jb = 0; jb < nsb; ++jb) {
-            egs::launch_potrf_block(m, k, jb, wk, p->d_info, p->n, s);
-            if (jb + 1 < nsb)
-                egs::launch_trsm_rows(m, k, (int64_t) k * nsb + jb + 1, nsb - jb - 1, jb, jb + 1, wk, s);
-        }
-        prof_end(p, pi);
-        if (k + 1 < nt) {
-            const int below = nt - k - 1;
-            double rows_below = 0.0, upd = 0.0;
-            for (int i = k + 1; i < nt; ++i) {
-                const double ri = real_rows(p, i);
-                rows_below += ri;
-                upd += ri * (ri + 1.0) * ts;                       // SYRK on the diagonal tile
-                for (int j = k + 1; j < i; ++j) upd += 2.0 * ri * real_rows(p, j) * ts;   // GEMM
-            }
-            pi = prof_begin(p, EGS_PROFILE_PANEL_TRSM, rows_below * ts * ts);
-            egs::launch_trsm_rows(m, k, (int64_t) (k + 1) * nsb, below * nsb, 0, nsb, wk, s);
-            prof_end(p, pi);
-            pi = prof_begin(p, EGS_PROFILE_UPDATE, upd);
-            egs::launch_update(m, p->d_pairs, below * (below + 1) / 2, k, s);
-            prof_end(p, pi);
-        }
+    double *wk = p->winv(k);
+    const double kb = real_rows(p, k);
+    int pi = prof_begin(p, EGS_PROFILE_DIAGONAL, kb * kb * kb / 3.0);
+    for (int jb = 0; jb < nsb; ++jb) {
+        egs::launch_potrf_block(m, k, jb, wk, p->d_info, p->n, s);
+        if (jb + 1 < nsb) egs::launch_trsm_rows(m, k, (int64_t) k * nsb + jb + 1, nsb - jb - 1, jb, jb + 1, wk, s);
     }
+    prof_end(p, pi);
+    if (k + 1 < nt) {
+        double rows_below = 0.0;
+        for (int i = k + 1; i < nt; ++i) rows_below += real_rows(p, i);
+        pi = prof_begin(p, EGS_PROFILE_PANEL_TRSM, rows_below * ts * ts);
+        egs::launch_trsm_rows(m, k, (int64_t) (k + 1) * nsb, (nt - k - 1) * nsb, 0, nsb, wk, s);
+        prof_end(p, pi);
+    }
+}
+
+// Enqueue the tiled Cholesky with a lookahead of one panel: the critical
+// stream updates tile column k+1, factors it and solves its panel while the
+// bulk stream applies panel k to the columns beyond k+1.  The update order
+// lists tiles by descending column, so both tile sets are contiguous ranges of
+// it.  The caller checks d_info after synchronizing the critical stream, which
+// ends waiting for the bulk stream.
+int run_potrf(egs_problem_s *p) {
+    hipStream_t s = p->ctx->stream, b = p->ctx->bulk;
+    const egs::TileMap m = p->map();
+    const int nt = p->nt;
+    for (int k = (int) p->ev_panel.size(); k < nt; ++k) {
+        hipEvent_t e1 = nullptr, e2 = nullptr;
+        EGS_HIP(hipEventCreateWithFlags(&e1, hipEventDisableTiming));
+        EGS_HIP(hipEventCreateWithFlags(&e2, hipEventDisableTiming));
+        p->ev_panel.push_back(e1);
+        p->ev_bulk.push_back(e2);
+    }
+    EGS_HIP(hipMemsetAsync(p->d_info, 0, sizeof(int), s));
+    factor_panel(p, 0);
+    EGS_HIP(hipEventRecord(p->ev_panel[0], s));
+    for (int k = 0; k + 1 < nt; ++k) {
+        const int beyond = nt - k - 2;                 // tile columns k+2 .. nt-1
+        const int n_beyond = beyond * (beyond + 1) / 2;
+        const int n_next = nt - k - 1;                 // tiles (i, k+1), i >= k+1
+        EGS_HIP(hipStreamWaitEvent(b, p->ev_panel[k], 0));
+        if (n_beyond > 0) {
+            const int pi = prof_begin(p, EGS_PROFILE_UPDATE, update_flops(p, k + 2, nt), b);
+            egs::launch_update(m, p->d_pairs, n_beyond, k, b);
+            prof_end(p, pi, b);
+        }
+        EGS_HIP(hipEventRecord(p->ev_bulk[k], b));
+        if (k > 0) EGS_HIP(hipStreamWaitEvent(s, p->ev_bulk[k - 1], 0));
+        const int pi = prof_begin(p, EGS_PROFILE_UPDATE, update_flops(p, k + 1, k + 2), s);
+        egs::launch_update(m, p->d_pairs + n_beyond, n_next, k, s);
+        prof_end(p, pi, s);
+        factor_panel(p, k + 1);
+        EGS_HIP(hipEventRecord(p->ev_panel[k + 1], s));
+    }
+    if (nt >= 2) EGS_HIP(hipStreamWaitEvent(s, p->ev_bulk[nt - 2], 0));
     EGS_HIP(hipGetLastError());
     return EGS_SUCCESS;
 }
@@ -414,7 +463,9 @@ int egs_init(int device, egs_context_t *ctx) {
     EGS_HIP(hipSetDevice(device));
     auto *c = new egs_context_s();
     c->device = device;
-    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&c->bulk, hipStreamNonBlocking) != hipSuccess) {
+        if (c->stream) hipStreamDestroy(c->stream);
         delete c;
         return fail(EGS_ERR_HIP, "hipStreamCreate failed");
     }
@@ -426,6 +477,7 @@ int egs_finalize(egs_context_t ctx) {
     if (!ctx) return EGS_SUCCESS;
     hipSetDevice(ctx->device);
     hipStreamSynchronize(ctx->stream);
+    hipStreamSynchronize(ctx->bulk);
     // problems still alive lose their device memory; destroying them later is safe
     while (!ctx->problems.empty()) {
         egs_problem_s *p = ctx->problems.back();
@@ -433,6 +485,7 @@ int egs_finalize(egs_context_t ctx) {
         p->ctx = nullptr;
     }
     hipStreamDestroy(ctx->stream);
+    hipStreamDestroy(ctx->bulk);
     delete ctx;
     return EGS_SUCCESS;
 }
@@ -441,6 +494,7 @@ int egs_synchronize(egs_context_t ctx) {
     if (!ctx) return fail(EGS_ERR_INVALID_ARG, "ctx is null");
     EGS_HIP(hipSetDevice(ctx->device));
     EGS_HIP(hipStreamSynchronize(ctx->stream));
+    EGS_HIP(hipStreamSynchronize(ctx->bulk));
     return EGS_SUCCESS;
 }
 

@@ -213,7 +213,12 @@ __global__ void __launch_bounds__(kThreads) update_kernel(TileMap map, const int
     Acc64 acc;
     acc_zero(acc);
     wg_gemm_nt(acc, A, ts, B, ts, ts, sm);
-    acc_foreach(acc, [&](int r, int c, double v) { C[r + (int64_t) c * ts] -= v; });
+    // read all 16 values of C first: one memory round trip for the read-modify-write
+    double cv[16];
+    int q = 0;
+    acc_foreach(acc, [&](int r, int c, double) { cv[q++] = C[r + (int64_t) c * ts]; });
+    q = 0;
+    acc_foreach(acc, [&](int r, int c, double v) { C[r + (int64_t) c * ts] = cv[q++] - v; });
 }
 
 void launch_potrf_block(const TileMap &map, int k, int jb, double *winv_k, int *info, int64_t n, hipStream_t s) {
@@ -228,6 +233,10 @@ void launch_trsm_rows(const TileMap &map, int k, int64_t g0, int count, int jb0,
 
 void launch_update(const TileMap &map, const int2 *pairs, int npairs, int k, hipStream_t s) {
     if (npairs <= 0) return;
+    if (update192_supported(map.ts)) {
+        launch_update192(map, pairs, npairs, k, s);
+        return;
+    }
     const int nsb = map.ts / kBlk;
     hipLaunchKernelGGL(update_kernel, dim3(nsb * nsb, npairs), dim3(kThreads), 0, s, map, pairs, k);
 }

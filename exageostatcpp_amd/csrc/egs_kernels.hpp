@@ -31,6 +31,9 @@ void launch_trsm_rows(const TileMap &map, int k, int64_t g0, int count, int jb0,
 // Trailing update C_ij -= L_ik L_jk^T for the first npairs lower tiles of the
 // update order (all (i, j) with i >= j > k).
 void launch_update(const TileMap &map, const int2 *pairs, int npairs, int k, hipStream_t s);
+// 192 x 192-block variant of the update (egs_update.hip), used when ts % 192 == 0.
+bool update192_supported(int ts);
+void launch_update192(const TileMap &map, const int2 *pairs, int npairs, int k, hipStream_t s);
 
 // ---- triangular solves and reductions (egs_solve.hip)
 // Forward solve of tile row k: y_k = L_kk^{-1} z_k in place, with per-tile

@@ -1,0 +1,165 @@
+// Trailing SYRK/GEMM update of the tiled Cholesky on gfx950, the kernel that
+// carries ~95% of the factorization flops (the reference's Chameleon runs the
+// same tile update C_ij -= L_ik L_jk^T as dsyrk/dgemm tasks).
+//
+// One workgroup of 512 threads (8 waves) owns a 192 x 192 block of C_ij and
+// contracts over K = ts with v_mfma_f64_16x16x4f64:
+//   * wave w covers rows (w % 2) * 96 .. +96 and columns (w / 2) * 48 .. +48,
+//     i.e. 6 x 3 independent 16 x 16 accumulators (72 FP64 values per lane);
+//   * A and B are staged through LDS in K-chunks of 16 columns, double
+//     buffered: the global loads of chunk c+1 are issued before the MFMAs of
+//     chunk c and land in the other LDS buffer afterwards, so each chunk costs
+//     one barrier and the HBM/L2 latency hides behind 72 MFMAs per wave;
+//   * global loads read 192 contiguous doubles per K-column (coalesced).
+// The operand panel is either the tile arena itself or a contiguous FP64 copy
+// of panel k (PanelRef), so the inner loop has no type dispatch.
+#include "egs_common.hpp"
+#include "egs_kernels.hpp"
+
+namespace egs {
+
+namespace {
+
+constexpr int kBM = 192;           // block rows (and columns)
+constexpr int kKC = 16;            // K chunk
+constexpr int kNT = 512;           // threads per workgroup
+constexpr int kLds = kBM + 4;      // LDS row stride in doubles
+constexpr int kPer = kBM * kKC / kNT;   // elements of A (and of B) each thread stages per chunk
+
+// Where the operand tiles of panel k live: in the tile arena itself (FP64 plan)
+// or in a contiguous FP64 copy of the panel (mixed plans), tile i of the panel
+// at pbase + (i - k - 1) * pstride.
+struct PanelRef {
+    const char *pbase;
+    int64_t pstride;
+    int contiguous;
+};
+
+__device__ __forceinline__ const double *panel_tile(const TileMap &map, const PanelRef &pr, int i, int k) {
+    if (pr.contiguous) return reinterpret_cast<const double *>(pr.pbase + (int64_t) (i - k - 1) * pr.pstride);
+    return reinterpret_cast<const double *>(map.base + map.off[i * map.nt + k]);
+}
+
+__global__ void __launch_bounds__(kNT, 1) update192_kernel(TileMap map, PanelRef pr, const int2 *pairs, int k) {
+    __shared__ double sA[2][kKC][kLds];
+    __shared__ double sB[2][kKC][kLds];
+    const int ts = map.ts;
+    const int nsb = ts / kBM;
+    const int nsub = nsb * nsb;
+    const int pair = blockIdx.x / nsub;
+    const int sub = blockIdx.x % nsub;
+    const int sbm = sub % nsb, sbn = sub / nsb;
+    const int2 ij = pairs[pair];
+    if (ij.x == ij.y && sbm < sbn) return;   // strictly upper block of a diagonal tile
+
+    const double *A = panel_tile(map, pr, ij.x, k) + sbm * kBM;
+    const double *B = panel_tile(map, pr, ij.y, k) + sbn * kBM;
+
+    const int t = threadIdx.x;
+    const int lane = t & 63, w = t >> 6;
+    const int wm = w & 1, wn = w >> 1;
+    const int lr = lane & 15, lk = lane >> 4;
+
+    // staging: element e = t + q * kNT of the 192 x 16 chunk sits at row e % 192,
+    // column e / 192; its offset inside the tile (in elements) fits in 32 bits.
+    uint32_t goff[kPer];
+    int loff[kPer];
+#pragma unroll
+    for (int q = 0; q < kPer; ++q) {
+        const int e = t + q * kNT;
+        goff[q] = (uint32_t) ((e % kBM) + (e / kBM) * ts);
+        loff[q] = (e / kBM) * kLds + (e % kBM);
+    }
+    const uint32_t kstep = (uint32_t) (kKC * ts);
+
+    f64x4 acc[6][3];
+#pragma unroll
+    for (int mi = 0; mi < 6; ++mi)
+#pragma unroll
+        for (int ni = 0; ni < 3; ++ni) acc[mi][ni] = f64x4{0.0, 0.0, 0.0, 0.0};
+
+    double ra[kPer], rb[kPer];
+#pragma unroll
+    for (int q = 0; q < kPer; ++q) {
+        ra[q] = A[goff[q]];
+        rb[q] = B[goff[q]];
+    }
+#pragma unroll
+    for (int q = 0; q < kPer; ++q) {
+        (&sA[0][0][0])[loff[q]] = ra[q];
+        (&sB[0][0][0])[loff[q]] = rb[q];
+    }
+    __syncthreads();
+
+    const int nchunks = ts / kKC;
+    uint32_t kofs = 0;
+    for (int c = 0; c < nchunks; ++c) {
+        const int buf = c & 1;
+        const bool more = c + 1 < nchunks;
+        kofs += kstep;
+        if (more) {
+#pragma unroll
+            for (int q = 0; q < kPer; ++q) {
+                ra[q] = A[goff[q] + kofs];
+                rb[q] = B[goff[q] + kofs];
+            }
+        }
+#pragma unroll
+        for (int kk = 0; kk < kKC; kk += 4) {
+            double a[6], b[3];
+#pragma unroll
+            for (int mi = 0; mi < 6; ++mi) a[mi] = sA[buf][kk + lk][wm * 96 + mi * 16 + lr];
+#pragma unroll
+            for (int ni = 0; ni < 3; ++ni) b[ni] = sB[buf][kk + lk][wn * 48 + ni * 16 + lr];
+#pragma unroll
+            for (int mi = 0; mi < 6; ++mi)
+#pragma unroll
+                for (int ni = 0; ni < 3; ++ni) acc[mi][ni] = mfma_f64(b[ni], a[mi], acc[mi][ni]);
+        }
+        if (more) {
+            double *dA = &sA[buf ^ 1][0][0];
+            double *dB = &sB[buf ^ 1][0][0];
+#pragma unroll
+            for (int q = 0; q < kPer; ++q) {
+                dA[loff[q]] = ra[q];
+                dB[loff[q]] = rb[q];
+            }
+        }
+        __syncthreads();
+    }
+
+    // C -= acc.  Lane l, element v of acc[mi][ni] is
+    // C[wm*96 + mi*16 + l%16][wn*48 + ni*16 + l/16 + 4v]  (FP64 MFMA output order).
+    // The 12 values of one mi are loaded together before any store so that the
+    // read-modify-write costs one memory round trip per mi, not one per value.
+    double *C = tile_f64(map, ij.x, ij.y);
+    const uint32_t cbase = (uint32_t) ((sbm * kBM + wm * 96 + lr) + (sbn * kBM + wn * 48 + lk) * ts);
+#pragma unroll
+    for (int mi = 0; mi < 6; ++mi) {
+        double cv[3][4];
+#pragma unroll
+        for (int ni = 0; ni < 3; ++ni)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) cv[ni][v] = __builtin_nontemporal_load(&C[cbase + (uint32_t) (mi * 16 + (ni * 16 + 4 * v) * ts)]);
+#pragma unroll
+        for (int ni = 0; ni < 3; ++ni)
+#pragma unroll
+            for (int v = 0; v < 4; ++v)
+                __builtin_nontemporal_store(cv[ni][v] - acc[mi][ni][v],
+                                            &C[cbase + (uint32_t) (mi * 16 + (ni * 16 + 4 * v) * ts)]);
+    }
+}
+
+}  // namespace
+
+bool update192_supported(int ts) { return ts % kBM == 0; }
+
+void launch_update192(const TileMap &map, const int2 *pairs, int npairs, int k, hipStream_t s) {
+    if (npairs <= 0) return;
+    const int nsb = map.ts / kBM;
+    const unsigned blocks = (unsigned) npairs * (unsigned) (nsb * nsb);
+    PanelRef pr{nullptr, 0, 0};
+    hipLaunchKernelGGL(update192_kernel, dim3(blocks), dim3(kNT), 0, s, map, pr, pairs, k);
+}
+
+}  // namespace egs
