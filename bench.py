@@ -3,17 +3,23 @@
 Workload (one step = one full likelihood evaluation, ExaGeoStatMLETile):
 univariate stationary Matern, theta = (1.0, 0.05, 0.5), n = 100,000 locations
 on a jittered 317 x 317 grid in [0,1]^2 (glibc rand() after srand(0), Morton
-order), observations z = L(theta) e generated on the GPU (dlarnv stream, seed
-0), dense tile size 960.  Each step regenerates Sigma(theta) in HBM, factors
-it, and reduces log|Sigma| and z' Sigma^-1 z; inputs are resident in HBM when
-the timed region starts.
+order), observations z = L(theta) e generated on the GPU in FP64 (dlarnv
+stream, seed 0), dense tile size 960.  Each step regenerates Sigma(theta) in
+HBM, factors it, and reduces log|Sigma| and z' Sigma^-1 z; inputs are resident
+in HBM when the timed region starts.
 
-Prints one JSON line (see the keys below).  The roofline entry is the trailing
-SYRK/GEMM update (the dominant kernel): algorithmic FP64 flops per launch over
-its average launch time, timed with HIP events on the library stream during the
-timed steps.  The CPU baseline times the oracle (a C restatement of the same
-algorithm, OpenMP over tiles) on a bounded sample and scales it to n by the
-n^3 (factorization) and n^2 (covariance, solve) laws.
+The headline value is the mixed-precision plan (tiles within band 2 of the
+diagonal FP64, band 3-10 FP32, the rest BF16 products with FP32 accumulation,
+z' Sigma^-1 z refined to an FP64 residual <= 1e-12).  The same data is first
+timed with the FP64 plan; the line reports both rates, the speedup and the
+relative log-likelihood error of the mixed plan against FP64.
+
+The roofline entry is the dominant kernel of the mixed run (the update class
+with the largest time): algorithmic flops per launch over its average launch
+time, timed with HIP events on the stream it runs on.  The CPU baseline times
+the oracle (a C restatement of the same algorithm, OpenMP over tiles) on a
+bounded sample and scales it to n by the n^3 (factorization) and n^2
+(covariance, solve) laws.
 """
 import argparse
 import json
@@ -25,11 +31,14 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "llh evals/s at n=100k, mixed vs FP64 GPU (rel llh err ≤1e-6); serve p50/p99"
-PEAK_FP64_TFLOPS = 78.6
+PEAK = {"fp64": 78.6, "fp32": 157.3, "bf16": 2500.0}
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
 
 
 def cpu_baseline(n_target, n_sample, dts, theta):
-    import numpy as np  # noqa: F401
     from oracle import orc
 
     x, y = orc.generate_locations(n_sample, seed=0)
@@ -45,11 +54,54 @@ def cpu_baseline(n_target, n_sample, dts, theta):
         "unit": "llh evals/s",
         "cores": threads,
         "kind": "port",
-        "sample": "oracle (C restatement: tiled Cholesky + Matern covariance, OpenMP over tiles) on one "
+        "sample": "oracle (C restatement: tiled Cholesky + Matern covariance, OpenMP over tiles) on one FP64 "
                   "likelihood evaluation at n=%d, dts=%d, same theta; measured %.2f s (covariance %.2f s, "
                   "factorization %.2f s, solve %.2f s) and scaled to n=%d by n^3 for the factorization and n^2 "
                   "for covariance and solve" % (n_sample, dts, wall, r["t_gen"], r["t_facto"], r["t_solve"], n_target),
     }
+
+
+def timed(egs, ctx, prob, theta, steps, warmup, label):
+    for i in range(warmup):
+        t0 = time.perf_counter()
+        prob.loglik(theta)
+        log("%s warmup %d: %.3f s" % (label, i, time.perf_counter() - t0))
+    prob.profile(True)
+    ctx.synchronize()
+    phases = {"covariance_ms": 0.0, "factor_ms": 0.0, "solve_ms": 0.0}
+    res = None
+    t0 = time.perf_counter()
+    for i in range(steps):
+        res = prob.loglik(theta)
+        for k, v in prob.timings().items():
+            phases[k] += v
+        log("%s step %d: llh %.10f" % (label, i, res[0]))
+    ctx.synchronize()
+    elapsed = time.perf_counter() - t0
+    kinds = (("update_fp64", egs.capi.PROFILE_UPDATE), ("update_fp32", egs.capi.PROFILE_UPDATE_F32),
+             ("update_bf16", egs.capi.PROFILE_UPDATE_BF16), ("covariance", egs.capi.PROFILE_COVARIANCE),
+             ("panel_trsm", egs.capi.PROFILE_PANEL_TRSM), ("diagonal", egs.capi.PROFILE_DIAGONAL),
+             ("forward_solve", egs.capi.PROFILE_SOLVE), ("refine", egs.capi.PROFILE_REFINE))
+    ks = {name: prob.kernel_stats(kid) for name, kid in kinds}
+    prob.profile(False)
+    return {
+        "evals_per_s": steps / elapsed,
+        "ms_per_step": 1000.0 * elapsed / steps,
+        "loglik": res[0], "logdet": res[1], "quad": res[2],
+        "phases_ms_per_step": {k: v / steps for k, v in phases.items()},
+        "kernels": ks,
+        "refinement": prob.refinement(),
+    }
+
+
+def kernel_table(ks, steps):
+    out = {}
+    for k, v in ks.items():
+        if v["launches"] == 0:
+            continue
+        out[k] = {"ms": v["ms"] / steps, "launches": v["launches"] // steps,
+                  "tflops": (v["flops"] / (v["ms"] * 1e-3) / 1e12) if v["ms"] > 0 and v["flops"] > 0 else None}
+    return out
 
 
 def main():
@@ -57,8 +109,12 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--fp64-steps", type=int, default=None, help="timed FP64 steps (default: --steps)")
     ap.add_argument("--n", type=int, default=100000)
     ap.add_argument("--dts", type=int, default=960)
+    ap.add_argument("--band-fp64", type=int, default=2)
+    ap.add_argument("--band-fp32", type=int, default=10)
+    ap.add_argument("--plain-bf16", action="store_true", help="single BF16 product instead of the split one")
     ap.add_argument("--cpu-sample", type=int, default=6000)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
@@ -75,78 +131,76 @@ def main():
     ctx = egs.Context(0)
     prob = egs.Problem(ctx, "UnivariateMaternStationary", x, y, dts=args.dts)
     prob.generate_observations(theta, seed=0)
-    print("setup: n=%d, data generated in %.1f s" % (args.n, time.perf_counter() - t_setup), file=sys.stderr,
-          flush=True)
+    log("setup: n=%d, data generated in %.1f s" % (args.n, time.perf_counter() - t_setup))
 
-    for i in range(args.warmup):
-        t0 = time.perf_counter()
-        prob.loglik(theta)
-        print("warmup %d: %.3f s" % (i, time.perf_counter() - t0), file=sys.stderr, flush=True)
+    fp64_steps = args.fp64_steps if args.fp64_steps is not None else args.steps
+    fp64 = timed(egs, ctx, prob, theta, fp64_steps, args.warmup, "fp64")
+    mode = egs.PRECISION_MIXED_BF16 if args.plain_bf16 else egs.PRECISION_MIXED
+    prob.set_precision(mode, args.band_fp64, args.band_fp32)
+    mixed = timed(egs, ctx, prob, theta, args.steps, args.warmup, "mixed")
+    rel_err = abs(mixed["loglik"] - fp64["loglik"]) / abs(fp64["loglik"])
 
-    prob.profile(True)
-    ctx.synchronize()
-    llh = None
-    phases = {"covariance_ms": 0.0, "factor_ms": 0.0, "solve_ms": 0.0}
-    t0 = time.perf_counter()
-    for i in range(args.steps):
-        llh, logdet, quad = prob.loglik(theta)
-        for k, v in prob.timings().items():
-            phases[k] += v
-        print("step %d: llh %.10f" % (i, llh), file=sys.stderr, flush=True)
-    ctx.synchronize()
-    elapsed = time.perf_counter() - t0
-    upd = prob.kernel_stats(egs.capi.PROFILE_UPDATE)
-    cov = prob.kernel_stats(egs.capi.PROFILE_COVARIANCE)
-    kstats = {name: prob.kernel_stats(kid) for name, kid in (("update", egs.capi.PROFILE_UPDATE),
-                                                            ("covariance", egs.capi.PROFILE_COVARIANCE),
-                                                            ("panel_trsm", egs.capi.PROFILE_PANEL_TRSM),
-                                                            ("diagonal", egs.capi.PROFILE_DIAGONAL),
-                                                            ("solve", egs.capi.PROFILE_SOLVE))}
-    prob.profile(False)
+    # dominant kernel of the mixed run
+    peaks = {"update_fp64": ("fp64", 1.0), "update_fp32": ("fp32", 1.0),
+             "update_bf16": ("bf16", 1.0 if args.plain_bf16 else 3.0)}
+    dom = max(peaks, key=lambda k: mixed["kernels"][k]["ms"])
+    d = mixed["kernels"][dom]
+    achieved = d["flops"] / (d["ms"] * 1e-3) / 1e12 if d["ms"] > 0 else 0.0
+    prec, mfma_per_flop = peaks[dom]
+    dom64 = fp64["kernels"]["update_fp64"]
+    ach64 = dom64["flops"] / (dom64["ms"] * 1e-3) / 1e12 if dom64["ms"] > 0 else 0.0
 
-    ms_per_step = 1000.0 * elapsed / args.steps
-    achieved = upd["flops"] / (upd["ms"] * 1e-3) / 1e12 if upd["ms"] > 0 else 0.0
     out = {
         "metric": METRIC,
-        "value": args.steps / elapsed,
+        "value": mixed["evals_per_s"],
         "unit": "llh evals/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms_per_step,
+        "ms_per_step": mixed["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f64+f32+bf16",
         "data": "synthetic: jittered grid locations (srand(0)), z = L(theta) e from the dlarnv stream, "
-                "generated on the GPU",
-        "config": {"workload": "univariate Matern likelihood, n=%d, theta=(1.0, 0.05, 0.5), dts=%d, FP64"
-                               % (args.n, args.dts),
-                   "n": args.n, "dts": args.dts, "precision": "fp64"},
+                "generated on the GPU in FP64",
+        "config": {"workload": "univariate Matern likelihood, n=%d, theta=(1.0, 0.05, 0.5), dts=%d, mixed plan "
+                               "FP64 band <=%d, FP32 band <=%d, %s beyond" % (
+                                   args.n, args.dts, args.band_fp64, args.band_fp32,
+                                   "BF16" if args.plain_bf16 else "split-BF16"),
+                   "n": args.n, "dts": args.dts, "band_fp64": args.band_fp64, "band_fp32": args.band_fp32,
+                   "bf16_products": 1 if args.plain_bf16 else 3},
         "roofline": {
             "bound": "mfma",
-            "kernel": "update_kernel (trailing SYRK/GEMM, FP64 MFMA)",
+            "kernel": dom,
             "achieved": achieved,
-            "peak": PEAK_FP64_TFLOPS,
+            "peak": PEAK[prec],
             "unit": "TFLOP/s",
-            "frac": achieved / PEAK_FP64_TFLOPS,
+            "frac": achieved * mfma_per_flop / PEAK[prec],
             "traffic": None,
-            "launches": upd["launches"],
-            "avg_launch_ms": upd["ms"] / max(1, upd["launches"]),
+            "launches": d["launches"],
+            "avg_launch_ms": d["ms"] / max(1, d["launches"]),
+            "note": "achieved counts algorithmic flops; frac counts the MFMA work (x%d for split BF16 products)"
+                    % int(mfma_per_flop),
         },
-        "phases_ms_per_step": {k: v / args.steps for k, v in phases.items()},
-        "kernels_per_step": {k: {"ms": v["ms"] / args.steps, "launches": v["launches"] // args.steps,
-                                 "tflops": (v["flops"] / (v["ms"] * 1e-3) / 1e12) if v["ms"] > 0 and v["flops"] > 0
-                                 else None} for k, v in kstats.items()},
-        "covariance_gbps": None,
-        "loglik": llh,
+        "fp64": {
+            "evals_per_s": fp64["evals_per_s"], "ms_per_step": fp64["ms_per_step"], "steps": fp64_steps,
+            "loglik": fp64["loglik"],
+            "whole_eval_tflops": args.n ** 3 / 3.0 / (fp64["ms_per_step"] * 1e-3) / 1e12,
+            "update_kernel": {"achieved": ach64, "peak": PEAK["fp64"], "frac": ach64 / PEAK["fp64"]},
+            "phases_ms_per_step": fp64["phases_ms_per_step"],
+            "kernels_per_step": kernel_table(fp64["kernels"], fp64_steps),
+        },
+        "mixed": {
+            "loglik": mixed["loglik"], "rel_llh_err_vs_fp64": rel_err,
+            "logdet_err_vs_fp64": mixed["logdet"] - fp64["logdet"],
+            "quad_rel_err_vs_fp64": (mixed["quad"] - fp64["quad"]) / fp64["quad"],
+            "refinement": mixed["refinement"],
+            "speedup_vs_fp64": mixed["evals_per_s"] / fp64["evals_per_s"],
+            "phases_ms_per_step": mixed["phases_ms_per_step"],
+            "kernels_per_step": kernel_table(mixed["kernels"], args.steps),
+        },
     }
-    if cov["launches"]:
-        nt = -(-args.n // args.dts)
-        tile_bytes = 8.0 * args.dts * args.dts
-        out["covariance_gbps"] = nt * (nt + 1) / 2 * tile_bytes / (cov["ms"] / cov["launches"] * 1e-3) / 1e9
-    whole_flops = args.n ** 3 / 3.0
-    out["whole_eval_tflops"] = whole_flops / (ms_per_step * 1e-3) / 1e12
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args.n, args.cpu_sample, args.dts, theta)
     prob.close()

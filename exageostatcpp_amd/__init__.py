@@ -7,4 +7,5 @@ mirror of ExaGeoStat<T> lives in include/exageostat_amd/ and the kernels in
 csrc/.
 """
 from .capi import (Context, EGSError, NotPositiveDefinite, Problem, generate_locations, lib, srand,  # noqa: F401
-                   KERNEL_BIVARIATE, KERNEL_UNIVARIATE, PRECISION_FP64, PRECISION_MIXED, EXPORTS)
+                   KERNEL_BIVARIATE, KERNEL_UNIVARIATE, PRECISION_FP64, PRECISION_MIXED, PRECISION_MIXED_BF16,
+                   EXPORTS)

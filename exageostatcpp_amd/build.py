@@ -15,7 +15,7 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libexageostat_amd.so")
 
-SOURCES = ["egs_covgen.hip", "egs_chol.hip", "egs_update.hip", "egs_solve.hip", "egs_capi.cpp"]
+SOURCES = ["egs_covgen.hip", "egs_chol.hip", "egs_update.hip", "egs_mixed.hip", "egs_solve.hip", "egs_capi.cpp"]
 ARCH = os.environ.get("EGS_OFFLOAD_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 

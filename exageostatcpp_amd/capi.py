@@ -28,11 +28,15 @@ KERNELS = {"UnivariateMaternStationary": 0, "univariate_matern_stationary": 0,
            "BivariateMaternParsimonious": 1, "bivariate_matern_parsimonious": 1}
 PRECISION_FP64 = 0
 PRECISION_MIXED = 1
+PRECISION_MIXED_BF16 = 2
 PROFILE_UPDATE = 0
 PROFILE_COVARIANCE = 1
 PROFILE_PANEL_TRSM = 2
 PROFILE_DIAGONAL = 3
 PROFILE_SOLVE = 4
+PROFILE_UPDATE_F32 = 5
+PROFILE_UPDATE_BF16 = 6
+PROFILE_REFINE = 7
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _i64 = ctypes.c_int64
@@ -71,6 +75,7 @@ EXPORTS = {
     "egs_solve_logdet_quad": ([_vp, _dp, _dp], _int),
     "egs_get_matrix": ([_vp, _dp, _i64], _int),
     "egs_last_timings": ([_vp, _dp, _dp, _dp], _int),
+    "egs_last_refinement": ([_vp, ctypes.POINTER(_int), _dp], _int),
     "egs_problem_profile": ([_vp, _int], _int),
     "egs_problem_kernel_stats": ([_vp, _int, ctypes.POINTER(_i64), _dp, _dp], _int),
     "egs_generate_locations": ([_i64, _int, _dp, _dp, _dp], _int),
@@ -225,6 +230,11 @@ class Problem:
         a, b, c = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
         _check(lib().egs_last_timings(self.handle, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)))
         return {"covariance_ms": a.value, "factor_ms": b.value, "solve_ms": c.value}
+
+    def refinement(self):
+        it, res = _int(), ctypes.c_double()
+        _check(lib().egs_last_refinement(self.handle, ctypes.byref(it), ctypes.byref(res)))
+        return {"iterations": it.value, "residual": res.value}
 
     def profile(self, enable=True):
         _check(lib().egs_problem_profile(self.handle, 1 if enable else 0))

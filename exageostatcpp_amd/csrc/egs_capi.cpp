@@ -76,6 +76,21 @@ struct egs_problem_s {
     int dts = 0, ts = 0, nt = 0, nsb = 0;
     int64_t n_loc = 0, n = 0, npad = 0;
     int mode = EGS_PRECISION_FP64, band64 = 2, band32 = 10;
+    // mixed plans: per-class update lists (descending column) and, for class c,
+    // cls_count[c][j] = number of its tiles in columns >= j
+    int2 *d_pairs_cls[3] = {nullptr, nullptr, nullptr};
+    std::vector<int2> h_pairs_cls[3];
+    std::vector<int> cls_count[3];
+    // panel k as FP64 column-major, FP32 column-major and FP32 row-major, in
+    // slot k % 2: with the lookahead the critical stream writes panel k+1 while
+    // the bulk stream still reads panel k
+    double *d_p64[2] = {nullptr, nullptr};
+    float *d_p32[2] = {nullptr, nullptr}, *d_p32t[2] = {nullptr, nullptr};
+    double *d_cg = nullptr;                  // refinement vectors x, r, p, q, w (npad each)
+    double *d_mv = nullptr;                  // matrix-vector chunk partials
+    double *d_dot = nullptr;                 // dot-product block partials
+    int refine_iters = 0;
+    double refine_resid = 0.0;
 
     double *d_x = nullptr, *d_y = nullptr, *d_z = nullptr;
     char *d_arena = nullptr;
@@ -192,6 +207,21 @@ void free_problem_buffers(egs_problem_s *p) {
     hipFree(p->d_aux);
     hipFree(p->d_partials);
     hipFree(p->d_info);
+    for (auto &q : p->d_pairs_cls) {
+        hipFree(q);
+        q = nullptr;
+    }
+    for (int b = 0; b < 2; ++b) {
+        hipFree(p->d_p64[b]);
+        hipFree(p->d_p32[b]);
+        hipFree(p->d_p32t[b]);
+        p->d_p64[b] = nullptr;
+        p->d_p32[b] = p->d_p32t[b] = nullptr;
+    }
+    hipFree(p->d_cg);
+    hipFree(p->d_mv);
+    hipFree(p->d_dot);
+    p->d_cg = p->d_mv = p->d_dot = nullptr;
     for (auto &e : p->ev)
         if (e) hipEventDestroy(e);
     for (auto &e : p->prof.ev)
@@ -261,6 +291,51 @@ double real_rows(const egs_problem_s *p, int i) {
 uint8_t tile_store(const egs_problem_s *p, int i, int j) {
     if (p->mode == EGS_PRECISION_FP64) return egs::kStoreF64;
     return (i - j) <= p->band64 ? egs::kStoreF64 : egs::kStoreF32;
+}
+
+// Update class of tile (i, j) under a mixed plan: 0 FP64, 1 FP32, 2 BF16.
+int tile_class(const egs_problem_s *p, int i, int j) {
+    const int d = i - j;
+    return d <= p->band64 ? 0 : (d <= p->band32 ? 1 : 2);
+}
+
+bool is_mixed(const egs_problem_s *p) { return p->mode != EGS_PRECISION_FP64; }
+
+// Buffers of a mixed plan: class lists, panel copies, refinement vectors.
+int setup_mixed(egs_problem_s *p) {
+    const int nt = p->nt;
+    for (int c = 0; c < 3; ++c) {
+        p->h_pairs_cls[c].clear();
+        p->cls_count[c].assign((size_t) nt + 1, 0);
+    }
+    for (int j = nt - 1; j >= 0; --j) {
+        for (int i = j; i < nt; ++i) p->h_pairs_cls[tile_class(p, i, j)].push_back(int2{i, j});
+        for (int c = 0; c < 3; ++c) p->cls_count[c][j] = (int) p->h_pairs_cls[c].size();
+    }
+    for (int c = 0; c < 3; ++c) {
+        hipFree(p->d_pairs_cls[c]);
+        p->d_pairs_cls[c] = nullptr;
+        const size_t bytes = sizeof(int2) * std::max<size_t>(1, p->h_pairs_cls[c].size());
+        if (hipMalloc(&p->d_pairs_cls[c], bytes) != hipSuccess)
+            return fail(EGS_ERR_OUT_OF_MEMORY, "cannot allocate the update lists");
+        if (!p->h_pairs_cls[c].empty())
+            EGS_HIP(hipMemcpy(p->d_pairs_cls[c], p->h_pairs_cls[c].data(), sizeof(int2) * p->h_pairs_cls[c].size(),
+                              hipMemcpyHostToDevice));
+    }
+    const size_t panel = (size_t) p->npad * p->ts;
+    if (!p->d_p64[0]) {
+        if (hipMalloc(&p->d_p64[0], panel * sizeof(double)) != hipSuccess ||
+            hipMalloc(&p->d_p64[1], panel * sizeof(double)) != hipSuccess ||
+            hipMalloc(&p->d_p32[0], panel * sizeof(float)) != hipSuccess ||
+            hipMalloc(&p->d_p32[1], panel * sizeof(float)) != hipSuccess ||
+            hipMalloc(&p->d_p32t[0], panel * sizeof(float)) != hipSuccess ||
+            hipMalloc(&p->d_p32t[1], panel * sizeof(float)) != hipSuccess ||
+            hipMalloc(&p->d_cg, 5 * sizeof(double) * (size_t) p->npad) != hipSuccess ||
+            hipMalloc(&p->d_mv, sizeof(double) * (size_t) egs::matvec_chunks(p->n) * p->npad) != hipSuccess ||
+            hipMalloc(&p->d_dot, sizeof(double) * egs::dot_blocks()) != hipSuccess)
+            return fail(EGS_ERR_OUT_OF_MEMORY, "cannot allocate the mixed-precision work buffers");
+    }
+    return EGS_SUCCESS;
 }
 
 // (Re)build the tile layout in HBM for the current plan.
@@ -341,8 +416,55 @@ void factor_panel(egs_problem_s *p, int k) {
         double rows_below = 0.0;
         for (int i = k + 1; i < nt; ++i) rows_below += real_rows(p, i);
         pi = prof_begin(p, EGS_PROFILE_PANEL_TRSM, rows_below * ts * ts);
-        egs::launch_trsm_rows(m, k, (int64_t) (k + 1) * nsb, (nt - k - 1) * nsb, 0, nsb, wk, s);
+        if (!is_mixed(p)) {
+            egs::launch_trsm_rows(m, k, (int64_t) (k + 1) * nsb, (nt - k - 1) * nsb, 0, nsb, wk, s);
+        } else {
+            // rows within band64 of the diagonal are FP64 tiles, the rest FP32
+            const int last64 = std::min(nt - 1, k + p->band64);
+            if (last64 >= k + 1)
+                egs::launch_trsm_rows(m, k, (int64_t) (k + 1) * nsb, (last64 - k) * nsb, 0, nsb, wk, s, false);
+            if (last64 + 1 < nt)
+                egs::launch_trsm_rows(m, k, (int64_t) (last64 + 1) * nsb, (nt - 1 - last64) * nsb, 0, nsb, wk, s,
+                                      true);
+        }
         prof_end(p, pi);
+        if (is_mixed(p)) egs::launch_panel_convert(m, k, p->d_p64[k & 1], p->d_p32[k & 1], p->d_p32t[k & 1], s);
+    }
+}
+
+// Algorithmic flops of updating, with one panel, the class-c tiles (c < 0: all)
+// of tile columns [j0, j1).
+double update_flops_cls(const egs_problem_s *p, int c, int j0, int j1) {
+    const double ts = (double) p->ts;
+    double f = 0.0;
+    for (int j = j0; j < j1; ++j) {
+        const double rj = real_rows(p, j);
+        for (int i = j; i < p->nt; ++i) {
+            if (c >= 0 && tile_class(p, i, j) != c) continue;
+            f += (i == j) ? rj * (rj + 1.0) * ts : 2.0 * real_rows(p, i) * rj * ts;
+        }
+    }
+    return f;
+}
+
+// Updates of step k under a mixed plan for tile columns [j0, nt) restricted to
+// the column range encoded by the class prefix counts: entries [lo_c, hi_c) of
+// each class list.
+void launch_mixed_update(egs_problem_s *p, int k, int jcol_lo, int jcol_hi, hipStream_t st) {
+    const egs::TileMap m = p->map();
+    static const int prof_id[3] = {EGS_PROFILE_UPDATE, EGS_PROFILE_UPDATE_F32, EGS_PROFILE_UPDATE_BF16};
+    for (int c = 0; c < 3; ++c) {
+        // class-c tiles with column in [jcol_lo, jcol_hi): list entries [count[jcol_hi], count[jcol_lo])
+        const int beg = p->cls_count[c][jcol_hi];
+        const int end = p->cls_count[c][jcol_lo];
+        if (end <= beg) continue;
+        const int pi = prof_begin(p, prof_id[c], update_flops_cls(p, c, jcol_lo, jcol_hi), st);
+        const int2 *pr = p->d_pairs_cls[c] + beg;
+        const int slot = k & 1;
+        if (c == 0) egs::launch_update192_panel(m, p->d_p64[slot], pr, end - beg, k, st);
+        else if (c == 1) egs::launch_update192_f32(m, p->d_p32[slot], pr, end - beg, k, st);
+        else egs::launch_update192_bf16(m, p->d_p32t[slot], pr, end - beg, k, p->mode == EGS_PRECISION_MIXED, st);
+        prof_end(p, pi, st);
     }
 }
 
@@ -371,16 +493,22 @@ int run_potrf(egs_problem_s *p) {
         const int n_beyond = beyond * (beyond + 1) / 2;
         const int n_next = nt - k - 1;                 // tiles (i, k+1), i >= k+1
         EGS_HIP(hipStreamWaitEvent(b, p->ev_panel[k], 0));
-        if (n_beyond > 0) {
+        if (is_mixed(p)) {
+            if (k + 2 < nt) launch_mixed_update(p, k, k + 2, nt, b);
+        } else if (n_beyond > 0) {
             const int pi = prof_begin(p, EGS_PROFILE_UPDATE, update_flops(p, k + 2, nt), b);
             egs::launch_update(m, p->d_pairs, n_beyond, k, b);
             prof_end(p, pi, b);
         }
         EGS_HIP(hipEventRecord(p->ev_bulk[k], b));
         if (k > 0) EGS_HIP(hipStreamWaitEvent(s, p->ev_bulk[k - 1], 0));
-        const int pi = prof_begin(p, EGS_PROFILE_UPDATE, update_flops(p, k + 1, k + 2), s);
-        egs::launch_update(m, p->d_pairs + n_beyond, n_next, k, s);
-        prof_end(p, pi, s);
+        if (is_mixed(p)) {
+            launch_mixed_update(p, k, k + 1, k + 2, s);
+        } else {
+            const int pi = prof_begin(p, EGS_PROFILE_UPDATE, update_flops(p, k + 1, k + 2), s);
+            egs::launch_update(m, p->d_pairs + n_beyond, n_next, k, s);
+            prof_end(p, pi, s);
+        }
         factor_panel(p, k + 1);
         EGS_HIP(hipEventRecord(p->ev_panel[k + 1], s));
     }
@@ -400,13 +528,15 @@ int read_info(egs_problem_s *p) {
     return EGS_SUCCESS;
 }
 
-// Forward solve of d_work in place with the per-tile partials.
-void run_forward(egs_problem_s *p) {
+// Forward solve of d_work in place; per-tile log-det / dot partials go to
+// `partials` (d_partials unless given).
+void run_forward(egs_problem_s *p, double *partials = nullptr) {
     hipStream_t s = p->ctx->stream;
     const egs::TileMap m = p->map();
+    if (!partials) partials = p->d_partials;
     const int pi = prof_begin(p, EGS_PROFILE_SOLVE, (double) p->n * (double) p->n);
     for (int k = 0; k < p->nt; ++k) {
-        egs::launch_trsv_diag(m, k, p->winv(k), p->d_work, p->d_partials, s);
+        egs::launch_trsv_diag(m, k, p->winv(k), p->d_work, partials, s);
         egs::launch_trsv_panel(m, k, p->d_work, s);
     }
     prof_end(p, pi);
@@ -419,6 +549,92 @@ void run_backward(egs_problem_s *p) {
         egs::launch_trsv_t_partial(m, k, p->d_work, p->d_aux, s);
         egs::launch_trsv_t_diag(m, k, p->winv(k), p->d_work, p->d_aux, s);
     }
+}
+
+// out = (L L^T)^{-1} in with the current factor (both vectors npad long).
+void apply_factor_inverse(egs_problem_s *p, const double *in, double *out) {
+    hipStream_t s = p->ctx->stream;
+    hipMemcpyAsync(p->d_work, in, sizeof(double) * (size_t) p->npad, hipMemcpyDeviceToDevice, s);
+    run_forward(p, p->d_mv);   // partials of a preconditioner solve are not needed
+    run_backward(p);
+    hipMemcpyAsync(out, p->d_work, sizeof(double) * (size_t) p->npad, hipMemcpyDeviceToDevice, s);
+}
+
+int host_dot(egs_problem_s *p, const double *x, const double *y, double &out) {
+    egs::launch_dot(x, y, p->npad, p->d_dot, p->ctx->stream);
+    double part[1024];
+    const int nb = egs::dot_blocks();
+    EGS_HIP(hipMemcpyAsync(part, p->d_dot, sizeof(double) * nb, hipMemcpyDeviceToHost, p->ctx->stream));
+    EGS_HIP(hipStreamSynchronize(p->ctx->stream));
+    double acc = 0.0;
+    for (int b = 0; b < nb; ++b) acc += part[b];
+    out = acc;
+    return EGS_SUCCESS;
+}
+
+// q = Sigma v in FP64 (covariance regenerated from the locations at the
+// problem's current theta); q = base - Sigma v when base is given.
+int sigma_times(egs_problem_s *p, const double *v, double *q, const double *base) {
+    egs::MaternParams prm;
+    int st = make_params(p->kernel, p->metric, p->cov_theta, prm);
+    if (st) return st;
+    const double *dz = (p->kernel == EGS_KERNEL_UNIVARIATE_MATERN_STATIONARY) ? p->d_z : nullptr;
+    egs::launch_cov_matvec(p->d_mv, v, p->n, p->npad, prm, p->d_x, p->d_y, dz, p->ctx->stream);
+    egs::launch_matvec_reduce(q, base, base ? -1.0 : 1.0, p->d_mv, egs::matvec_chunks(p->n), p->n, p->npad,
+                              p->ctx->stream);
+    EGS_HIP(hipGetLastError());
+    return EGS_SUCCESS;
+}
+
+// Iterative refinement of Sigma x = b for mixed plans: conjugate gradients
+// preconditioned by the mixed factor, residuals b - Sigma x in FP64 with
+// Sigma regenerated from the locations.  Stops at ||b - Sigma x|| <=
+// tol ||b||.  x is left in d_cg[0 .. npad).
+constexpr double kRefineTol = 1e-12;
+constexpr int kRefineMaxIter = 60;
+
+int refine_solve(egs_problem_s *p, const double *b) {
+    hipStream_t s = p->ctx->stream;
+    const int64_t np = p->npad;
+    double *X = p->d_cg, *R = X + np, *P = R + np, *Q = P + np, *W = Q + np;
+    const int pi = prof_begin(p, EGS_PROFILE_REFINE, 0.0);
+    double bb = 0.0, rr = 0.0, rz = 0.0, pq = 0.0;
+    int st = host_dot(p, b, b, bb);
+    if (st) return st;
+    apply_factor_inverse(p, b, X);
+    st = sigma_times(p, X, R, b);
+    if (st) return st;
+    st = host_dot(p, R, R, rr);
+    if (st) return st;
+    int it = 0;
+    if (bb > 0.0 && std::sqrt(rr / bb) > kRefineTol) {
+        apply_factor_inverse(p, R, W);
+        egs::launch_axpby(P, W, 1.0, 0.0, np, s);
+        if ((st = host_dot(p, R, W, rz))) return st;
+        for (it = 1; it <= kRefineMaxIter; ++it) {
+            if ((st = sigma_times(p, P, Q, nullptr))) return st;
+            if ((st = host_dot(p, P, Q, pq))) return st;
+            if (!(pq > 0.0)) break;
+            const double alpha = rz / pq;
+            egs::launch_axpby(X, P, alpha, 1.0, np, s);
+            egs::launch_axpby(R, Q, -alpha, 1.0, np, s);
+            if ((st = host_dot(p, R, R, rr))) return st;
+            if (std::sqrt(rr / bb) <= kRefineTol) break;
+            apply_factor_inverse(p, R, W);
+            double rz_new = 0.0;
+            if ((st = host_dot(p, R, W, rz_new))) return st;
+            const double beta = rz_new / rz;
+            rz = rz_new;
+            egs::launch_axpby(P, W, 1.0, beta, np, s);
+        }
+    }
+    // true residual of the returned solution
+    if ((st = sigma_times(p, X, R, b))) return st;
+    if ((st = host_dot(p, R, R, rr))) return st;
+    prof_end(p, pi);
+    p->refine_iters = it;
+    p->refine_resid = bb > 0.0 ? std::sqrt(rr / bb) : 0.0;
+    return EGS_SUCCESS;
 }
 
 int read_partials(egs_problem_s *p, double &logdet, double &quad) {
@@ -628,15 +844,19 @@ int egs_problem_device_bytes(egs_problem_t p, int64_t *bytes) {
 int egs_problem_set_precision(egs_problem_t p, int mode, int band_fp64, int band_fp32) {
     int st = check_problem(p);
     if (st) return st;
-    if (mode != EGS_PRECISION_FP64 && mode != EGS_PRECISION_MIXED)
+    if (mode != EGS_PRECISION_FP64 && mode != EGS_PRECISION_MIXED && mode != EGS_PRECISION_MIXED_BF16)
         return fail(EGS_ERR_INVALID_ARG, "unknown precision mode %d", mode);
-    if (mode == EGS_PRECISION_MIXED)
-        return fail(EGS_ERR_UNSUPPORTED, "mixed precision is not available in this build");
     if (band_fp64 < 0 || band_fp32 < band_fp64) return fail(EGS_ERR_INVALID_ARG, "bands must satisfy 0 <= fp64 <= fp32");
+    if (mode != EGS_PRECISION_FP64 && !egs::update192_supported(p->ts))
+        return fail(EGS_ERR_UNSUPPORTED, "mixed precision needs a dense tile size that is a multiple of 192 (got %d)",
+                    p->ts);
+    EGS_HIP(hipStreamSynchronize(p->ctx->stream));
     p->mode = mode;
     p->band64 = band_fp64;
     p->band32 = band_fp32;
-    return layout_tiles(p);
+    int st2 = layout_tiles(p);
+    if (st2) return st2;
+    return is_mixed(p) ? setup_mixed(p) : EGS_SUCCESS;
 }
 
 int egs_problem_set_observations(egs_problem_t p, const double *z, int64_t n) {
@@ -691,6 +911,10 @@ int egs_solve_logdet_quad(egs_problem_t p, double *logdet, double *quad) {
     double ld, q;
     st = read_partials(p, ld, q);
     if (st) return st;
+    if (is_mixed(p)) {
+        if ((st = refine_solve(p, p->d_obs))) return st;
+        if ((st = host_dot(p, p->d_obs, p->d_cg, q))) return st;
+    }
     if (logdet) *logdet = ld;
     if (quad) *quad = q;
     return EGS_SUCCESS;
@@ -728,6 +952,18 @@ int egs_loglik(egs_problem_t p, const double *theta, double *loglik, double *log
     double ld, q;
     st = read_partials(p, ld, q);
     if (st) return st;
+    if (is_mixed(p)) {
+        // z' Sigma^{-1} z from the refined FP64 solution instead of ||L^{-1} z||^2
+        st = refine_solve(p, p->d_obs);
+        if (st) return st;
+        st = host_dot(p, p->d_obs, p->d_cg, q);
+        if (st) return st;
+        EGS_HIP(hipEventRecord(p->ev[3], s));
+        EGS_HIP(hipStreamSynchronize(s));
+        float ms2 = 0;
+        hipEventElapsedTime(&ms2, p->ev[2], p->ev[3]);
+        p->t_solve = ms2;
+    }
     const double n = (double) p->n;
     *loglik = -0.5 * q - 0.5 * ld - (double) (n / 2.0) * std::log(2.0 * kPi);
     if (logdet) *logdet = ld;
@@ -753,6 +989,14 @@ int egs_problem_kernel_stats(egs_problem_t p, int kernel, int64_t *launches, dou
     if (launches) *launches = k.launches;
     if (total_ms) *total_ms = k.ms;
     if (flops) *flops = k.flops;
+    return EGS_SUCCESS;
+}
+
+int egs_last_refinement(egs_problem_t p, int *iterations, double *residual) {
+    int st = check_problem(p);
+    if (st) return st;
+    if (iterations) *iterations = p->refine_iters;
+    if (residual) *residual = p->refine_resid;
     return EGS_SUCCESS;
 }
 
@@ -876,9 +1120,16 @@ int egs_predict(egs_problem_t p, const double *theta, int64_t n_miss, const doub
         st = run_potrf(p);
         if (st) return st;
     }
-    egs::launch_copy_pad(p->d_work, p->d_obs, p->n, p->npad, s);
-    run_forward(p);
-    run_backward(p);
+    const double *weights = p->d_work;
+    if (is_mixed(p)) {
+        st = refine_solve(p, p->d_obs);
+        if (st) return st;
+        weights = p->d_cg;
+    } else {
+        egs::launch_copy_pad(p->d_work, p->d_obs, p->n, p->npad, s);
+        run_forward(p);
+        run_backward(p);
+    }
     egs::MaternParams prm;
     st = make_params(p->kernel, p->metric, theta, prm);
     if (st) return st;
@@ -891,7 +1142,7 @@ int egs_predict(egs_problem_t p, const double *theta, int64_t n_miss, const doub
     if (e == hipSuccess && three_d) e = hipMemcpyAsync(dzm, zm, sizeof(double) * n_miss, hipMemcpyHostToDevice, s);
     if (e == hipSuccess) {
         egs::launch_cov_gemv(d_out, rows, p->n, prm, dxm, dym, three_d ? dzm : nullptr, p->d_x, p->d_y,
-                             three_d ? p->d_z : nullptr, p->d_work, s);
+                             three_d ? p->d_z : nullptr, weights, s);
         e = hipGetLastError();
     }
     if (e == hipSuccess) e = hipMemcpyAsync(zmiss, d_out, sizeof(double) * (size_t) rows, hipMemcpyDeviceToHost, s);

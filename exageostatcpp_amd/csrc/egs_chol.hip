@@ -42,7 +42,8 @@ __device__ __forceinline__ void acc_zero(Acc64 &acc) {
 }
 
 // acc += A (64 x K, column-major, lda) * B (64 x K, column-major, ldb)^T.
-__device__ void wg_gemm_nt(Acc64 &acc, const double *__restrict__ A, int64_t lda, const double *__restrict__ B,
+template <typename TA>
+__device__ void wg_gemm_nt(Acc64 &acc, const TA *__restrict__ A, int64_t lda, const double *__restrict__ B,
                            int64_t ldb, int K, GemmSmem &sm) {
     const int t = threadIdx.x;
     const int lane = t & 63;
@@ -53,7 +54,7 @@ __device__ void wg_gemm_nt(Acc64 &acc, const double *__restrict__ A, int64_t lda
 #pragma unroll
         for (int e = t; e < kBlk * kKc; e += kThreads) {
             const int m = e & 63, kk = e >> 6;
-            sm.a[kk][m] = A[m + (int64_t) (k0 + kk) * lda];
+            sm.a[kk][m] = (double) A[m + (int64_t) (k0 + kk) * lda];
             sm.b[kk][m] = B[m + (int64_t) (k0 + kk) * ldb];
         }
         __syncthreads();
@@ -153,6 +154,7 @@ __global__ void __launch_bounds__(kThreads) potrf_block_kernel(TileMap map, int 
 // ------------------------------------------------------------------------
 // Row-block triangular solve against L_kk, column blocks [jb0, jb1).
 // ------------------------------------------------------------------------
+template <typename TX>
 __global__ void __launch_bounds__(kThreads) trsm_rows_kernel(TileMap map, int k, int64_t g0, int jb0, int jb1,
                                                              const double *winv_k) {
     __shared__ GemmSmem sm;
@@ -162,7 +164,7 @@ __global__ void __launch_bounds__(kThreads) trsm_rows_kernel(TileMap map, int k,
     const int64_t g = g0 + blockIdx.x;
     const int i = (int) (g / nsb);
     const int rb = (int) (g % nsb);
-    double *X = tile_f64(map, i, k) + rb * kBlk;
+    TX *X = reinterpret_cast<TX *>(map.base + map.off[i * map.nt + k]) + rb * kBlk;
     const double *L = tile_f64(map, k, k);
     const int lane = threadIdx.x & 63;
     const int w = threadIdx.x >> 6;
@@ -173,8 +175,8 @@ __global__ void __launch_bounds__(kThreads) trsm_rows_kernel(TileMap map, int k,
         Acc64 acc;
         acc_zero(acc);
         if (jb > 0) wg_gemm_nt(acc, X, ts, L + jb * kBlk, ts, jb * kBlk, sm);
-        double *Xj = X + (int64_t) jb * kBlk * ts;
-        acc_foreach(acc, [&](int r, int c, double v) { sT[r][c] = Xj[r + (int64_t) c * ts] - v; });
+        TX *Xj = X + (int64_t) jb * kBlk * ts;
+        acc_foreach(acc, [&](int r, int c, double v) { sT[r][c] = (double) Xj[r + (int64_t) c * ts] - v; });
         __syncthreads();
         // X_j = T W_j^T :  acc2[r][c] = sum_l T[r][l] W[c][l]
         const double *W = winv_k + (int64_t) jb * kBlk * kBlk;
@@ -191,7 +193,7 @@ __global__ void __launch_bounds__(kThreads) trsm_rows_kernel(TileMap map, int k,
             acc2.v[1][0] = mfma_f64(b1, a0, acc2.v[1][0]);
             acc2.v[1][1] = mfma_f64(b1, a1, acc2.v[1][1]);
         }
-        acc_foreach(acc2, [&](int r, int c, double v) { Xj[r + (int64_t) c * ts] = v; });
+        acc_foreach(acc2, [&](int r, int c, double v) { Xj[r + (int64_t) c * ts] = (TX) v; });
         __syncthreads();
     }
 }
@@ -226,9 +228,12 @@ void launch_potrf_block(const TileMap &map, int k, int jb, double *winv_k, int *
 }
 
 void launch_trsm_rows(const TileMap &map, int k, int64_t g0, int count, int jb0, int jb1, const double *winv_k,
-                      hipStream_t s) {
+                      hipStream_t s, bool x_f32) {
     if (count <= 0 || jb1 <= jb0) return;
-    hipLaunchKernelGGL(trsm_rows_kernel, dim3(count), dim3(kThreads), 0, s, map, k, g0, jb0, jb1, winv_k);
+    if (x_f32)
+        hipLaunchKernelGGL(trsm_rows_kernel<float>, dim3(count), dim3(kThreads), 0, s, map, k, g0, jb0, jb1, winv_k);
+    else
+        hipLaunchKernelGGL(trsm_rows_kernel<double>, dim3(count), dim3(kThreads), 0, s, map, k, g0, jb0, jb1, winv_k);
 }
 
 void launch_update(const TileMap &map, const int2 *pairs, int npairs, int k, hipStream_t s) {

@@ -27,7 +27,7 @@ void launch_potrf_block(const TileMap &map, int k, int jb, double *winv_k, int *
 // (global 64-row block g lives in tile g / nsb at row (g % nsb) * 64), column
 // blocks [jb0, jb1).
 void launch_trsm_rows(const TileMap &map, int k, int64_t g0, int count, int jb0, int jb1, const double *winv_k,
-                      hipStream_t s);
+                      hipStream_t s, bool x_f32 = false);
 // Trailing update C_ij -= L_ik L_jk^T for the first npairs lower tiles of the
 // update order (all (i, j) with i >= j > k).
 void launch_update(const TileMap &map, const int2 *pairs, int npairs, int k, hipStream_t s);
@@ -52,6 +52,27 @@ void launch_trmv(const TileMap &map, const double *e, double *out, hipStream_t s
 void launch_cov_gemv(double *out, int64_t rows, int64_t cols, const MaternParams &prm, const double *x1,
                      const double *y1, const double *z1, const double *x2, const double *y2, const double *z2,
                      const double *xv, hipStream_t s);
+// partial[cb * npad + r] = sum over column chunk cb of Sigma(r, c) v[c] (FP64,
+// regenerated from the locations); matvec_chunks(n) chunks.
+void launch_cov_matvec(double *partial, const double *v, int64_t n, int64_t npad, const MaternParams &prm,
+                       const double *x, const double *y, const double *z, hipStream_t s);
+int matvec_chunks(int64_t n);
+// out = base + scale * (sum of the chunk partials), padding rows set to zero.
+void launch_matvec_reduce(double *out, const double *base, double scale, const double *partial, int nchunks,
+                          int64_t n, int64_t npad, hipStream_t s);
+// partial[b] (b < dot_blocks()) = fixed-order partial sums of x . y.
+int dot_blocks();
+void launch_dot(const double *x, const double *y, int64_t n, double *partial, hipStream_t s);
+// y = a x + b y
+void launch_axpby(double *y, const double *x, double a, double b, int64_t n, hipStream_t s);
+// Mixed-precision factorization pieces (egs_mixed.hip).
+void launch_panel_convert(const TileMap &map, int k, double *p64, float *p32, float *p32t, hipStream_t s);
+void launch_update192_f32(const TileMap &map, const float *p32, const int2 *pairs, int npairs, int k, hipStream_t s);
+void launch_update192_bf16(const TileMap &map, const float *p32t, const int2 *pairs, int npairs, int k, bool split,
+                           hipStream_t s);
+// FP64 update reading panel k from a contiguous FP64 copy (tile i at p64 + (i-k-1) ts^2).
+void launch_update192_panel(const TileMap &map, const double *p64, const int2 *pairs, int npairs, int k,
+                            hipStream_t s);
 // e = LAPACK dlarnv(3, iseed) N(0,1) values (48-bit LCG stream, Box-Muller).
 void launch_dlarnv_normal(double *e, int64_t n, uint64_t seed48, hipStream_t s);
 // dst = src (n doubles), padding [n, npad) with zeros.

@@ -93,9 +93,14 @@ __global__ void __launch_bounds__(kThreads) trsv_panel_kernel(TileMap map, int k
     const double *yk = z + (int64_t) k * ts;
     for (int e = t; e < ts; e += kThreads) sy[e] = yk[e];
     __syncthreads();
-    const double *A = tile_f64(map, i, k) + rb * kBlk;
     double s = 0.0;
-    for (int c = g; c < ts; c += 4) s += A[r + (int64_t) c * ts] * sy[c];
+    if (map.store[i * map.nt + k] == kStoreF32) {
+        const float *A = tile_f32(map, i, k) + rb * kBlk;
+        for (int c = g; c < ts; c += 4) s += (double) A[r + (int64_t) c * ts] * sy[c];
+    } else {
+        const double *A = tile_f64(map, i, k) + rb * kBlk;
+        for (int c = g; c < ts; c += 4) s += A[r + (int64_t) c * ts] * sy[c];
+    }
     sp[g][r] = s;
     __syncthreads();
     if (t < kBlk) {
@@ -117,12 +122,18 @@ __global__ void __launch_bounds__(kThreads) trsv_t_partial_kernel(TileMap map, i
     const double *yi = z + (int64_t) i * ts;
     for (int e = t; e < ts; e += kThreads) sy[e] = yi[e];
     __syncthreads();
+    const bool f32 = map.store[i * map.nt + k] == kStoreF32;
     const double *A = tile_f64(map, i, k);
+    const float *Af = tile_f32(map, i, k);
     // wave w reduces columns cb*64 + w*16 .. +15, lanes stride the rows
     for (int cc = 0; cc < 16; ++cc) {
         const int c = cb * kBlk + w * 16 + cc;
         double s = 0.0;
-        for (int r0 = lane; r0 < ts; r0 += 64) s += A[r0 + (int64_t) c * ts] * sy[r0];
+        if (f32) {
+            for (int r0 = lane; r0 < ts; r0 += 64) s += (double) Af[r0 + (int64_t) c * ts] * sy[r0];
+        } else {
+            for (int r0 = lane; r0 < ts; r0 += 64) s += A[r0 + (int64_t) c * ts] * sy[r0];
+        }
         // fixed-order butterfly over the 64 lanes
         for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
         if (lane == 0) scratch[(int64_t) (i - k - 1) * ts + c] = s;
@@ -183,13 +194,18 @@ __global__ void __launch_bounds__(kThreads) trmv_kernel(TileMap map, const doubl
     const int r = t & 63, g = t >> 6;
     double s = 0.0;
     for (int k = 0; k <= i; ++k) {
-        const double *A = tile_f64(map, i, k) + rb * kBlk;
         const double *ek = e + (int64_t) k * ts;
         const int cend = (k < i) ? ts : (rb + 1) * kBlk;
-        for (int c = g; c < cend; c += 4) {
-            const int row_in_tile = rb * kBlk + r;
-            if (k == i && c > row_in_tile) continue;
-            s += A[r + (int64_t) c * ts] * ek[c];
+        const int row_in_tile = rb * kBlk + r;
+        if (map.store[i * map.nt + k] == kStoreF32) {
+            const float *A = tile_f32(map, i, k) + rb * kBlk;
+            for (int c = g; c < cend; c += 4) s += (double) A[r + (int64_t) c * ts] * ek[c];
+        } else {
+            const double *A = tile_f64(map, i, k) + rb * kBlk;
+            for (int c = g; c < cend; c += 4) {
+                if (k == i && c > row_in_tile) continue;
+                s += A[r + (int64_t) c * ts] * ek[c];
+            }
         }
     }
     sp[g][r] = s;
@@ -244,6 +260,130 @@ __global__ void __launch_bounds__(kThreads) dlarnv_normal_kernel(double *e, int6
 __global__ void __launch_bounds__(kThreads) copy_pad_kernel(double *dst, const double *src, int64_t n, int64_t npad) {
     const int64_t i = (int64_t) blockIdx.x * kThreads + threadIdx.x;
     if (i < npad) dst[i] = (i < n) ? src[i] : 0.0;
+}
+
+// ------------------------------------------------------------------------
+// Matrix-free Sigma v in FP64 for the refinement residual: the covariance is
+// regenerated from the locations, never stored.  Workgroup (row block rb,
+// column chunk cb) writes partial[cb * npad + row] = sum over the chunk's
+// columns; reduce kernels add the chunks in a fixed order.
+// ------------------------------------------------------------------------
+constexpr int kChunk = 2048;
+
+struct MatvecArgs {
+    double *partial;
+    const double *v;
+    int64_t n, npad;
+    MaternParams prm;
+    Locs loc;
+    int three_d;
+};
+
+__global__ void __launch_bounds__(kThreads) cov_matvec_kernel(MatvecArgs a) {
+    __shared__ double sx[kChunk], sy[kChunk], sz[kChunk], sv[kChunk];
+    __shared__ double sp[4][kBlk];
+    const int t = threadIdx.x;
+    const int r = t & 63, g = t >> 6;
+    const int64_t row = (int64_t) blockIdx.x * kBlk + r;
+    const int64_t c0 = (int64_t) blockIdx.y * kChunk;
+    const int shift = a.prm.kernel == 1 ? 1 : 0;
+    const bool three_d = a.three_d != 0;
+    for (int e = t; e < kChunk; e += kThreads) {
+        const int64_t c = c0 + e;
+        const bool ok = c < a.n;
+        const int64_t l = c >> shift;
+        sx[e] = ok ? a.loc.x[l] : 0.0;
+        sy[e] = ok ? a.loc.y[l] : 0.0;
+        sz[e] = (ok && three_d) ? a.loc.z[l] : 0.0;
+        sv[e] = ok ? a.v[c] : 0.0;
+    }
+    __syncthreads();
+    double s = 0.0;
+    if (row < a.n) {
+        const int64_t li = row >> shift;
+        const double rx = a.loc.x[li], ry = a.loc.y[li], rz = three_d ? a.loc.z[li] : 0.0;
+        const int rvar = (int) (row & 1);
+        const int cend = (int) min<int64_t>(kChunk, a.n - c0);
+        for (int e = g; e < cend; e += 4) {
+            const int64_t c = c0 + e;
+            const int pair = shift ? (rvar * 2 + (int) (c & 1)) : 0;
+            const double d = distance(rx, ry, rz, sx[e], sy[e], sz[e], a.prm.metric, three_d) / a.prm.range;
+            const double cov = (d == 0.0) ? a.prm.sig[pair]
+                                          : matern_scaled(d, a.prm.con[pair], a.prm.sig[pair], a.prm.nu[pair]);
+            s += cov * sv[e];
+        }
+    }
+    sp[g][r] = s;
+    __syncthreads();
+    if (t < kBlk) {
+        const int64_t rr = (int64_t) blockIdx.x * kBlk + t;
+        a.partial[(int64_t) blockIdx.y * a.npad + rr] = ((sp[0][t] + sp[1][t]) + sp[2][t]) + sp[3][t];
+    }
+}
+
+// out[i] = base[i] + scale * sum_cb partial[cb][i]  (fixed chunk order), for i < npad.
+__global__ void __launch_bounds__(kThreads) matvec_reduce_kernel(double *out, const double *base, double scale,
+                                                                 const double *partial, int nchunks, int64_t n,
+                                                                 int64_t npad) {
+    const int64_t i = (int64_t) blockIdx.x * kThreads + threadIdx.x;
+    if (i >= npad) return;
+    double s = 0.0;
+    for (int cb = 0; cb < nchunks; ++cb) s += partial[(int64_t) cb * npad + i];
+    const double b = base ? base[i] : 0.0;
+    out[i] = (i < n) ? b + scale * s : 0.0;
+}
+
+// Fixed-order dot products: block b sums indices b, b + G, b + 2G, ... then the
+// host adds the G block sums in order.
+constexpr int kDotBlocks = 256;
+
+__global__ void __launch_bounds__(kThreads) dot_kernel(const double *x, const double *y, int64_t n, double *partial) {
+    __shared__ double red[kThreads];
+    double s = 0.0;
+    const int64_t stride = (int64_t) kDotBlocks * kThreads;
+    for (int64_t i = (int64_t) blockIdx.x * kThreads + threadIdx.x; i < n; i += stride) s += x[i] * y[i];
+    s = wg_sum(s, red);
+    if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
+// y = a * x + b * y
+__global__ void __launch_bounds__(kThreads) axpby_kernel(double *y, const double *x, double a, double b, int64_t n) {
+    const int64_t i = (int64_t) blockIdx.x * kThreads + threadIdx.x;
+    if (i < n) y[i] = a * x[i] + b * y[i];
+}
+
+void launch_cov_matvec(double *partial, const double *v, int64_t n, int64_t npad, const MaternParams &prm,
+                       const double *x, const double *y, const double *z, hipStream_t s) {
+    MatvecArgs a;
+    a.partial = partial;
+    a.v = v;
+    a.n = n;
+    a.npad = npad;
+    a.prm = prm;
+    a.loc = Locs{x, y, z};
+    a.three_d = z != nullptr;
+    const unsigned rows = (unsigned) (npad / kBlk);
+    const unsigned chunks = (unsigned) ((n + kChunk - 1) / kChunk);
+    hipLaunchKernelGGL(cov_matvec_kernel, dim3(rows, chunks), dim3(kThreads), 0, s, a);
+}
+
+int matvec_chunks(int64_t n) { return (int) ((n + kChunk - 1) / kChunk); }
+
+void launch_matvec_reduce(double *out, const double *base, double scale, const double *partial, int nchunks,
+                          int64_t n, int64_t npad, hipStream_t s) {
+    hipLaunchKernelGGL(matvec_reduce_kernel, dim3((unsigned) ((npad + kThreads - 1) / kThreads)), dim3(kThreads), 0,
+                       s, out, base, scale, partial, nchunks, n, npad);
+}
+
+int dot_blocks() { return kDotBlocks; }
+
+void launch_dot(const double *x, const double *y, int64_t n, double *partial, hipStream_t s) {
+    hipLaunchKernelGGL(dot_kernel, dim3(kDotBlocks), dim3(kThreads), 0, s, x, y, n, partial);
+}
+
+void launch_axpby(double *y, const double *x, double a, double b, int64_t n, hipStream_t s) {
+    hipLaunchKernelGGL(axpby_kernel, dim3((unsigned) ((n + kThreads - 1) / kThreads)), dim3(kThreads), 0, s, y, x, a,
+                       b, n);
 }
 
 void launch_trsv_diag(const TileMap &map, int k, const double *winv_k, double *z, double *partials, hipStream_t s) {

@@ -162,4 +162,13 @@ void launch_update192(const TileMap &map, const int2 *pairs, int npairs, int k, 
     hipLaunchKernelGGL(update192_kernel, dim3(blocks), dim3(kNT), 0, s, map, pr, pairs, k);
 }
 
+void launch_update192_panel(const TileMap &map, const double *p64, const int2 *pairs, int npairs, int k,
+                            hipStream_t s) {
+    if (npairs <= 0) return;
+    const int nsb = map.ts / kBM;
+    const unsigned blocks = (unsigned) npairs * (unsigned) (nsb * nsb);
+    PanelRef pr{reinterpret_cast<const char *>(p64), (int64_t) map.ts * map.ts * (int64_t) sizeof(double), 1};
+    hipLaunchKernelGGL(update192_kernel, dim3(blocks), dim3(kNT), 0, s, map, pr, pairs, k);
+}
+
 }  // namespace egs

@@ -39,9 +39,13 @@ extern "C" {
 #define EGS_METRIC_EUCLIDEAN 0
 #define EGS_METRIC_GREAT_CIRCLE 1
 
-/* Precision plans for the factorization. */
+/* Precision plans for the factorization.  MIXED: FP64 near the diagonal,
+ * FP32 further out, and beyond that BF16 MFMA products of split operands
+ * (hi*hi + hi*lo + lo*hi, FP32 accumulation).  MIXED_BF16: the same with a
+ * single BF16 product.  Both refine z'Sigma^{-1}z with FP64 residuals. */
 #define EGS_PRECISION_FP64 0
 #define EGS_PRECISION_MIXED 1
+#define EGS_PRECISION_MIXED_BF16 2
 
 typedef struct egs_context_s *egs_context_t;
 typedef struct egs_problem_s *egs_problem_t;
@@ -83,10 +87,15 @@ int egs_problem_destroy(egs_problem_t problem);
 /* Device bytes the problem holds. */
 int egs_problem_device_bytes(egs_problem_t problem, int64_t *bytes);
 
-/* Precision plan: EGS_PRECISION_FP64, or EGS_PRECISION_MIXED with tiles whose
- * tile distance |i - j| <= band_fp64 in FP64, <= band_fp32 in FP32 and the
- * rest in BF16 products accumulated in FP32. */
+/* Precision plan: EGS_PRECISION_FP64, or a mixed plan with tiles whose tile
+ * distance |i - j| <= band_fp64 in FP64, <= band_fp32 in FP32 and the rest in
+ * BF16 products accumulated in FP32 (needs a tile size that is a multiple of
+ * 192).  Under a mixed plan the factor's diagonal gives log|Sigma| and
+ * z'Sigma^{-1}z comes from conjugate gradients preconditioned by the factor,
+ * with residuals b - Sigma x evaluated in FP64 until ||r|| <= 1e-12 ||b||. */
 int egs_problem_set_precision(egs_problem_t problem, int mode, int band_fp64, int band_fp32);
+/* Iterations and final relative FP64 residual of the last refinement. */
+int egs_last_refinement(egs_problem_t problem, int *iterations, double *residual);
 
 /* Observations Z (length n_loc * P): the measurements descriptor Z and its
  * restore copy Z_COPY of ChameleonImplementation.cpp:113-125. */
@@ -133,7 +142,10 @@ int egs_last_timings(egs_problem_t problem, double *t_covariance, double *t_fact
 #define EGS_PROFILE_PANEL_TRSM 2    /* panel triangular solve, one launch per step */
 #define EGS_PROFILE_DIAGONAL 3      /* diagonal-tile factorization (all its launches) */
 #define EGS_PROFILE_SOLVE 4         /* forward solve with log-det and dot product */
-#define EGS_PROFILE_KERNELS 5
+#define EGS_PROFILE_UPDATE_F32 5    /* mixed plans: FP32-band update */
+#define EGS_PROFILE_UPDATE_BF16 6   /* mixed plans: BF16-band update */
+#define EGS_PROFILE_REFINE 7        /* mixed plans: refinement of z'Sigma^{-1}z */
+#define EGS_PROFILE_KERNELS 8
 int egs_problem_profile(egs_problem_t problem, int enable);
 int egs_problem_kernel_stats(egs_problem_t problem, int kernel, int64_t *launches, double *total_ms, double *flops);
 

@@ -1,0 +1,96 @@
+"""Mixed-precision plans (FP64 band / FP32 band / BF16 products) against the
+FP64 oracle, through the C ABI."""
+import sys
+
+import numpy as np
+import pytest
+
+import exageostatcpp_amd as egs
+from oracle import orc
+
+pytestmark = pytest.mark.gpu
+
+
+def log(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = egs.Context(0)
+    yield c
+    c.close()
+
+
+@pytest.fixture(scope="module")
+def data4k():
+    # 21 tiles of 192: every band class (FP64 <= 2, FP32 3..10, BF16 > 10) occurs
+    x, y = orc.generate_locations(4000, seed=0)
+    th = [1.0, 0.05, 0.5]
+    z = orc.generate_observations(0, (x, y), th, 192, seed=0)
+    ref = orc.loglik(0, (x, y), th, z, 192)
+    return x, y, th, z, ref
+
+
+def test_mixed_loglik_within_1e6(ctx, data4k):
+    x, y, th, z, ref = data4k
+    log("mixed (split BF16) loglik n=4000 dts=192")
+    p = egs.Problem(ctx, 0, x, y, dts=192)
+    p.set_observations(z)
+    p.set_precision(egs.PRECISION_MIXED, 2, 10)
+    llh, ld, q = p.loglik(th)
+    rf = p.refinement()
+    rel = abs(llh - ref["llh"]) / abs(ref["llh"])
+    log("  llh %.10f oracle %.10f rel %.2e logdet err %.2e quad err %.2e refine %s" %
+        (llh, ref["llh"], rel, ld - ref["logdet"], q - ref["quad"], rf))
+    assert rf["residual"] <= 1e-12
+    assert abs(q - ref["quad"]) <= 1e-9 * abs(ref["quad"])
+    assert rel <= 1e-6
+
+
+def test_mixed_plain_bf16_refines_quad(ctx, data4k):
+    x, y, th, z, ref = data4k
+    log("mixed (plain BF16) loglik n=4000 dts=192")
+    p = egs.Problem(ctx, 0, x, y, dts=192)
+    p.set_observations(z)
+    p.set_precision(egs.PRECISION_MIXED_BF16, 2, 10)
+    llh, ld, q = p.loglik(th)
+    rf = p.refinement()
+    log("  llh %.10f oracle %.10f rel %.2e refine %s" % (llh, ref["llh"], abs(llh - ref["llh"]) / abs(ref["llh"]), rf))
+    assert rf["residual"] <= 1e-12
+    assert abs(q - ref["quad"]) <= 1e-9 * abs(ref["quad"])
+    assert abs(llh - ref["llh"]) <= 1e-3 * abs(ref["llh"])
+
+
+def test_mixed_fp64_only_band_equals_fp64(ctx):
+    # bands that cover every tile give the FP64 factorization
+    x, y = orc.generate_locations(1500, seed=3)
+    th = [1.0, 0.1, 1.0]
+    z = orc.generate_observations(0, (x, y), th, 192, seed=1)
+    ref = orc.loglik(0, (x, y), th, z, 192)
+    p = egs.Problem(ctx, 0, x, y, dts=192)
+    p.set_observations(z)
+    p.set_precision(egs.PRECISION_MIXED, 100, 100)
+    llh, _, _ = p.loglik(th)
+    assert abs(llh - ref["llh"]) <= 1e-10 * abs(ref["llh"])
+
+
+def test_mixed_kriging_matches_oracle(ctx, data4k):
+    x, y, th, z, _ = data4k
+    log("mixed kriging")
+    rng = np.random.default_rng(2)
+    xm, ym = rng.random(40), rng.random(40)
+    p = egs.Problem(ctx, 0, x, y, dts=192)
+    p.set_observations(z)
+    p.set_precision(egs.PRECISION_MIXED, 2, 10)
+    zm, _ = p.predict(th, xm, ym)
+    zo, _ = orc.krige(0, (x, y), (xm, ym), th, z, 192)
+    assert np.max(np.abs(zm - zo)) < 1e-8
+
+
+def test_mixed_needs_tile_multiple_of_192(ctx):
+    x, y = orc.generate_locations(500, seed=1)
+    p = egs.Problem(ctx, 0, x, y, dts=320)
+    with pytest.raises(egs.EGSError, match="multiple of 192"):
+        p.set_precision(egs.PRECISION_MIXED, 2, 10)
