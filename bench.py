@@ -117,7 +117,19 @@ def main():
     ap.add_argument("--plain-bf16", action="store_true", help="single BF16 product instead of the split one")
     ap.add_argument("--cpu-sample", type=int, default=6000)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only run a 1 GiB device-to-device copy (calibrates FETCH_SIZE/WRITE_SIZE counters)")
     args = ap.parse_args()
+    if args.copy_probe:
+        import torch
+
+        a = torch.ones(2 ** 28, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        b.copy_(a)
+        torch.cuda.synchronize()
+        log("copy probe: 1 GiB read, 1 GiB written")
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU: the likelihood path is built and timed on a single MI355X; "
               "--gpus %d is not supported" % args.gpus, file=sys.stderr)

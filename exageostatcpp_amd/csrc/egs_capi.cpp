@@ -85,7 +85,8 @@ struct egs_problem_s {
     // slot k % 2: with the lookahead the critical stream writes panel k+1 while
     // the bulk stream still reads panel k
     double *d_p64[2] = {nullptr, nullptr};
-    float *d_p32[2] = {nullptr, nullptr}, *d_p32t[2] = {nullptr, nullptr};
+    float *d_p32[2] = {nullptr, nullptr};
+    __bf16 *d_p16h[2] = {nullptr, nullptr}, *d_p16l[2] = {nullptr, nullptr};
     double *d_cg = nullptr;                  // refinement vectors x, r, p, q, w (npad each)
     double *d_mv = nullptr;                  // matrix-vector chunk partials
     double *d_dot = nullptr;                 // dot-product block partials
@@ -214,9 +215,11 @@ void free_problem_buffers(egs_problem_s *p) {
     for (int b = 0; b < 2; ++b) {
         hipFree(p->d_p64[b]);
         hipFree(p->d_p32[b]);
-        hipFree(p->d_p32t[b]);
+        hipFree(p->d_p16h[b]);
+        hipFree(p->d_p16l[b]);
         p->d_p64[b] = nullptr;
-        p->d_p32[b] = p->d_p32t[b] = nullptr;
+        p->d_p32[b] = nullptr;
+        p->d_p16h[b] = p->d_p16l[b] = nullptr;
     }
     hipFree(p->d_cg);
     hipFree(p->d_mv);
@@ -328,8 +331,10 @@ int setup_mixed(egs_problem_s *p) {
             hipMalloc(&p->d_p64[1], panel * sizeof(double)) != hipSuccess ||
             hipMalloc(&p->d_p32[0], panel * sizeof(float)) != hipSuccess ||
             hipMalloc(&p->d_p32[1], panel * sizeof(float)) != hipSuccess ||
-            hipMalloc(&p->d_p32t[0], panel * sizeof(float)) != hipSuccess ||
-            hipMalloc(&p->d_p32t[1], panel * sizeof(float)) != hipSuccess ||
+            hipMalloc(&p->d_p16h[0], panel * sizeof(__bf16)) != hipSuccess ||
+            hipMalloc(&p->d_p16h[1], panel * sizeof(__bf16)) != hipSuccess ||
+            hipMalloc(&p->d_p16l[0], panel * sizeof(__bf16)) != hipSuccess ||
+            hipMalloc(&p->d_p16l[1], panel * sizeof(__bf16)) != hipSuccess ||
             hipMalloc(&p->d_cg, 5 * sizeof(double) * (size_t) p->npad) != hipSuccess ||
             hipMalloc(&p->d_mv, sizeof(double) * (size_t) egs::matvec_chunks(p->n) * p->npad) != hipSuccess ||
             hipMalloc(&p->d_dot, sizeof(double) * egs::dot_blocks()) != hipSuccess)
@@ -428,7 +433,8 @@ void factor_panel(egs_problem_s *p, int k) {
                                       true);
         }
         prof_end(p, pi);
-        if (is_mixed(p)) egs::launch_panel_convert(m, k, p->d_p64[k & 1], p->d_p32[k & 1], p->d_p32t[k & 1], s);
+        if (is_mixed(p))
+            egs::launch_panel_convert(m, k, p->d_p64[k & 1], p->d_p32[k & 1], p->d_p16h[k & 1], p->d_p16l[k & 1], s);
     }
 }
 
@@ -463,7 +469,9 @@ void launch_mixed_update(egs_problem_s *p, int k, int jcol_lo, int jcol_hi, hipS
         const int slot = k & 1;
         if (c == 0) egs::launch_update192_panel(m, p->d_p64[slot], pr, end - beg, k, st);
         else if (c == 1) egs::launch_update192_f32(m, p->d_p32[slot], pr, end - beg, k, st);
-        else egs::launch_update192_bf16(m, p->d_p32t[slot], pr, end - beg, k, p->mode == EGS_PRECISION_MIXED, st);
+        else
+            egs::launch_update192_bf16(m, p->d_p16h[slot], p->d_p16l[slot], pr, end - beg, k,
+                                       p->mode == EGS_PRECISION_MIXED, st);
         prof_end(p, pi, st);
     }
 }
@@ -475,7 +483,13 @@ void launch_mixed_update(egs_problem_s *p, int k, int jcol_lo, int jcol_hi, hipS
 // it.  The caller checks d_info after synchronizing the critical stream, which
 // ends waiting for the bulk stream.
 int run_potrf(egs_problem_s *p) {
-    hipStream_t s = p->ctx->stream, b = p->ctx->bulk;
+    // EGS_NO_LOOKAHEAD=1 runs every launch on one stream (for measuring kernels
+    // without the overlap of the two streams).
+    static const bool no_lookahead = [] {
+        const char *v = std::getenv("EGS_NO_LOOKAHEAD");
+        return v && v[0] == '1';
+    }();
+    hipStream_t s = p->ctx->stream, b = no_lookahead ? p->ctx->stream : p->ctx->bulk;
     const egs::TileMap m = p->map();
     const int nt = p->nt;
     for (int k = (int) p->ev_panel.size(); k < nt; ++k) {

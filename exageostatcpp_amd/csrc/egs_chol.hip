@@ -14,18 +14,21 @@
 //     workgroup per 64 x 64 output block with K = ts.
 // Every contraction uses v_mfma_f64_16x16x4f64 through wg_gemm_nt: four waves
 // each own a 32 x 32 quadrant of the 64 x 64 output; K is staged through LDS in
-// chunks of 16 columns.
+// chunks of 32 columns, double buffered, with the next chunk's global loads
+// issued before the current chunk's MFMAs (these kernels run on few
+// workgroups, so each one has to hide its own memory latency).
 #include "egs_common.hpp"
 #include "egs_kernels.hpp"
 
 namespace egs {
 
-constexpr int kKc = 16;          // K chunk staged in LDS
+constexpr int kKc = 32;          // K chunk staged in LDS
 constexpr int kLdsPad = 65;      // LDS row stride (doubles) for 64-wide panels
+constexpr int kStage = kBlk * kKc / kThreads;   // elements of A (and of B) per thread per chunk
 
 struct GemmSmem {
-    double a[kKc][kLdsPad];
-    double b[kKc][kLdsPad];
+    double a[2][kKc][kLdsPad];
+    double b[2][kKc][kLdsPad];
 };
 
 // acc[ni][mi][v] holds C[mq*32 + mi*16 + lane%16][nq*32 + ni*16 + lane/16 + 4*v]
@@ -42,6 +45,7 @@ __device__ __forceinline__ void acc_zero(Acc64 &acc) {
 }
 
 // acc += A (64 x K, column-major, lda) * B (64 x K, column-major, ldb)^T.
+// K must be a multiple of kKc.
 template <typename TA>
 __device__ void wg_gemm_nt(Acc64 &acc, const TA *__restrict__ A, int64_t lda, const double *__restrict__ B,
                            int64_t ldb, int K, GemmSmem &sm) {
@@ -50,24 +54,53 @@ __device__ void wg_gemm_nt(Acc64 &acc, const TA *__restrict__ A, int64_t lda, co
     const int w = t >> 6;
     const int mq = w & 1, nq = w >> 1;
     const int lr = lane & 15, lk = lane >> 4;
-    for (int k0 = 0; k0 < K; k0 += kKc) {
+    // element e = t + q * 256 of a 64 x kKc chunk: row e % 64, column e / 64
+    const int srow = t & 63;
+    double ra[kStage], rb[kStage];
 #pragma unroll
-        for (int e = t; e < kBlk * kKc; e += kThreads) {
-            const int m = e & 63, kk = e >> 6;
-            sm.a[kk][m] = (double) A[m + (int64_t) (k0 + kk) * lda];
-            sm.b[kk][m] = B[m + (int64_t) (k0 + kk) * ldb];
+    for (int q = 0; q < kStage; ++q) {
+        const int col = (t >> 6) + q * (kThreads / kBlk);
+        ra[q] = (double) A[srow + (int64_t) col * lda];
+        rb[q] = B[srow + (int64_t) col * ldb];
+    }
+#pragma unroll
+    for (int q = 0; q < kStage; ++q) {
+        const int col = (t >> 6) + q * (kThreads / kBlk);
+        sm.a[0][col][srow] = ra[q];
+        sm.b[0][col][srow] = rb[q];
+    }
+    __syncthreads();
+    const int nch = K / kKc;
+    for (int c = 0; c < nch; ++c) {
+        const int buf = c & 1;
+        const bool more = c + 1 < nch;
+        if (more) {
+            const int64_t k0 = (int64_t) (c + 1) * kKc;
+#pragma unroll
+            for (int q = 0; q < kStage; ++q) {
+                const int col = (t >> 6) + q * (kThreads / kBlk);
+                ra[q] = (double) A[srow + (k0 + col) * lda];
+                rb[q] = B[srow + (k0 + col) * ldb];
+            }
         }
-        __syncthreads();
 #pragma unroll
         for (int kk = 0; kk < kKc; kk += 4) {
-            double a0 = sm.a[kk + lk][mq * 32 + lr];
-            double a1 = sm.a[kk + lk][mq * 32 + 16 + lr];
-            double b0 = sm.b[kk + lk][nq * 32 + lr];
-            double b1 = sm.b[kk + lk][nq * 32 + 16 + lr];
+            const double a0 = sm.a[buf][kk + lk][mq * 32 + lr];
+            const double a1 = sm.a[buf][kk + lk][mq * 32 + 16 + lr];
+            const double b0 = sm.b[buf][kk + lk][nq * 32 + lr];
+            const double b1 = sm.b[buf][kk + lk][nq * 32 + 16 + lr];
             acc.v[0][0] = mfma_f64(b0, a0, acc.v[0][0]);
             acc.v[0][1] = mfma_f64(b0, a1, acc.v[0][1]);
             acc.v[1][0] = mfma_f64(b1, a0, acc.v[1][0]);
             acc.v[1][1] = mfma_f64(b1, a1, acc.v[1][1]);
+        }
+        if (more) {
+#pragma unroll
+            for (int q = 0; q < kStage; ++q) {
+                const int col = (t >> 6) + q * (kThreads / kBlk);
+                sm.a[buf ^ 1][col][srow] = ra[q];
+                sm.b[buf ^ 1][col][srow] = rb[q];
+            }
         }
         __syncthreads();
     }

@@ -37,6 +37,18 @@ __device__ __forceinline__ float *tile_f32(const TileMap &m, int i, int j) {
     return reinterpret_cast<float *>(m.base + m.off[i * m.nt + j]);
 }
 
+// Workgroups are dispatched to the 8 XCDs of an MI355X round-robin (workgroup w
+// runs on XCD w % 8), and each XCD has its own L2.  xcd_remap gives XCD x a
+// contiguous range of the logical block order, so blocks that share operand
+// rows (consecutive logical blocks) meet in the same L2.  It is a bijection
+// of [0, n).
+__device__ __forceinline__ int xcd_remap(int w, int n) {
+    constexpr int kXcd = 8;
+    const int q = n / kXcd, r = n % kXcd;
+    const int x = w % kXcd, i = w / kXcd;
+    return x * q + min(x, r) + i;
+}
+
 typedef double f64x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 

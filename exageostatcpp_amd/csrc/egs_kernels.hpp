@@ -66,10 +66,11 @@ void launch_dot(const double *x, const double *y, int64_t n, double *partial, hi
 // y = a x + b y
 void launch_axpby(double *y, const double *x, double a, double b, int64_t n, hipStream_t s);
 // Mixed-precision factorization pieces (egs_mixed.hip).
-void launch_panel_convert(const TileMap &map, int k, double *p64, float *p32, float *p32t, hipStream_t s);
+void launch_panel_convert(const TileMap &map, int k, double *p64, float *p32, __bf16 *p16h, __bf16 *p16l,
+                          hipStream_t s);
 void launch_update192_f32(const TileMap &map, const float *p32, const int2 *pairs, int npairs, int k, hipStream_t s);
-void launch_update192_bf16(const TileMap &map, const float *p32t, const int2 *pairs, int npairs, int k, bool split,
-                           hipStream_t s);
+void launch_update192_bf16(const TileMap &map, const __bf16 *p16h, const __bf16 *p16l, const int2 *pairs, int npairs,
+                           int k, bool split, hipStream_t s);
 // FP64 update reading panel k from a contiguous FP64 copy (tile i at p64 + (i-k-1) ts^2).
 void launch_update192_panel(const TileMap &map, const double *p64, const int2 *pairs, int npairs, int k,
                             hipStream_t s);

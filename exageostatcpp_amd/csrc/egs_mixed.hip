@@ -10,10 +10,11 @@
 //                     MFMAs, ~16 significant bits); a single hi*hi product is the
 //                     plain-BF16 option.
 // After panel k is solved, panel_convert_kernel writes it once in the layouts
-// the three update kernels read: FP64 column-major, FP32 column-major, and FP32
-// row-major (K-contiguous, for the BF16 operands that pack 8 consecutive K
-// values per lane).  Diagonal tiles, their factorization, the log-determinant
-// and every residual stay FP64.
+// the three update kernels read: FP64 column-major, FP32 column-major, and the
+// BF16 hi and lo parts row-major (K-contiguous: an MFMA operand is 8
+// consecutive K values of one row, one 16-byte load).  The split costs no
+// arithmetic in the GEMM.  Diagonal tiles, their factorization, the
+// log-determinant and every residual stay FP64.
 #include "egs_common.hpp"
 #include "egs_kernels.hpp"
 
@@ -36,7 +37,7 @@ __device__ __forceinline__ f32x4 mfma_bf16(const bf16x8 &x, const bf16x8 &y, f32
 // P32T (row-major) at slot i - k - 1.  One workgroup per 64 x 64 block.
 // ------------------------------------------------------------------------
 __global__ void __launch_bounds__(kThreads) panel_convert_kernel(TileMap map, int k, double *p64, float *p32,
-                                                                 float *p32t) {
+                                                                 __bf16 *p16h, __bf16 *p16l) {
     __shared__ float sT[kBlk][kBlk + 1];
     const int ts = map.ts;
     const int nsb = ts / kBlk;
@@ -62,7 +63,11 @@ __global__ void __launch_bounds__(kThreads) panel_convert_kernel(TileMap map, in
     // row-major: element (row, col) at row * ts + col; consecutive threads walk columns
     for (int rr = t >> 6; rr < kBlk; rr += kThreads / kBlk) {
         const int cc = t & 63;
-        p32t[slot_off + (int64_t) (br * kBlk + rr) * ts + bc * kBlk + cc] = sT[cc][rr];
+        const float x = sT[cc][rr];
+        const __bf16 hi = (__bf16) x;
+        const int64_t o = slot_off + (int64_t) (br * kBlk + rr) * ts + bc * kBlk + cc;
+        p16h[o] = hi;
+        p16l[o] = (__bf16) (x - (float) hi);
     }
 }
 
@@ -73,7 +78,7 @@ __global__ void __launch_bounds__(kThreads) panel_convert_kernel(TileMap map, in
 // FP32 MFMA output: lane l, element v is D[4*(l/16) + v][l%16].
 // ------------------------------------------------------------------------
 constexpr int kKC32 = 32;
-constexpr int kLds32 = kBM + 4;
+constexpr int kLds32 = kBM + 16;   // 208 floats: the 4 K-rows a wave reads use disjoint banks
 constexpr int kPer32 = kBM * kKC32 / kNT;   // 12
 
 __global__ void __launch_bounds__(kNT, 1) update192_f32_kernel(TileMap map, const float *p32, const int2 *pairs,
@@ -83,8 +88,9 @@ __global__ void __launch_bounds__(kNT, 1) update192_f32_kernel(TileMap map, cons
     const int ts = map.ts;
     const int nsb = ts / kBM;
     const int nsub = nsb * nsb;
-    const int pair = blockIdx.x / nsub;
-    const int sub = blockIdx.x % nsub;
+    const int bid = xcd_remap((int) blockIdx.x, (int) gridDim.x);
+    const int pair = bid / nsub;
+    const int sub = bid % nsub;
     const int sbm = sub % nsb, sbn = sub / nsb;
     const int2 ij = pairs[pair];
     const int64_t tsz = (int64_t) ts * ts;
@@ -95,14 +101,8 @@ __global__ void __launch_bounds__(kNT, 1) update192_f32_kernel(TileMap map, cons
     const int wm = w & 1, wn = w >> 1;
     const int lr = lane & 15, lk = lane >> 4;
 
-    uint32_t goff[kPer32];
-    int loff[kPer32];
-#pragma unroll
-    for (int q = 0; q < kPer32; ++q) {
-        const int e = t + q * kNT;
-        goff[q] = (uint32_t) ((e % kBM) + (e / kBM) * ts);
-        loff[q] = (e / kBM) * kLds32 + (e % kBM);
-    }
+    auto goff = [&](int q) { const int e = t + q * kNT; return (uint32_t) ((e % kBM) + (e / kBM) * ts); };
+    auto loff = [&](int q) { const int e = t + q * kNT; return (e / kBM) * kLds32 + (e % kBM); };
     const uint32_t kstep = (uint32_t) (kKC32 * ts);
 
     f32x4 acc[6][3];
@@ -111,123 +111,122 @@ __global__ void __launch_bounds__(kNT, 1) update192_f32_kernel(TileMap map, cons
 #pragma unroll
         for (int ni = 0; ni < 3; ++ni) acc[mi][ni] = f32x4{0.f, 0.f, 0.f, 0.f};
 
-    float ra[kPer32], rb[kPer32];
-#pragma unroll
-    for (int q = 0; q < kPer32; ++q) {
-        ra[q] = A[goff[q]];
-        rb[q] = B[goff[q]];
-    }
-#pragma unroll
-    for (int q = 0; q < kPer32; ++q) {
-        (&sA[0][0][0])[loff[q]] = ra[q];
-        (&sB[0][0][0])[loff[q]] = rb[q];
-    }
-    __syncthreads();
+    constexpr int kDepth = 2;   // chunks of loads in flight (register ring)
+    float ra[kDepth][kPer32], rb[kDepth][kPer32];
     const int nchunks = ts / kKC32;
-    uint32_t kofs = 0;
-    for (int c = 0; c < nchunks; ++c) {
-        const int buf = c & 1;
-        const bool more = c + 1 < nchunks;
-        kofs += kstep;
-        if (more) {
+    auto issue = [&](int set, int chunk) {
+        const uint32_t kofs = (uint32_t) chunk * kstep;
 #pragma unroll
-            for (int q = 0; q < kPer32; ++q) {
-                ra[q] = A[goff[q] + kofs];
-                rb[q] = B[goff[q] + kofs];
+        for (int q = 0; q < kPer32; ++q) {
+            ra[set][q] = A[goff(q) + kofs];
+            rb[set][q] = B[goff(q) + kofs];
+        }
+    };
+    auto stage = [&](int set, int buf) {
+        float *dA = &sA[buf][0][0];
+        float *dB = &sB[buf][0][0];
+#pragma unroll
+        for (int q = 0; q < kPer32; ++q) {
+            dA[loff(q)] = ra[set][q];
+            dB[loff(q)] = rb[set][q];
+        }
+    };
+#pragma unroll
+    for (int u = 0; u < kDepth; ++u)
+        if (u < nchunks) issue(u, u);
+    stage(0, 0);
+    __syncthreads();
+    for (int c0 = 0; c0 < nchunks; c0 += kDepth) {
+#pragma unroll
+        for (int u = 0; u < kDepth; ++u) {
+            const int c = c0 + u;
+            const int buf = c & 1;
+            if (c + kDepth < nchunks) issue(u, c + kDepth);
+#pragma unroll
+            for (int kk = 0; kk < kKC32; kk += 4) {
+                float a[6], b[3];
+#pragma unroll
+                for (int mi = 0; mi < 6; ++mi) a[mi] = sA[buf][kk + lk][wm * 96 + mi * 16 + lr];
+#pragma unroll
+                for (int ni = 0; ni < 3; ++ni) b[ni] = sB[buf][kk + lk][wn * 48 + ni * 16 + lr];
+#pragma unroll
+                for (int mi = 0; mi < 6; ++mi)
+#pragma unroll
+                    for (int ni = 0; ni < 3; ++ni) acc[mi][ni] = mfma_f32(b[ni], a[mi], acc[mi][ni]);
             }
+            if (c + 1 < nchunks) stage((u + 1) % kDepth, buf ^ 1);
+            __syncthreads();
         }
-#pragma unroll
-        for (int kk = 0; kk < kKC32; kk += 4) {
-            float a[6], b[3];
-#pragma unroll
-            for (int mi = 0; mi < 6; ++mi) a[mi] = sA[buf][kk + lk][wm * 96 + mi * 16 + lr];
-#pragma unroll
-            for (int ni = 0; ni < 3; ++ni) b[ni] = sB[buf][kk + lk][wn * 48 + ni * 16 + lr];
-#pragma unroll
-            for (int mi = 0; mi < 6; ++mi)
-#pragma unroll
-                for (int ni = 0; ni < 3; ++ni) acc[mi][ni] = mfma_f32(b[ni], a[mi], acc[mi][ni]);
-        }
-        if (more) {
-            float *dA = &sA[buf ^ 1][0][0];
-            float *dB = &sB[buf ^ 1][0][0];
-#pragma unroll
-            for (int q = 0; q < kPer32; ++q) {
-                dA[loff[q]] = ra[q];
-                dB[loff[q]] = rb[q];
-            }
-        }
-        __syncthreads();
     }
+    // C -= acc in two batches of 36 values (one memory round trip each)
     float *C = tile_f32(map, ij.x, ij.y);
     const uint32_t cbase = (uint32_t) ((sbm * kBM + wm * 96 + lr) + (sbn * kBM + wn * 48 + 4 * lk) * ts);
 #pragma unroll
-    for (int mi = 0; mi < 6; ++mi) {
-        float cv[3][4];
+    for (int m0 = 0; m0 < 6; m0 += 3) {
+        float cv[3][3][4];
 #pragma unroll
-        for (int ni = 0; ni < 3; ++ni)
+        for (int mi = 0; mi < 3; ++mi)
 #pragma unroll
-            for (int v = 0; v < 4; ++v) cv[ni][v] = C[cbase + (uint32_t) (mi * 16 + (ni * 16 + v) * ts)];
+            for (int ni = 0; ni < 3; ++ni)
 #pragma unroll
-        for (int ni = 0; ni < 3; ++ni)
+                for (int v = 0; v < 4; ++v)
+                    cv[mi][ni][v] = __builtin_nontemporal_load(
+                        &C[cbase + (uint32_t) ((m0 + mi) * 16 + (ni * 16 + v) * ts)]);
 #pragma unroll
-            for (int v = 0; v < 4; ++v) C[cbase + (uint32_t) (mi * 16 + (ni * 16 + v) * ts)] = cv[ni][v] - acc[mi][ni][v];
+        for (int mi = 0; mi < 3; ++mi)
+#pragma unroll
+            for (int ni = 0; ni < 3; ++ni)
+#pragma unroll
+                for (int v = 0; v < 4; ++v)
+                    __builtin_nontemporal_store(cv[mi][ni][v] - acc[m0 + mi][ni][v],
+                                                &C[cbase + (uint32_t) ((m0 + mi) * 16 + (ni * 16 + v) * ts)]);
     }
 }
 
 // ------------------------------------------------------------------------
-// BF16-class update: C_ij (FP32) -= A B^T with BF16 products on the
-// row-major FP32 panel.  K chunk 32 = one v_mfma_f32_16x16x32_bf16 step; LDS
-// holds the chunk m-major (32 K-values per row) so that a lane reads its 8
-// consecutive K values with two 16-byte loads.  Lane l supplies row l%16 and
-// K values 8*(l/16) .. +7 of both operands.
+// BF16-class update: C_ij (FP32) -= A B^T from the pre-split BF16 panels.
+// K chunk 32 = one v_mfma_f32_16x16x32_bf16 step.  LDS holds each chunk as 4
+// planes (K values 8p .. 8p+7) of 16-byte row vectors: the 16 lanes that read
+// one plane read 16 consecutive rows, 256 contiguous bytes, and the plane stride
+// of 196 rows puts the 4 planes a group of staging threads writes on different
+// banks.  Lane l supplies row l%16, K values 8*(l/16) .. +7 of both operands;
+// the FP32 MFMA result element v of lane l is D[4*(l/16) + v][l%16].
 // ------------------------------------------------------------------------
 constexpr int kKC16 = 32;
-constexpr int kRow16 = kKC16 + 4;          // floats per LDS row (16-byte aligned)
-constexpr int kPer16 = kBM * kKC16 / 4 / kNT;   // float4 loads per operand per thread: 3
-
-__device__ __forceinline__ void split_bf16(const f32x4v &x0, const f32x4v &x1, bf16x8 &hi, bf16x8 &lo) {
-#pragma unroll
-    for (int e = 0; e < 4; ++e) {
-        const __bf16 h0 = (__bf16) x0[e];
-        const __bf16 h1 = (__bf16) x1[e];
-        hi[e] = h0;
-        hi[e + 4] = h1;
-        lo[e] = (__bf16) (x0[e] - (float) h0);
-        lo[e + 4] = (__bf16) (x1[e] - (float) h1);
-    }
-}
+constexpr int kVecRow = kKC16 / 8;        // 16-byte vectors per row per chunk (planes)
+constexpr int kPlane = kBM + 4;           // rows per LDS plane, padded
+constexpr int kVecs = kBM * kVecRow;      // 768 vectors per operand part per chunk
 
 template <bool kSplit>
-__global__ void __launch_bounds__(kNT, 1) update192_bf16_kernel(TileMap map, const float *p32t, const int2 *pairs,
-                                                                int k) {
-    __shared__ f32x4v sA[2][kBM][kRow16 / 4];
-    __shared__ f32x4v sB[2][kBM][kRow16 / 4];
+__global__ void __launch_bounds__(kNT, 1) update192_bf16_kernel(TileMap map, const __bf16 *p16h, const __bf16 *p16l,
+                                                                const int2 *pairs, int k) {
+    constexpr int kParts = kSplit ? 2 : 1;
+    __shared__ bf16x8 sA[2][kParts][kVecRow][kPlane];
+    __shared__ bf16x8 sB[2][kParts][kVecRow][kPlane];
     const int ts = map.ts;
     const int nsb = ts / kBM;
     const int nsub = nsb * nsb;
-    const int pair = blockIdx.x / nsub;
-    const int sub = blockIdx.x % nsub;
+    const int bid = xcd_remap((int) blockIdx.x, (int) gridDim.x);
+    const int pair = bid / nsub;
+    const int sub = bid % nsub;
     const int sbm = sub % nsb, sbn = sub / nsb;
     const int2 ij = pairs[pair];
     const int64_t tsz = (int64_t) ts * ts;
     // row-major tiles: element (row, col) at row * ts + col
-    const float *A = p32t + (int64_t) (ij.x - k - 1) * tsz + (int64_t) sbm * kBM * ts;
-    const float *B = p32t + (int64_t) (ij.y - k - 1) * tsz + (int64_t) sbn * kBM * ts;
+    const int64_t aoff = (int64_t) (ij.x - k - 1) * tsz + (int64_t) sbm * kBM * ts;
+    const int64_t boff = (int64_t) (ij.y - k - 1) * tsz + (int64_t) sbn * kBM * ts;
+    const __bf16 *src[4] = {p16h + aoff, p16l + aoff, p16h + boff, p16l + boff};
     const int t = threadIdx.x;
     const int lane = t & 63, w = t >> 6;
     const int wm = w & 1, wn = w >> 1;
     const int lr = lane & 15, lk = lane >> 4;
 
-    uint32_t goff[kPer16];
-    int lrow[kPer16], lc4[kPer16];
-#pragma unroll
-    for (int q = 0; q < kPer16; ++q) {
-        const int e = t + q * kNT;          // 0 .. 1535: row e / 8, float4 column e % 8
-        lrow[q] = e >> 3;
-        lc4[q] = e & 7;
-        goff[q] = (uint32_t) ((e >> 3) * ts + (e & 7) * 4);
-    }
+    // staging: per operand part, vector e (< 768) is row e / 4, vector e % 4;
+    // threads take e = t and, for t < 256, e = 512 + t.
+    const bool two = t < kVecs - kNT;
+    const int e0 = t, e1 = kNT + t;
+    const uint32_t g0 = (uint32_t) ((e0 >> 2) * ts + (e0 & 3) * 8);
+    const uint32_t g1 = (uint32_t) ((e1 >> 2) * ts + (e1 & 3) * 8);
 
     f32x4 acc[6][3];
 #pragma unroll
@@ -235,84 +234,106 @@ __global__ void __launch_bounds__(kNT, 1) update192_bf16_kernel(TileMap map, con
 #pragma unroll
         for (int ni = 0; ni < 3; ++ni) acc[mi][ni] = f32x4{0.f, 0.f, 0.f, 0.f};
 
-    f32x4v ra[kPer16], rb[kPer16];
+    constexpr int kDepth = 3;   // chunks of loads in flight (register ring)
+    bf16x8 r0[kDepth][2 * kParts], r1[kDepth][2 * kParts];
+    auto issue = [&](int set, int chunk) {
+        const uint32_t kofs = (uint32_t) (chunk * kKC16);
 #pragma unroll
-    for (int q = 0; q < kPer16; ++q) {
-        ra[q] = *reinterpret_cast<const f32x4v *>(A + goff[q]);
-        rb[q] = *reinterpret_cast<const f32x4v *>(B + goff[q]);
-    }
+        for (int p = 0; p < kParts; ++p) {
+            r0[set][p] = *reinterpret_cast<const bf16x8 *>(src[p] + g0 + kofs);
+            r0[set][kParts + p] = *reinterpret_cast<const bf16x8 *>(src[2 + p] + g0 + kofs);
+            if (two) {
+                r1[set][p] = *reinterpret_cast<const bf16x8 *>(src[p] + g1 + kofs);
+                r1[set][kParts + p] = *reinterpret_cast<const bf16x8 *>(src[2 + p] + g1 + kofs);
+            }
+        }
+    };
+    auto stage = [&](int set, int buf) {
 #pragma unroll
-    for (int q = 0; q < kPer16; ++q) {
-        sA[0][lrow[q]][lc4[q]] = ra[q];
-        sB[0][lrow[q]][lc4[q]] = rb[q];
-    }
-    __syncthreads();
+        for (int p = 0; p < kParts; ++p) {
+            sA[buf][p][e0 & 3][e0 >> 2] = r0[set][p];
+            sB[buf][p][e0 & 3][e0 >> 2] = r0[set][kParts + p];
+            if (two) {
+                sA[buf][p][e1 & 3][e1 >> 2] = r1[set][p];
+                sB[buf][p][e1 & 3][e1 >> 2] = r1[set][kParts + p];
+            }
+        }
+    };
     const int nchunks = ts / kKC16;
-    for (int c = 0; c < nchunks; ++c) {
-        const int buf = c & 1;
-        const bool more = c + 1 < nchunks;
-        if (more) {
-            const uint32_t kofs = (uint32_t) ((c + 1) * kKC16);
 #pragma unroll
-            for (int q = 0; q < kPer16; ++q) {
-                ra[q] = *reinterpret_cast<const f32x4v *>(A + goff[q] + kofs);
-                rb[q] = *reinterpret_cast<const f32x4v *>(B + goff[q] + kofs);
+    for (int u = 0; u < kDepth; ++u)
+        if (u < nchunks) issue(u, u);
+    stage(0, 0);
+    __syncthreads();
+    for (int c0 = 0; c0 < nchunks; c0 += kDepth) {
+#pragma unroll
+        for (int u = 0; u < kDepth; ++u) {
+            const int c = c0 + u;
+            const int buf = c & 1;
+            if (c + kDepth < nchunks) issue(u, c + kDepth);
+            bf16x8 ah[6], bh[3];
+#pragma unroll
+            for (int mi = 0; mi < 6; ++mi) ah[mi] = sA[buf][0][lk][wm * 96 + mi * 16 + lr];
+#pragma unroll
+            for (int ni = 0; ni < 3; ++ni) bh[ni] = sB[buf][0][lk][wn * 48 + ni * 16 + lr];
+            if (kSplit) {
+                bf16x8 al[6], bl[3];
+#pragma unroll
+                for (int mi = 0; mi < 6; ++mi) al[mi] = sA[buf][kParts - 1][lk][wm * 96 + mi * 16 + lr];
+#pragma unroll
+                for (int ni = 0; ni < 3; ++ni) bl[ni] = sB[buf][kParts - 1][lk][wn * 48 + ni * 16 + lr];
+#pragma unroll
+                for (int mi = 0; mi < 6; ++mi)
+#pragma unroll
+                    for (int ni = 0; ni < 3; ++ni) {
+                        acc[mi][ni] = mfma_bf16(bh[ni], ah[mi], acc[mi][ni]);
+                        acc[mi][ni] = mfma_bf16(bh[ni], al[mi], acc[mi][ni]);
+                        acc[mi][ni] = mfma_bf16(bl[ni], ah[mi], acc[mi][ni]);
+                    }
+            } else {
+#pragma unroll
+                for (int mi = 0; mi < 6; ++mi)
+#pragma unroll
+                    for (int ni = 0; ni < 3; ++ni) acc[mi][ni] = mfma_bf16(bh[ni], ah[mi], acc[mi][ni]);
             }
+            if (c + 1 < nchunks) stage((u + 1) % kDepth, buf ^ 1);
+            __syncthreads();
         }
-        bf16x8 ah[6], al[6], bh[3], bl[3];
-#pragma unroll
-        for (int mi = 0; mi < 6; ++mi) {
-            const int row = wm * 96 + mi * 16 + lr;
-            split_bf16(sA[buf][row][2 * lk], sA[buf][row][2 * lk + 1], ah[mi], al[mi]);
-        }
-#pragma unroll
-        for (int ni = 0; ni < 3; ++ni) {
-            const int row = wn * 48 + ni * 16 + lr;
-            split_bf16(sB[buf][row][2 * lk], sB[buf][row][2 * lk + 1], bh[ni], bl[ni]);
-        }
-#pragma unroll
-        for (int mi = 0; mi < 6; ++mi)
-#pragma unroll
-            for (int ni = 0; ni < 3; ++ni) {
-                acc[mi][ni] = mfma_bf16(bh[ni], ah[mi], acc[mi][ni]);
-                if (kSplit) {
-                    acc[mi][ni] = mfma_bf16(bh[ni], al[mi], acc[mi][ni]);
-                    acc[mi][ni] = mfma_bf16(bl[ni], ah[mi], acc[mi][ni]);
-                }
-            }
-        if (more) {
-#pragma unroll
-            for (int q = 0; q < kPer16; ++q) {
-                sA[buf ^ 1][lrow[q]][lc4[q]] = ra[q];
-                sB[buf ^ 1][lrow[q]][lc4[q]] = rb[q];
-            }
-        }
-        __syncthreads();
     }
+    // C -= acc in two batches of 36 values (one memory round trip each)
     float *C = tile_f32(map, ij.x, ij.y);
     const uint32_t cbase = (uint32_t) ((sbm * kBM + wm * 96 + lr) + (sbn * kBM + wn * 48 + 4 * lk) * ts);
 #pragma unroll
-    for (int mi = 0; mi < 6; ++mi) {
-        float cv[3][4];
+    for (int m0 = 0; m0 < 6; m0 += 3) {
+        float cv[3][3][4];
 #pragma unroll
-        for (int ni = 0; ni < 3; ++ni)
+        for (int mi = 0; mi < 3; ++mi)
 #pragma unroll
-            for (int v = 0; v < 4; ++v) cv[ni][v] = C[cbase + (uint32_t) (mi * 16 + (ni * 16 + v) * ts)];
+            for (int ni = 0; ni < 3; ++ni)
 #pragma unroll
-        for (int ni = 0; ni < 3; ++ni)
+                for (int v = 0; v < 4; ++v)
+                    cv[mi][ni][v] = __builtin_nontemporal_load(
+                        &C[cbase + (uint32_t) ((m0 + mi) * 16 + (ni * 16 + v) * ts)]);
 #pragma unroll
-            for (int v = 0; v < 4; ++v) C[cbase + (uint32_t) (mi * 16 + (ni * 16 + v) * ts)] = cv[ni][v] - acc[mi][ni][v];
+        for (int mi = 0; mi < 3; ++mi)
+#pragma unroll
+            for (int ni = 0; ni < 3; ++ni)
+#pragma unroll
+                for (int v = 0; v < 4; ++v)
+                    __builtin_nontemporal_store(cv[mi][ni][v] - acc[m0 + mi][ni][v],
+                                                &C[cbase + (uint32_t) ((m0 + mi) * 16 + (ni * 16 + v) * ts)]);
     }
 }
 
 }  // namespace
 
-void launch_panel_convert(const TileMap &map, int k, double *p64, float *p32, float *p32t, hipStream_t s) {
+void launch_panel_convert(const TileMap &map, int k, double *p64, float *p32, __bf16 *p16h, __bf16 *p16l,
+                          hipStream_t s) {
     const int rows = map.nt - k - 1;
     if (rows <= 0) return;
     const int nsb = map.ts / kBlk;
     hipLaunchKernelGGL(panel_convert_kernel, dim3((unsigned) (rows * nsb * nsb)), dim3(kThreads), 0, s, map, k, p64,
-                       p32, p32t);
+                       p32, p16h, p16l);
 }
 
 void launch_update192_f32(const TileMap &map, const float *p32, const int2 *pairs, int npairs, int k, hipStream_t s) {
@@ -322,13 +343,13 @@ void launch_update192_f32(const TileMap &map, const float *p32, const int2 *pair
                        p32, pairs, k);
 }
 
-void launch_update192_bf16(const TileMap &map, const float *p32t, const int2 *pairs, int npairs, int k, bool split,
-                           hipStream_t s) {
+void launch_update192_bf16(const TileMap &map, const __bf16 *p16h, const __bf16 *p16l, const int2 *pairs, int npairs,
+                           int k, bool split, hipStream_t s) {
     if (npairs <= 0) return;
     const int nsb = map.ts / kBM;
     const dim3 grid((unsigned) npairs * (unsigned) (nsb * nsb));
-    if (split) hipLaunchKernelGGL(update192_bf16_kernel<true>, grid, dim3(kNT), 0, s, map, p32t, pairs, k);
-    else hipLaunchKernelGGL(update192_bf16_kernel<false>, grid, dim3(kNT), 0, s, map, p32t, pairs, k);
+    if (split) hipLaunchKernelGGL(update192_bf16_kernel<true>, grid, dim3(kNT), 0, s, map, p16h, p16l, pairs, k);
+    else hipLaunchKernelGGL(update192_bf16_kernel<false>, grid, dim3(kNT), 0, s, map, p16h, p16l, pairs, k);
 }
 
 }  // namespace egs

@@ -31,49 +31,84 @@ __device__ double wg_sum(double v, double *red) {
 }
 
 // ------------------------------------------------------------------------
-// Forward solve of diagonal tile k, with the log-det and quad partials.
+// Forward solve of diagonal tile k, with the log-det and quad partials.  The
+// 64-row blocks of the tile are solved in order; each block's GEMV over the
+// already-solved part is spread over 16 waves (column groups) with four
+// independent partial sums per thread so that many loads are in flight.
 // ------------------------------------------------------------------------
-__global__ void __launch_bounds__(kThreads) trsv_diag_kernel(TileMap map, int k, const double *winv_k, double *z,
-                                                             double *partials) {
+constexpr int kSolveThreads = 1024;
+constexpr int kGroups = kSolveThreads / kBlk;   // 16
+
+__device__ double wg_sum_1024(double v, double *red) {
+    const int t = threadIdx.x;
+    red[t] = v;
+    __syncthreads();
+    for (int s = kSolveThreads / 2; s > 0; s >>= 1) {
+        if (t < s) red[t] += red[t + s];
+        __syncthreads();
+    }
+    const double r = red[0];
+    __syncthreads();
+    return r;
+}
+
+__global__ void __launch_bounds__(kSolveThreads) trsv_diag_kernel(TileMap map, int k, const double *winv_k, double *z,
+                                                                  double *partials) {
     __shared__ double sy[kMaxTs];
-    __shared__ double sp[4][kBlk];
+    __shared__ double sp[kGroups][kBlk];
     __shared__ double st[kBlk];
-    __shared__ double red[kThreads];
+    __shared__ double red[kSolveThreads];
     const int ts = map.ts;
     const int nsb = ts / kBlk;
     const int t = threadIdx.x;
     const int r = t & 63, g = t >> 6;
     const double *L = tile_f64(map, k, k);
     double *zk = z + (int64_t) k * ts;
-    for (int e = t; e < ts; e += kThreads) sy[e] = zk[e];
+    for (int e = t; e < ts; e += kSolveThreads) sy[e] = zk[e];
     __syncthreads();
     for (int jb = 0; jb < nsb; ++jb) {
         // t = z_jb - L[jb, 0:jb] y[0:jb]
-        const int row = jb * kBlk + r;
-        double s = 0.0;
-        for (int c = g; c < jb * kBlk; c += 4) s += L[row + (int64_t) c * ts] * sy[c];
-        sp[g][r] = s;
+        const double *Lr = L + jb * kBlk + r;
+        double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+        const int cend = jb * kBlk;
+        int c = g;
+        for (; c + 3 * kGroups < cend; c += 4 * kGroups) {
+            s0 += Lr[(int64_t) c * ts] * sy[c];
+            s1 += Lr[(int64_t) (c + kGroups) * ts] * sy[c + kGroups];
+            s2 += Lr[(int64_t) (c + 2 * kGroups) * ts] * sy[c + 2 * kGroups];
+            s3 += Lr[(int64_t) (c + 3 * kGroups) * ts] * sy[c + 3 * kGroups];
+        }
+        for (; c < cend; c += kGroups) s0 += Lr[(int64_t) c * ts] * sy[c];
+        sp[g][r] = (s0 + s1) + (s2 + s3);
         __syncthreads();
-        if (t < kBlk) st[t] = sy[jb * kBlk + t] - (((sp[0][t] + sp[1][t]) + sp[2][t]) + sp[3][t]);
+        if (t < kBlk) {
+            double a = 0.0;
+            for (int q = 0; q < kGroups; ++q) a += sp[q][t];
+            st[t] = sy[jb * kBlk + t] - a;
+        }
         __syncthreads();
-        // y_jb = W_jb t  (W lower triangular)
+        // y_jb = W_jb t  (W lower triangular), 64 x 64 over 16 groups
         const double *W = winv_k + (int64_t) jb * kBlk * kBlk;
         double u = 0.0;
-        for (int c = g; c <= r; c += 4) u += W[r + c * kBlk] * st[c];
+        for (int cc = g; cc <= r; cc += kGroups) u += W[r + cc * kBlk] * st[cc];
         sp[g][r] = u;
         __syncthreads();
-        if (t < kBlk) sy[jb * kBlk + t] = ((sp[0][t] + sp[1][t]) + sp[2][t]) + sp[3][t];
+        if (t < kBlk) {
+            double a = 0.0;
+            for (int q = 0; q < kGroups; ++q) a += sp[q][t];
+            sy[jb * kBlk + t] = a;
+        }
         __syncthreads();
     }
     double ld = 0.0, q = 0.0;
-    for (int e = t; e < ts; e += kThreads) {
+    for (int e = t; e < ts; e += kSolveThreads) {
         const double d = L[e + (int64_t) e * ts];
         if (d > 0.0) ld += log(d);
         q += sy[e] * sy[e];
         zk[e] = sy[e];
     }
-    ld = wg_sum(ld, red);
-    q = wg_sum(q, red);
+    ld = wg_sum_1024(ld, red);
+    q = wg_sum_1024(q, red);
     if (t == 0) {
         partials[2 * k] = ld;
         partials[2 * k + 1] = q;
@@ -140,11 +175,12 @@ __global__ void __launch_bounds__(kThreads) trsv_t_partial_kernel(TileMap map, i
     }
 }
 
-// Backward solve of diagonal tile k: y_k = L_kk^{-T} (z_k - sum_i partials_i).
-__global__ void __launch_bounds__(kThreads) trsv_t_diag_kernel(TileMap map, int k, const double *winv_k, double *z,
-                                                               const double *scratch) {
+// Backward solve of diagonal tile k: y_k = L_kk^{-T} (z_k - sum_i partials_i),
+// 64-column blocks from the last one back, 16 waves per block GEMV.
+__global__ void __launch_bounds__(kSolveThreads) trsv_t_diag_kernel(TileMap map, int k, const double *winv_k,
+                                                                    double *z, const double *scratch) {
     __shared__ double sy[kMaxTs];
-    __shared__ double sp[4][kBlk];
+    __shared__ double sp[kGroups][kBlk];
     __shared__ double st[kBlk];
     const int ts = map.ts;
     const int nsb = ts / kBlk;
@@ -153,33 +189,45 @@ __global__ void __launch_bounds__(kThreads) trsv_t_diag_kernel(TileMap map, int 
     const int c = t & 63, g = t >> 6;
     const double *L = tile_f64(map, k, k);
     double *zk = z + (int64_t) k * ts;
-    for (int e = t; e < ts; e += kThreads) {
+    for (int e = t; e < ts; e += kSolveThreads) {
         double v = zk[e];
         for (int i = k + 1; i < nt; ++i) v -= scratch[(int64_t) (i - k - 1) * ts + e];
         sy[e] = v;
     }
     __syncthreads();
     for (int jb = nsb - 1; jb >= 0; --jb) {
-        // t[c] = sy[jb*64 + c] - sum_{r >= (jb+1)*64} L[r][jb*64 + c] y[r]
-        const int col = jb * kBlk + c;
-        double s = 0.0;
-        for (int rr = (jb + 1) * kBlk + g; rr < ts; rr += 4) s += L[rr + (int64_t) col * ts] * sy[rr];
-        sp[g][c] = s;
+        // t[c] = sy[jb*64 + c] - sum_{r >= (jb+1)*64} L[r][jb*64 + c] y[r]; wave g takes rows r = base + g + 16 m
+        const double *Lc = L + (int64_t) (jb * kBlk + c) * ts;
+        double s0 = 0.0, s1 = 0.0;
+        const int rbeg = (jb + 1) * kBlk;
+        int rr = rbeg + g;
+        for (; rr + kGroups < ts; rr += 2 * kGroups) {
+            s0 += Lc[rr] * sy[rr];
+            s1 += Lc[rr + kGroups] * sy[rr + kGroups];
+        }
+        for (; rr < ts; rr += kGroups) s0 += Lc[rr] * sy[rr];
+        sp[g][c] = s0 + s1;
         __syncthreads();
-        if (t < kBlk) st[t] = sy[jb * kBlk + t] - (((sp[0][t] + sp[1][t]) + sp[2][t]) + sp[3][t]);
+        if (t < kBlk) {
+            double a = 0.0;
+            for (int q = 0; q < kGroups; ++q) a += sp[q][t];
+            st[t] = sy[jb * kBlk + t] - a;
+        }
         __syncthreads();
         // y[c] = sum_{r >= c} W[r][c] t[r]
         const double *W = winv_k + (int64_t) jb * kBlk * kBlk;
         double u = 0.0;
-        for (int rr = c + g; rr < kBlk; rr += 4) {
-            if (rr >= c) u += W[rr + c * kBlk] * st[rr];
-        }
+        for (int r2 = c + g; r2 < kBlk; r2 += kGroups) u += W[r2 + c * kBlk] * st[r2];
         sp[g][c] = u;
         __syncthreads();
-        if (t < kBlk) sy[jb * kBlk + t] = ((sp[0][t] + sp[1][t]) + sp[2][t]) + sp[3][t];
+        if (t < kBlk) {
+            double a = 0.0;
+            for (int q = 0; q < kGroups; ++q) a += sp[q][t];
+            sy[jb * kBlk + t] = a;
+        }
         __syncthreads();
     }
-    for (int e = t; e < ts; e += kThreads) zk[e] = sy[e];
+    for (int e = t; e < ts; e += kSolveThreads) zk[e] = sy[e];
 }
 
 // out = L e, one workgroup per 64 rows.
@@ -387,7 +435,7 @@ void launch_axpby(double *y, const double *x, double a, double b, int64_t n, hip
 }
 
 void launch_trsv_diag(const TileMap &map, int k, const double *winv_k, double *z, double *partials, hipStream_t s) {
-    hipLaunchKernelGGL(trsv_diag_kernel, dim3(1), dim3(kThreads), 0, s, map, k, winv_k, z, partials);
+    hipLaunchKernelGGL(trsv_diag_kernel, dim3(1), dim3(kSolveThreads), 0, s, map, k, winv_k, z, partials);
 }
 
 void launch_trsv_panel(const TileMap &map, int k, double *z, hipStream_t s) {
@@ -406,7 +454,7 @@ void launch_trsv_t_partial(const TileMap &map, int k, const double *z, double *s
 
 void launch_trsv_t_diag(const TileMap &map, int k, const double *winv_k, double *z, const double *scratch,
                         hipStream_t s) {
-    hipLaunchKernelGGL(trsv_t_diag_kernel, dim3(1), dim3(kThreads), 0, s, map, k, winv_k, z, scratch);
+    hipLaunchKernelGGL(trsv_t_diag_kernel, dim3(1), dim3(kSolveThreads), 0, s, map, k, winv_k, z, scratch);
 }
 
 void launch_trmv(const TileMap &map, const double *e, double *out, hipStream_t s) {

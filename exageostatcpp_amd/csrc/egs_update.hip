@@ -7,9 +7,9 @@
 //   * wave w covers rows (w % 2) * 96 .. +96 and columns (w / 2) * 48 .. +48,
 //     i.e. 6 x 3 independent 16 x 16 accumulators (72 FP64 values per lane);
 //   * A and B are staged through LDS in K-chunks of 16 columns, double
-//     buffered: the global loads of chunk c+1 are issued before the MFMAs of
-//     chunk c and land in the other LDS buffer afterwards, so each chunk costs
-//     one barrier and the HBM/L2 latency hides behind 72 MFMAs per wave;
+//     buffered, fed by a ring of registers that keeps the global loads of the
+//     next two chunks in flight while a chunk is multiplied, so each chunk
+//     costs one barrier and the HBM/L2 latency hides behind 144 MFMAs per wave;
 //   * global loads read 192 contiguous doubles per K-column (coalesced).
 // The operand panel is either the tile arena itself or a contiguous FP64 copy
 // of panel k (PanelRef), so the inner loop has no type dispatch.
@@ -23,7 +23,7 @@ namespace {
 constexpr int kBM = 192;           // block rows (and columns)
 constexpr int kKC = 16;            // K chunk
 constexpr int kNT = 512;           // threads per workgroup
-constexpr int kLds = kBM + 4;      // LDS row stride in doubles
+constexpr int kLds = kBM + 16;     // LDS row stride in doubles: the 4 K-rows a wave reads land on disjoint bank halves
 constexpr int kPer = kBM * kKC / kNT;   // elements of A (and of B) each thread stages per chunk
 
 // Where the operand tiles of panel k live: in the tile arena itself (FP64 plan)
@@ -46,8 +46,9 @@ __global__ void __launch_bounds__(kNT, 1) update192_kernel(TileMap map, PanelRef
     const int ts = map.ts;
     const int nsb = ts / kBM;
     const int nsub = nsb * nsb;
-    const int pair = blockIdx.x / nsub;
-    const int sub = blockIdx.x % nsub;
+    const int bid = xcd_remap((int) blockIdx.x, (int) gridDim.x);
+    const int pair = bid / nsub;
+    const int sub = bid % nsub;
     const int sbm = sub % nsb, sbn = sub / nsb;
     const int2 ij = pairs[pair];
     if (ij.x == ij.y && sbm < sbn) return;   // strictly upper block of a diagonal tile
@@ -62,14 +63,9 @@ __global__ void __launch_bounds__(kNT, 1) update192_kernel(TileMap map, PanelRef
 
     // staging: element e = t + q * kNT of the 192 x 16 chunk sits at row e % 192,
     // column e / 192; its offset inside the tile (in elements) fits in 32 bits.
-    uint32_t goff[kPer];
-    int loff[kPer];
-#pragma unroll
-    for (int q = 0; q < kPer; ++q) {
-        const int e = t + q * kNT;
-        goff[q] = (uint32_t) ((e % kBM) + (e / kBM) * ts);
-        loff[q] = (e / kBM) * kLds + (e % kBM);
-    }
+    // The offsets are recomputed where used (cheap integer ops) to save registers.
+    auto goff = [&](int q) { const int e = t + q * kNT; return (uint32_t) ((e % kBM) + (e / kBM) * ts); };
+    auto loff = [&](int q) { const int e = t + q * kNT; return (e / kBM) * kLds + (e % kBM); };
     const uint32_t kstep = (uint32_t) (kKC * ts);
 
     f64x4 acc[6][3];
@@ -78,75 +74,82 @@ __global__ void __launch_bounds__(kNT, 1) update192_kernel(TileMap map, PanelRef
 #pragma unroll
         for (int ni = 0; ni < 3; ++ni) acc[mi][ni] = f64x4{0.0, 0.0, 0.0, 0.0};
 
-    double ra[kPer], rb[kPer];
+    // Register ring: chunks c+1 .. c+kDepth are in flight while chunk c is
+    // multiplied from LDS; set u of the ring holds chunks u, u + kDepth, ...
+    constexpr int kDepth = 2;
+    double ra[kDepth][kPer], rb[kDepth][kPer];
+    const int nchunks = ts / kKC;
+    auto issue = [&](int set, int chunk) {
+        const uint32_t kofs = (uint32_t) chunk * kstep;
 #pragma unroll
-    for (int q = 0; q < kPer; ++q) {
-        ra[q] = A[goff[q]];
-        rb[q] = B[goff[q]];
-    }
+        for (int q = 0; q < kPer; ++q) {
+            ra[set][q] = A[goff(q) + kofs];
+            rb[set][q] = B[goff(q) + kofs];
+        }
+    };
+    auto stage = [&](int set, int buf) {
+        double *dA = &sA[buf][0][0];
+        double *dB = &sB[buf][0][0];
 #pragma unroll
-    for (int q = 0; q < kPer; ++q) {
-        (&sA[0][0][0])[loff[q]] = ra[q];
-        (&sB[0][0][0])[loff[q]] = rb[q];
-    }
+        for (int q = 0; q < kPer; ++q) {
+            dA[loff(q)] = ra[set][q];
+            dB[loff(q)] = rb[set][q];
+        }
+    };
+#pragma unroll
+    for (int u = 0; u < kDepth; ++u)
+        if (u < nchunks) issue(u, u);
+    stage(0, 0);
     __syncthreads();
 
-    const int nchunks = ts / kKC;
-    uint32_t kofs = 0;
-    for (int c = 0; c < nchunks; ++c) {
-        const int buf = c & 1;
-        const bool more = c + 1 < nchunks;
-        kofs += kstep;
-        if (more) {
+    for (int c0 = 0; c0 < nchunks; c0 += kDepth) {
 #pragma unroll
-            for (int q = 0; q < kPer; ++q) {
-                ra[q] = A[goff[q] + kofs];
-                rb[q] = B[goff[q] + kofs];
+        for (int u = 0; u < kDepth; ++u) {
+            const int c = c0 + u;
+            const int buf = c & 1;
+            if (c + kDepth < nchunks) issue(u, c + kDepth);
+#pragma unroll
+            for (int kk = 0; kk < kKC; kk += 4) {
+                double a[6], b[3];
+#pragma unroll
+                for (int mi = 0; mi < 6; ++mi) a[mi] = sA[buf][kk + lk][wm * 96 + mi * 16 + lr];
+#pragma unroll
+                for (int ni = 0; ni < 3; ++ni) b[ni] = sB[buf][kk + lk][wn * 48 + ni * 16 + lr];
+#pragma unroll
+                for (int mi = 0; mi < 6; ++mi)
+#pragma unroll
+                    for (int ni = 0; ni < 3; ++ni) acc[mi][ni] = mfma_f64(b[ni], a[mi], acc[mi][ni]);
             }
+            if (c + 1 < nchunks) stage((u + 1) % kDepth, buf ^ 1);
+            __syncthreads();
         }
-#pragma unroll
-        for (int kk = 0; kk < kKC; kk += 4) {
-            double a[6], b[3];
-#pragma unroll
-            for (int mi = 0; mi < 6; ++mi) a[mi] = sA[buf][kk + lk][wm * 96 + mi * 16 + lr];
-#pragma unroll
-            for (int ni = 0; ni < 3; ++ni) b[ni] = sB[buf][kk + lk][wn * 48 + ni * 16 + lr];
-#pragma unroll
-            for (int mi = 0; mi < 6; ++mi)
-#pragma unroll
-                for (int ni = 0; ni < 3; ++ni) acc[mi][ni] = mfma_f64(b[ni], a[mi], acc[mi][ni]);
-        }
-        if (more) {
-            double *dA = &sA[buf ^ 1][0][0];
-            double *dB = &sB[buf ^ 1][0][0];
-#pragma unroll
-            for (int q = 0; q < kPer; ++q) {
-                dA[loff[q]] = ra[q];
-                dB[loff[q]] = rb[q];
-            }
-        }
-        __syncthreads();
     }
 
     // C -= acc.  Lane l, element v of acc[mi][ni] is
     // C[wm*96 + mi*16 + l%16][wn*48 + ni*16 + l/16 + 4v]  (FP64 MFMA output order).
-    // The 12 values of one mi are loaded together before any store so that the
-    // read-modify-write costs one memory round trip per mi, not one per value.
+    // Read-modify-write of C in three batches of 24 values (one memory round
+    // trip per batch).
     double *C = tile_f64(map, ij.x, ij.y);
     const uint32_t cbase = (uint32_t) ((sbm * kBM + wm * 96 + lr) + (sbn * kBM + wn * 48 + lk) * ts);
 #pragma unroll
-    for (int mi = 0; mi < 6; ++mi) {
-        double cv[3][4];
+    for (int m0 = 0; m0 < 6; m0 += 2) {
+        double cv[2][3][4];
 #pragma unroll
-        for (int ni = 0; ni < 3; ++ni)
+        for (int mi = 0; mi < 2; ++mi)
 #pragma unroll
-            for (int v = 0; v < 4; ++v) cv[ni][v] = __builtin_nontemporal_load(&C[cbase + (uint32_t) (mi * 16 + (ni * 16 + 4 * v) * ts)]);
+            for (int ni = 0; ni < 3; ++ni)
 #pragma unroll
-        for (int ni = 0; ni < 3; ++ni)
+                for (int v = 0; v < 4; ++v)
+                    cv[mi][ni][v] = __builtin_nontemporal_load(
+                        &C[cbase + (uint32_t) ((m0 + mi) * 16 + (ni * 16 + 4 * v) * ts)]);
 #pragma unroll
-            for (int v = 0; v < 4; ++v)
-                __builtin_nontemporal_store(cv[ni][v] - acc[mi][ni][v],
-                                            &C[cbase + (uint32_t) (mi * 16 + (ni * 16 + 4 * v) * ts)]);
+        for (int mi = 0; mi < 2; ++mi)
+#pragma unroll
+            for (int ni = 0; ni < 3; ++ni)
+#pragma unroll
+                for (int v = 0; v < 4; ++v)
+                    __builtin_nontemporal_store(cv[mi][ni][v] - acc[m0 + mi][ni][v],
+                                                &C[cbase + (uint32_t) ((m0 + mi) * 16 + (ni * 16 + 4 * v) * ts)]);
     }
 }
 
