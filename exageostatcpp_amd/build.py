@@ -14,8 +14,13 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libexageostat_amd.so")
+# C++ host API (ExaGeoStat<T>, Configurations, Kernel<T> plugins) over the C ABI
+API_SRC = os.path.join(HERE, "api", "ExaGeoStat.cpp")
+API_LIB = os.path.join(HERE, "libexageostat_cpp.so")
+CXX = os.environ.get("CXX", "g++")
 
-SOURCES = ["egs_covgen.hip", "egs_chol.hip", "egs_update.hip", "egs_mixed.hip", "egs_solve.hip", "egs_capi.cpp"]
+SOURCES = ["egs_covgen.hip", "egs_chol.hip", "egs_update.hip", "egs_mixed.hip", "egs_solve.hip", "egs_capi.cpp",
+           "egs_bobyqa.cpp"]
 ARCH = os.environ.get("EGS_OFFLOAD_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
@@ -48,7 +53,41 @@ def build(verbose=False):
     if r.returncode != 0:
         raise RuntimeError("link failed:\n" + r.stdout + r.stderr)
     os.replace(tmp, LIB)
+    build_api()
+    build_tests()
     return LIB
+
+
+def build_api():
+    """libexageostat_cpp.so: host-only C++, linked against the C ABI library."""
+    tmp = API_LIB + ".tmp"
+    cmd = [CXX, "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall", "-Wextra", "-I" + os.path.join(ROOT, "include"),
+           API_SRC, "-o", tmp, "-L" + HERE, "-lexageostat_amd", "-Wl,-rpath,$ORIGIN"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("C++ API build failed:\n" + r.stdout + r.stderr)
+    os.replace(tmp, API_LIB)
+    return API_LIB
+
+
+def build_tests():
+    """C++ test programs under tests/cpp (binaries in tests/cpp/_bin)."""
+    src_dir = os.path.join(ROOT, "tests", "cpp")
+    out_dir = os.path.join(src_dir, "_bin")
+    os.makedirs(out_dir, exist_ok=True)
+    outs = []
+    for name in sorted(os.listdir(src_dir)):
+        if not name.endswith(".cpp"):
+            continue
+        out = os.path.join(out_dir, name[:-4])
+        cmd = [CXX, "-O2", "-std=c++17", "-Wall", "-I" + os.path.join(ROOT, "include"), os.path.join(src_dir, name),
+               "-o", out + ".tmp", "-L" + HERE, "-lexageostat_cpp", "-lexageostat_amd", "-Wl,-rpath,$ORIGIN/../../../exageostatcpp_amd"]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError("test build failed for %s:\n%s" % (name, r.stdout + r.stderr))
+        os.replace(out + ".tmp", out)
+        outs.append(out)
+    return outs
 
 
 if __name__ == "__main__":

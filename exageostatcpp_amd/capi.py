@@ -81,7 +81,12 @@ EXPORTS = {
     "egs_generate_locations": ([_i64, _int, _dp, _dp, _dp], _int),
     "egs_generate_observations": ([_vp, _dp, _int, _dp], _int),
     "egs_predict": ([_vp, _dp, _i64, _dp, _dp, _dp, _dp, _dp, _dp], _int),
+    "egs_bobyqa": ([_int, _dp, _dp, _dp, ctypes.c_double, _int, ctypes.c_double, _vp, _vp, _dp,
+                    ctypes.POINTER(_int), ctypes.POINTER(_int)], _int),
+    "egs_data_modeling": ([_vp, _dp, _dp, _dp, ctypes.c_double, _int, _dp, _dp, ctypes.POINTER(_int)], _int),
 }
+
+OBJECTIVE_FN = ctypes.CFUNCTYPE(ctypes.c_double, _dp, _vp)
 
 
 def lib():
@@ -262,6 +267,44 @@ class Problem:
         _check(lib().egs_predict(self.handle, pth, len(xm), pxm, pym, pzm, out.ctypes.data_as(_dp), pza,
                                  ctypes.byref(mspe)))
         return out, mspe.value
+
+    def data_modeling(self, lb, ub, start=None, tolerance=1e-9, max_iterations=1000):
+        """Maximum-likelihood fit (StarpuRuntimeSolver<T>::ModelingOperations): BOBYQA over
+        [lb, ub] from start (default lb).  Returns (theta_hat, best loglik, evaluations)."""
+        lb, plb = _arr(lb)
+        ub, pub = _arr(ub)
+        st, pst = _arr(start)
+        theta = np.empty(len(lb))
+        llh, nev = ctypes.c_double(), _int()
+        _check(lib().egs_data_modeling(self.handle, plb, pub, pst, tolerance, max_iterations,
+                                       theta.ctypes.data_as(_dp), ctypes.byref(llh), ctypes.byref(nev)))
+        return theta, llh.value, nev.value
+
+
+def bobyqa(fn, x0, lb, ub, ftol_abs=0.0, maxeval=1000, rho_end=0.0):
+    """Minimise fn over [lb, ub] with the library's BOBYQA (host code, no GPU needed).
+    Returns (x_best, f_best, evaluations, reason) with reason 1 radius, 2 ftol, 3 maxeval."""
+    x = np.array(x0, dtype=np.float64)
+    lb, plb = _arr(lb)
+    ub, pub = _arr(ub)
+    n = len(x)
+    err = []
+
+    def _cb(px, _user):
+        try:
+            return float(fn(np.ctypeslib.as_array(px, shape=(n,)).copy()))
+        except Exception as e:  # surfaced after the call
+            err.append(e)
+            return float("nan")
+
+    cb = OBJECTIVE_FN(_cb)
+    fmin, nev, reason = ctypes.c_double(), _int(), _int()
+    code = lib().egs_bobyqa(n, x.ctypes.data_as(_dp), plb, pub, ftol_abs, maxeval, rho_end,
+                            ctypes.cast(cb, _vp), None, ctypes.byref(fmin), ctypes.byref(nev), ctypes.byref(reason))
+    if err:
+        raise err[0]
+    _check(code)
+    return x, fmin.value, nev.value, reason.value
 
 
 def srand(seed):

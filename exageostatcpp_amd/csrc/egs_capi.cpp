@@ -17,6 +17,7 @@
 #include <vector>
 
 #include "../../include/exageostat_amd.h"
+#include "egs_bobyqa.hpp"
 #include "egs_common.hpp"
 #include "egs_kernels.hpp"
 #include "egs_matern.hpp"
@@ -1178,6 +1179,96 @@ int egs_predict(egs_problem_t p, const double *theta, int64_t n_miss, const doub
             *mspe = -1.0;
         }
     }
+    return EGS_SUCCESS;
+}
+
+int egs_bobyqa(int n, double *x, const double *lb, const double *ub, double ftol_abs, int maxeval, double rho_end,
+               egs_objective_fn fn, void *user, double *fmin, int *nevals, int *reason) {
+    if (n < 1 || n > 64 || !x || !lb || !ub || !fn || maxeval < 1)
+        return fail(EGS_ERR_INVALID_ARG, "egs_bobyqa: invalid arguments (n=%d, maxeval=%d)", n, maxeval);
+    for (int i = 0; i < n; ++i)
+        if (!(std::isfinite(lb[i]) && std::isfinite(ub[i]) && ub[i] > lb[i]))
+            return fail(EGS_ERR_INVALID_ARG, "egs_bobyqa: bound %d is not a finite interval [%g, %g]", i, lb[i], ub[i]);
+    egs::BobyqaOptions opt;
+    opt.ftol_abs = ftol_abs > 0.0 ? ftol_abs : 0.0;
+    opt.maxeval = maxeval;
+    if (rho_end > 0.0) opt.rho_end = rho_end;
+    const egs::BobyqaResult r = egs::bobyqa_minimize(
+        n, x, lb, ub, [&](const double *v) { return fn(v, user); }, opt);
+    if (fmin) *fmin = r.fmin;
+    if (nevals) *nevals = r.nevals;
+    if (reason) *reason = static_cast<int>(r.status);
+    if (r.status == egs::BobyqaStatus::kFailure) return fail(EGS_ERR_INVALID_ARG, "egs_bobyqa: the objective returned NaN");
+    return EGS_SUCCESS;
+}
+
+int egs_data_modeling(egs_problem_t p, const double *lb, const double *ub, const double *start, double tolerance,
+                      int max_iterations, double *theta, double *loglik, int *nevals) {
+    int st = check_problem(p);
+    if (st) return st;
+    if (!lb || !ub || !theta || !loglik) return fail(EGS_ERR_INVALID_ARG, "egs_data_modeling: null argument");
+    if (max_iterations < 1) return fail(EGS_ERR_INVALID_ARG, "egs_data_modeling: max_iterations must be >= 1");
+    if (!p->has_obs) return fail(EGS_ERR_STATE, "the problem has no observations");
+    const int nt = p->ntheta();
+    std::vector<double> lo(lb, lb + nt), hi(ub, ub + nt), x(start ? start : lb, (start ? start : lb) + nt);
+    // A parameter with equal bounds is held fixed (Configurations::InitializeAllTheta
+    // pins estimated parameters this way, Configurations.cpp:204-211).
+    std::vector<int> free_idx;
+    for (int i = 0; i < nt; ++i) {
+        if (!(std::isfinite(lo[i]) && std::isfinite(hi[i]) && hi[i] >= lo[i]))
+            return fail(EGS_ERR_INVALID_ARG, "egs_data_modeling: bound %d is not a finite interval [%g, %g]", i, lo[i], hi[i]);
+        if (hi[i] > lo[i]) free_idx.push_back(i);
+        else x[i] = lo[i];
+    }
+    int eval_status = EGS_SUCCESS;
+    std::string eval_error;
+    double best = -INFINITY;
+    int count = 0;
+    auto llh_at = [&](const std::vector<double> &th) -> double {
+        double v = 0.0;
+        const int s2 = egs_loglik(p, th.data(), &v, nullptr, nullptr);
+        ++count;
+        if (s2 != EGS_SUCCESS) {
+            if (eval_status == EGS_SUCCESS) {
+                eval_status = s2;
+                eval_error = g_last_error;
+            }
+            return NAN;
+        }
+        if (v > best) {
+            best = v;
+            std::copy(th.begin(), th.end(), theta);
+        }
+        return v;
+    };
+    if (free_idx.empty()) {
+        llh_at(x);
+    } else {
+        const int nf = static_cast<int>(free_idx.size());
+        std::vector<double> xf(nf), lf(nf), hf(nf);
+        for (int j = 0; j < nf; ++j) {
+            xf[j] = x[free_idx[j]];
+            lf[j] = lo[free_idx[j]];
+            hf[j] = hi[free_idx[j]];
+        }
+        egs::BobyqaOptions opt;
+        opt.ftol_abs = tolerance > 0.0 ? tolerance : 0.0;
+        opt.maxeval = max_iterations;
+        std::vector<double> th(x);
+        egs::bobyqa_minimize(
+            nf, xf.data(), lf.data(), hf.data(),
+            [&](const double *v) {
+                for (int j = 0; j < nf; ++j) th[free_idx[j]] = v[j];
+                return -llh_at(th);
+            },
+            opt);
+    }
+    if (nevals) *nevals = count;
+    if (eval_status != EGS_SUCCESS) {
+        g_last_error = eval_error;
+        return eval_status;
+    }
+    *loglik = best;
     return EGS_SUCCESS;
 }
 

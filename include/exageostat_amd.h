@@ -171,6 +171,30 @@ int egs_generate_observations(egs_problem_t problem, const double *theta, int se
 int egs_predict(egs_problem_t problem, const double *theta, int64_t n_miss, const double *xm, const double *ym,
                 const double *zm, double *zmiss, const double *z_actual, double *mspe);
 
+/* ---- maximum-likelihood driver ----------------------------------------------
+ * Bound-constrained derivative-free minimisation (Powell's BOBYQA), the
+ * optimiser that StarpuRuntimeSolver<T>::ModelingOperations takes from NLopt
+ * (src/runtime-solver/concrete/StarpuRuntimeSolver.cpp:38-48).  fn is called
+ * with a point inside [lb, ub]; x holds the start on entry and the best point
+ * on return.  Stops when an improvement of the best value is below ftol_abs
+ * (0: off), after maxeval evaluations, or when the trust region shrinks to
+ * rho_end times the initial step (<= 0: 1e-7).  *reason: 1 radius, 2 ftol,
+ * 3 maxeval.  A NaN from fn ends the search with EGS_ERR_INVALID_ARG. */
+typedef double (*egs_objective_fn)(const double *x, void *user);
+int egs_bobyqa(int n, double *x, const double *lb, const double *ub, double ftol_abs, int maxeval, double rho_end,
+               egs_objective_fn fn, void *user, double *fmin, int *nevals, int *reason);
+
+/* StarpuRuntimeSolver<T>::ModelingOperations (StarpuRuntimeSolver.cpp:28-65):
+ * maximises egs_loglik over theta in [lb, ub] from start (NULL: lb, as
+ * Configurations::InitializeAllTheta sets it, Configurations.cpp:200) with
+ * ftol_abs = tolerance and at most max_iterations likelihood evaluations.
+ * theta receives the best point, *loglik the best value (NLopt's
+ * last_optimum_value), *nevals the evaluation count (may be NULL).  A failed
+ * evaluation (not positive definite, invalid theta) ends the search with its
+ * status, as the reference's exception does. */
+int egs_data_modeling(egs_problem_t problem, const double *lb, const double *ub, const double *start, double tolerance,
+                      int max_iterations, double *theta, double *loglik, int *nevals);
+
 #ifdef __cplusplus
 }
 #endif

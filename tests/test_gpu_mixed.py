@@ -94,3 +94,37 @@ def test_mixed_needs_tile_multiple_of_192(ctx):
     p = egs.Problem(ctx, 0, x, y, dts=320)
     with pytest.raises(egs.EGSError, match="multiple of 192"):
         p.set_precision(egs.PRECISION_MIXED, 2, 10)
+
+
+def _mle_fixture():
+    import json
+    import os
+    with open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "mle_n1600.json")) as fh:
+        return json.load(fh)
+
+
+def test_data_modeling_fp64_and_mixed_theta_hat(ctx):
+    """egs_data_modeling (BOBYQA on the GPU likelihood) at n = 1600: the FP64 fit
+    lands within 1e-4 of the independent optimum, the mixed fit (BF16 products
+    from band 4 on) within 1e-3 of it."""
+    g = _mle_fixture()
+    x, y = orc.generate_locations(g["n"], seed=0)
+    z = orc.generate_observations(0, (x, y), g["theta_true"], g["dts"], seed=0)
+    ref = np.array(g["theta_hat"])
+    log("data modeling n=1600 fp64")
+    p = egs.Problem(ctx, 0, x, y, dts=g["dts"])
+    p.set_observations(z)
+    th, llh, nev = p.data_modeling(g["lb"], g["ub"], tolerance=1e-9, max_iterations=1000)
+    log("  fp64 theta %s llh %.10f (%d evaluations)" % (th, llh, nev))
+    assert np.max(np.abs(th - ref)) < 1e-4
+    assert llh >= g["loglik"] - 1e-6
+    p.close()
+    log("data modeling n=1600 mixed")
+    pm = egs.Problem(ctx, 0, x, y, dts=192)
+    pm.set_observations(z)
+    pm.set_precision(egs.PRECISION_MIXED, 1, 3)
+    thm, llhm, nevm = pm.data_modeling(g["lb"], g["ub"], tolerance=1e-9, max_iterations=1000)
+    log("  mixed theta %s llh %.10f (%d evaluations)" % (thm, llhm, nevm))
+    assert np.max(np.abs(thm - ref)) < 1e-3
+    assert abs(llhm - g["loglik"]) < 1e-6 * abs(g["loglik"])
+    pm.close()
