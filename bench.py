@@ -61,6 +61,22 @@ def cpu_baseline(n_target, n_sample, dts, theta):
     }
 
 
+def measured_traffic(kernel, args):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter
+    summary (FETCH_SIZE + WRITE_SIZE, copy-calibrated), when it was taken on
+    this same workload; None otherwise."""
+    path = os.path.join(ROOT, "profiles", "%s_n%dk_traffic.json" % (kernel, args.n // 1000))
+    try:
+        with open(path) as fh:
+            t = json.load(fh)
+    except (OSError, ValueError):
+        return None
+    same = (t.get("kernel") == kernel and t.get("n") == args.n and t.get("dts") == args.dts
+            and t.get("band_fp64") == args.band_fp64 and t.get("band_fp32") == args.band_fp32
+            and t.get("bf16_products") == (1 if args.plain_bf16 else 3))
+    return t["fetch_bytes_per_launch"] + t["write_bytes_per_launch"] if same else None
+
+
 def timed(egs, ctx, prob, theta, steps, warmup, label):
     for i in range(warmup):
         t0 = time.perf_counter()
@@ -189,7 +205,8 @@ def main():
             "peak": PEAK[prec],
             "unit": "TFLOP/s",
             "frac": achieved * mfma_per_flop / PEAK[prec],
-            "traffic": None,
+            "traffic": measured_traffic(dom, args),
+            "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated)",
             "launches": d["launches"],
             "avg_launch_ms": d["ms"] / max(1, d["launches"]),
             "note": "achieved counts algorithmic flops; frac counts the MFMA work (x%d for split BF16 products)"
