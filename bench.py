@@ -120,6 +120,35 @@ def kernel_table(ks, steps):
     return out
 
 
+def mle_compare(egs, args):
+    """Maximum-likelihood fits of one data set with the FP64 and the mixed plan
+    (BOBYQA from the lower bounds, as ExaGeoStatDataModeling does)."""
+    n = args.n if args.n != 100000 else 40000
+    theta = args.theta or [1.0, 0.1, 1.0]
+    lb, ub = [0.01] * 3, [5.0] * 3
+    x, y = egs.generate_locations(n, seed=0)
+    ctx = egs.Context(0)
+    prob = egs.Problem(ctx, "UnivariateMaternStationary", x, y, dts=args.dts)
+    prob.generate_observations(theta, seed=0)
+    out = {"workload": "maximum-likelihood fit, n=%d, theta_true=%s, dts=%d, bounds [0.01, 5], ftol 1e-9, "
+                       "start at the lower bounds" % (n, theta, args.dts)}
+    for label, mode in (("fp64", None), ("mixed", egs.PRECISION_MIXED)):
+        if mode is not None:
+            prob.set_precision(mode, args.band_fp64, args.band_fp32)
+        t0 = time.perf_counter()
+        th, llh, nev = prob.data_modeling(lb, ub, tolerance=1e-9, max_iterations=args.maxeval)
+        dt = time.perf_counter() - t0
+        out[label] = {"theta_hat": list(th), "loglik": llh, "evaluations": nev, "seconds": dt}
+        log("%s: theta %s llh %.10f, %d evaluations, %.1f s" % (label, th, llh, nev, dt))
+    d = [abs(a - b) for a, b in zip(out["fp64"]["theta_hat"], out["mixed"]["theta_hat"])]
+    out["max_abs_theta_diff"] = max(d)
+    out["rel_loglik_diff"] = abs(out["mixed"]["loglik"] - out["fp64"]["loglik"]) / abs(out["fp64"]["loglik"])
+    out["mixed_plan"] = {"band_fp64": args.band_fp64, "band_fp32": args.band_fp32}
+    prob.close()
+    ctx.close()
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -133,6 +162,11 @@ def main():
     ap.add_argument("--plain-bf16", action="store_true", help="single BF16 product instead of the split one")
     ap.add_argument("--cpu-sample", type=int, default=6000)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--mle", action="store_true",
+                    help="fit theta with the FP64 and the mixed plan on the same data and report both estimates "
+                         "(defaults to n=40000, theta=(1, 0.1, 1.0), bounds [0.01, 5], ftol 1e-9)")
+    ap.add_argument("--theta", type=float, nargs=3, default=None)
+    ap.add_argument("--maxeval", type=int, default=500)
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a 1 GiB device-to-device copy (calibrates FETCH_SIZE/WRITE_SIZE counters)")
     args = ap.parse_args()
@@ -152,6 +186,9 @@ def main():
         sys.exit(2)
 
     import exageostatcpp_amd as egs
+
+    if args.mle:
+        return mle_compare(egs, args)
 
     theta = [1.0, 0.05, 0.5]
     t_setup = time.perf_counter()

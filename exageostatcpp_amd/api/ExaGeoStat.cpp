@@ -5,6 +5,8 @@
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <filesystem>
+#include <fstream>
 #include <iomanip>
 #include <iostream>
 #include <sstream>
@@ -160,6 +162,76 @@ template class BivariateMaternParsimonious<float>;
 template class BivariateMaternParsimonious<double>;
 
 }  // namespace kernels
+
+// ---------------------------------------------------------------- CSV data files
+namespace dataLoader {
+namespace csv {
+
+template<typename T>
+CSVLoader<T> *CSVLoader<T>::GetInstance() {
+    static CSVLoader<T> instance;
+    return &instance;
+}
+
+template<typename T>
+void CSVLoader<T>::ReadData(configurations::Configurations &aConfigurations, std::vector<T> &aMeasurementsMatrix,
+                            std::vector<T> &aXLocations, std::vector<T> &aYLocations, std::vector<T> &aZLocations,
+                            const int &aP) {
+    if (aConfigurations.GetDataPath().empty()) throw std::runtime_error("Please enter the path to data file.");
+    const std::string path = aConfigurations.GetDataPath();
+    const bool two_d = aConfigurations.GetDimension() == common::Dimension2D;
+    std::ifstream file(path);
+    if (!file.is_open()) throw std::runtime_error("Cannot read locations file: " + path);
+    std::string line;
+    int lines = 0;
+    while (std::getline(file, line)) {
+        if (line.empty()) continue;
+        std::vector<double> v;
+        std::istringstream iss(line);
+        std::string tok;
+        while (std::getline(iss, tok, ',')) v.push_back(std::stod(tok));
+        const size_t coords = two_d ? 2 : 3;
+        if (v.size() < coords + 1)
+            throw std::runtime_error("The data dimensions in the provided file isn't consistent with the dimensions input.");
+        if (v.size() < coords + aP)
+            throw std::runtime_error("The data P in the provided file isn't consistent with the Kernel's P.");
+        aXLocations.push_back(static_cast<T>(v[0]));
+        aYLocations.push_back(static_cast<T>(v[1]));
+        if (!two_d) aZLocations.push_back(static_cast<T>(v[2]));
+        for (int q = 0; q < aP; ++q) aMeasurementsMatrix.push_back(static_cast<T>(v[coords + q]));
+        ++lines;
+    }
+    aConfigurations.SetProblemSize(lines * aP / aConfigurations.GetTimeSlot());
+}
+
+template<typename T>
+void CSVLoader<T>::WriteData(const T &aMatrixPointer, const int &aProblemSize, const int &aP,
+                             std::string &aLoggerPath, dataunits::Locations<T> &aLocations) {
+    if (aLoggerPath.empty()) aLoggerPath = "synthetic_ds";
+    else aLoggerPath += (aLoggerPath.back() == '/') ? "synthetic_ds" : "/synthetic_ds";
+    std::error_code ec;
+    std::filesystem::create_directories(aLoggerPath, ec);
+    if (ec || !std::filesystem::exists(aLoggerPath)) throw std::runtime_error("Error creating directory: " + aLoggerPath);
+    const int nloc = aProblemSize / aP;
+    size_t i = 1;
+    const std::string log_base = aLoggerPath + "/log_" + std::to_string(nloc) + "_";
+    while (std::filesystem::exists(log_base + std::to_string(i))) ++i;
+    std::ofstream out(aLoggerPath + "/SYN_" + std::to_string(nloc) + "_" + std::to_string(i));
+    const T *z = &aMatrixPointer;
+    out << std::setprecision(15);
+    for (int l = 0; l < nloc; ++l) {
+        out << aLocations.GetLocationX()[l] << ',' << aLocations.GetLocationY()[l];
+        if (aLocations.GetLocationZ()) out << ',' << aLocations.GetLocationZ()[l];
+        for (int q = 0; q < aP; ++q) out << ',' << z[static_cast<size_t>(l) * aP + q];
+        out << '\n';
+    }
+}
+
+template class CSVLoader<float>;
+template class CSVLoader<double>;
+
+}  // namespace csv
+}  // namespace dataLoader
 
 // ---------------------------------------------------------------- results
 namespace results {
@@ -340,9 +412,24 @@ void ExaGeoStat<T>::ExaGeoStatLoadData(configurations::Configurations &aConfigur
                                        std::unique_ptr<ExaGeoStatData<T>> &aData) {
     std::srand(aConfigurations.GetSeed());
     auto kernel = CreateKernel<T>(aConfigurations);
-    const int n = aConfigurations.GetProblemSize();
     const Dimension dim = aConfigurations.GetDimension();
     if (dim == common::DimensionST) throw std::domain_error("space-time data is not on the MI355X path");
+    if (!aConfigurations.GetIsSynthetic()) {
+        // real data from a CSV file (CSVLoader<T>::LoadData)
+        std::vector<T> z, xs, ys, zs;
+        dataLoader::csv::CSVLoader<T>::GetInstance()->ReadData(aConfigurations, z, xs, ys, zs,
+                                                               kernel->GetVariablesNumber());
+        const int nloc = static_cast<int>(xs.size());
+        aData = std::make_unique<ExaGeoStatData<T>>(nloc, dim);
+        aData->GetLocations()->SetLocationX(*xs.data(), nloc);
+        aData->GetLocations()->SetLocationY(*ys.data(), nloc);
+        if (!zs.empty()) aData->GetLocations()->SetLocationZ(*zs.data(), nloc);
+        aData->GetMeasurements() = ToDouble(z);
+        results::Results::GetInstance()->SetIsSynthetic(false);
+        results::Results::GetInstance()->SetGeneratedLocationsNumber(nloc);
+        return;
+    }
+    const int n = aConfigurations.GetProblemSize();
     const double t0 = Seconds();
     std::vector<double> x(n), y(n), z(dim == common::Dimension3D ? n : 0);
     CheckStatus(egs_generate_locations(n, dim == common::Dimension3D ? 3 : 2, x.data(), y.data(),
@@ -364,6 +451,12 @@ void ExaGeoStat<T>::ExaGeoStatLoadData(configurations::Configurations &aConfigur
     obs.assign(static_cast<size_t>(n) * kernel->GetVariablesNumber(), 0.0);
     CheckStatus(egs_generate_observations(p, theta.data(), aConfigurations.GetSeed(), obs.data()),
                 "GenerateObservationsVector");
+    if (aConfigurations.GetLogger()) {
+        std::vector<T> zt(obs.begin(), obs.end());
+        std::string path = aConfigurations.GetLoggerPath();
+        dataLoader::csv::CSVLoader<T>::GetInstance()->WriteData(*zt.data(), static_cast<int>(zt.size()),
+                                                                kernel->GetVariablesNumber(), path, *loc);
+    }
     const double t1 = Seconds();
     const double nn = static_cast<double>(obs.size());
     auto *res = results::Results::GetInstance();

@@ -108,6 +108,14 @@ public:
     bool GetIsFisher() const { return mIsFisher; }
     void SetIsNonGaussian(bool a) { mIsNonGaussian = a; }
     bool GetIsNonGaussian() const { return mIsNonGaussian; }
+    // Real data: "x,y[,z],z_1[,z_2]" lines read by ExaGeoStatLoadData when the data is not synthetic.
+    void SetDataPath(const std::string &aPath) { mDataPath = aPath; }
+    const std::string &GetDataPath() const { return mDataPath; }
+    // Synthetic data is also written to <LoggerPath>/synthetic_ds/SYN_<n>_<i> when Logger is on.
+    void SetLogger(bool a) { mLogger = a; }
+    bool GetLogger() const { return mLogger; }
+    void SetLoggerPath(const std::string &aPath) { mLoggerPath = aPath; }
+    const std::string &GetLoggerPath() const { return mLoggerPath; }
 
     static void SetVerbosity(common::Verbose aVerbose) { mVerbosity = aVerbose; }
     static common::Verbose GetVerbosity() { return mVerbosity; }
@@ -138,6 +146,8 @@ private:
     double mTolerance = 0.1;
     int mZMiss = 0, mObservationNumber = 0;
     bool mIsMSPE = false, mIsIDW = false, mIsMLOEMMOM = false, mIsFisher = false, mIsNonGaussian = false;
+    std::string mDataPath, mLoggerPath;
+    bool mLogger = false;
     bool mIsThetaInit = false;
     static common::Verbose mVerbosity;
 };
@@ -309,6 +319,30 @@ private:
 };
 
 }  // namespace plugins
+
+namespace dataLoader {
+namespace csv {
+
+// CSV data files (src/data-loader/concrete/CSVLoader.cpp:36-234): one location
+// per line, "x,y,z_1..z_P" in 2D and "x,y,zloc,z_1..z_P" in 3D.
+template<typename T>
+class CSVLoader {
+public:
+    static CSVLoader<T> *GetInstance();
+    // Reads the file at aConfigurations.GetDataPath() and sets the problem size
+    // to lines * P / time slot; throws std::runtime_error with the reference's
+    // messages for a missing path, an unreadable file or inconsistent columns.
+    void ReadData(configurations::Configurations &aConfigurations, std::vector<T> &aMeasurementsMatrix,
+                  std::vector<T> &aXLocations, std::vector<T> &aYLocations, std::vector<T> &aZLocations,
+                  const int &aP);
+    // Writes <aLoggerPath>/synthetic_ds/SYN_<n/P>_<i> with 15 significant digits,
+    // i the first index whose log_<n/P>_<i> file does not exist.
+    void WriteData(const T &aMatrixPointer, const int &aProblemSize, const int &aP, std::string &aLoggerPath,
+                   dataunits::Locations<T> &aLocations);
+};
+
+}  // namespace csv
+}  // namespace dataLoader
 
 namespace results {
 

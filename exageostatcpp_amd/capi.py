@@ -84,6 +84,14 @@ EXPORTS = {
     "egs_bobyqa": ([_int, _dp, _dp, _dp, ctypes.c_double, _int, ctypes.c_double, _vp, _vp, _dp,
                     ctypes.POINTER(_int), ctypes.POINTER(_int)], _int),
     "egs_data_modeling": ([_vp, _dp, _dp, _dp, ctypes.c_double, _int, _dp, _dp, ctypes.POINTER(_int)], _int),
+    "egs_server_create": ([_int, _i64, _int, ctypes.POINTER(_vp)], _int),
+    "egs_server_destroy": ([_vp], _int),
+    "egs_server_add_region": ([_vp, _int, _i64, _dp, _dp, _dp, _dp, _int, _int, ctypes.POINTER(_int)], _int),
+    "egs_server_submit_loglik": ([_vp, _int, _dp, ctypes.POINTER(_i64)], _int),
+    "egs_server_submit_predict": ([_vp, _int, _dp, _i64, _dp, _dp, ctypes.POINTER(_i64)], _int),
+    "egs_server_wait": ([_vp, _i64, _dp, _dp, _i64, _dp], _int),
+    "egs_server_stats": ([_vp, ctypes.POINTER(_i64), _dp, _dp, _dp, ctypes.POINTER(_i64)], _int),
+    "egs_server_reset_stats": ([_vp], _int),
 }
 
 OBJECTIVE_FN = ctypes.CFUNCTYPE(ctypes.c_double, _dp, _vp)
@@ -305,6 +313,71 @@ def bobyqa(fn, x0, lb, ub, ftol_abs=0.0, maxeval=1000, rho_end=0.0):
         raise err[0]
     _check(code)
     return x, fmin.value, nev.value, reason.value
+
+
+class Server:
+    """Request loop over many regions on one GPU (egs_server_*)."""
+
+    def __init__(self, device=0, hbm_budget_bytes=64 << 30, workers=4):
+        h = _vp()
+        _check(lib().egs_server_create(device, int(hbm_budget_bytes), workers, ctypes.byref(h)))
+        self.handle = h
+        self._p = {}
+
+    def close(self):
+        if self.handle:
+            lib().egs_server_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def add_region(self, kernel, x, y, obs, z=None, metric=0, dts=320):
+        x, px = _arr(x)
+        y, py = _arr(y)
+        z, pz = _arr(z)
+        obs, po = _arr(obs)
+        rid = _int()
+        _check(lib().egs_server_add_region(self.handle, _kid(kernel), len(x), px, py, pz, po, metric, dts,
+                                           ctypes.byref(rid)))
+        self._p[rid.value] = 2 if _kid(kernel) == KERNEL_BIVARIATE else 1
+        return rid.value
+
+    def submit_loglik(self, region, theta):
+        th, pth = _arr(theta)
+        t = _i64()
+        _check(lib().egs_server_submit_loglik(self.handle, region, pth, ctypes.byref(t)))
+        return (t.value, None)
+
+    def submit_predict(self, region, theta, xm, ym):
+        th, pth = _arr(theta)
+        xm, pxm = _arr(xm)
+        ym, pym = _arr(ym)
+        t = _i64()
+        _check(lib().egs_server_submit_predict(self.handle, region, pth, len(xm), pxm, pym, ctypes.byref(t)))
+        return (t.value, len(xm) * self._p[region])
+
+    def wait(self, ticket):
+        """Returns (value, zmiss or None, latency_ms) of a submitted request."""
+        t, nz = ticket
+        v, lat = ctypes.c_double(), ctypes.c_double()
+        out = np.empty(nz) if nz else None
+        _check(lib().egs_server_wait(self.handle, t, ctypes.byref(v), None if out is None else out.ctypes.data_as(_dp),
+                                     nz or 0, ctypes.byref(lat)))
+        return v.value, out, lat.value
+
+    def stats(self):
+        c, p50, p99, ps, rb = _i64(), ctypes.c_double(), ctypes.c_double(), ctypes.c_double(), _i64()
+        _check(lib().egs_server_stats(self.handle, ctypes.byref(c), ctypes.byref(p50), ctypes.byref(p99),
+                                      ctypes.byref(ps), ctypes.byref(rb)))
+        return {"completed": c.value, "p50_ms": p50.value, "p99_ms": p99.value, "per_second": ps.value,
+                "resident_bytes": rb.value}
+
+    def reset_stats(self):
+        _check(lib().egs_server_reset_stats(self.handle))
 
 
 def srand(seed):

@@ -47,6 +47,10 @@ constexpr int kMaxTileSize = 2048;
 
 }  // namespace
 
+namespace egs {
+void set_last_error(const char *msg) { g_last_error = msg; }
+}  // namespace egs
+
 struct egs_problem_s;
 
 struct egs_context_s {
@@ -89,6 +93,13 @@ struct egs_problem_s {
     float *d_p32[2] = {nullptr, nullptr};
     __bf16 *d_p16h[2] = {nullptr, nullptr}, *d_p16l[2] = {nullptr, nullptr};
     double *d_cg = nullptr;                  // refinement vectors x, r, p, q, w (npad each)
+    // FP64 copy of the lower tiles of Sigma written with every covariance
+    // generation of a mixed plan (the refinement's Sigma v reads it), and the
+    // per-tile partials of that product; null when the copy does not fit
+    double *d_sig64 = nullptr, *d_prow = nullptr, *d_pcol = nullptr;
+    // inverses X_k = L_kk^{-T} of the diagonal tiles (preconditioner solves)
+    // and the solve's output vector
+    double *d_tinv = nullptr, *d_ybuf = nullptr;
     double *d_mv = nullptr;                  // matrix-vector chunk partials
     double *d_dot = nullptr;                 // dot-product block partials
     int refine_iters = 0;
@@ -101,6 +112,7 @@ struct egs_problem_s {
     uint8_t *d_store = nullptr;
     int2 *d_pairs = nullptr;
     double *d_winv = nullptr;
+    double *d_dscr = nullptr;   // ts * ts scratch of the diagonal-tile factorization
     double *d_obs = nullptr, *d_work = nullptr, *d_aux = nullptr, *d_partials = nullptr;
     int *d_info = nullptr;
 
@@ -191,6 +203,19 @@ int make_params(int kernel, int metric, const double *th, egs::MaternParams &prm
     return fail(EGS_ERR_INVALID_ARG, "unknown kernel %d", kernel);
 }
 
+void free_sigma_copy(egs_problem_s *p) {
+    hipFree(p->d_sig64);
+    hipFree(p->d_prow);
+    hipFree(p->d_pcol);
+    p->d_sig64 = p->d_prow = p->d_pcol = nullptr;
+}
+
+void free_tile_inverses(egs_problem_s *p) {
+    hipFree(p->d_tinv);
+    hipFree(p->d_ybuf);
+    p->d_tinv = p->d_ybuf = nullptr;
+}
+
 void free_problem_buffers(egs_problem_s *p) {
     if (p->ctx) {
         auto &v = p->ctx->problems;
@@ -204,6 +229,7 @@ void free_problem_buffers(egs_problem_s *p) {
     hipFree(p->d_store);
     hipFree(p->d_pairs);
     hipFree(p->d_winv);
+    hipFree(p->d_dscr);
     hipFree(p->d_obs);
     hipFree(p->d_work);
     hipFree(p->d_aux);
@@ -226,6 +252,8 @@ void free_problem_buffers(egs_problem_s *p) {
     hipFree(p->d_mv);
     hipFree(p->d_dot);
     p->d_cg = p->d_mv = p->d_dot = nullptr;
+    free_sigma_copy(p);
+    free_tile_inverses(p);
     for (auto &e : p->ev)
         if (e) hipEventDestroy(e);
     for (auto &e : p->prof.ev)
@@ -243,7 +271,7 @@ void free_problem_buffers(egs_problem_s *p) {
     p->d_off = nullptr;
     p->d_store = nullptr;
     p->d_pairs = nullptr;
-    p->d_winv = p->d_obs = p->d_work = p->d_aux = p->d_partials = nullptr;
+    p->d_winv = p->d_dscr = p->d_obs = p->d_work = p->d_aux = p->d_partials = nullptr;
     p->d_info = nullptr;
     for (auto &e : p->ev) e = nullptr;
 }
@@ -341,6 +369,24 @@ int setup_mixed(egs_problem_s *p) {
             hipMalloc(&p->d_dot, sizeof(double) * egs::dot_blocks()) != hipSuccess)
             return fail(EGS_ERR_OUT_OF_MEMORY, "cannot allocate the mixed-precision work buffers");
     }
+    if (!p->d_tinv) {
+        if (hipMalloc(&p->d_tinv, sizeof(double) * (size_t) p->nt * p->ts * p->ts) != hipSuccess ||
+            hipMalloc(&p->d_ybuf, sizeof(double) * (size_t) p->npad) != hipSuccess) {
+            free_tile_inverses(p);
+            return fail(EGS_ERR_OUT_OF_MEMORY, "cannot allocate the diagonal-tile inverses");
+        }
+    }
+    if (!p->d_sig64) {
+        // Optional: without it Sigma v regenerates the covariance on the fly.
+        const size_t ntiles = (size_t) p->nt * (p->nt + 1) / 2;
+        const size_t tile = (size_t) p->ts * p->ts;
+        if (hipMalloc(&p->d_sig64, ntiles * tile * sizeof(double)) != hipSuccess ||
+            hipMalloc(&p->d_prow, ntiles * p->ts * sizeof(double)) != hipSuccess ||
+            hipMalloc(&p->d_pcol, ntiles * p->ts * sizeof(double)) != hipSuccess) {
+            (void) hipGetLastError();
+            free_sigma_copy(p);
+        }
+    }
     return EGS_SUCCESS;
 }
 
@@ -384,7 +430,7 @@ int run_covariance(egs_problem_s *p, const double *theta) {
     const double *dz = (p->kernel == EGS_KERNEL_UNIVARIATE_MATERN_STATIONARY) ? p->d_z : nullptr;
     const int pi = prof_begin(p, EGS_PROFILE_COVARIANCE, 0.0);
     egs::launch_covgen_tiles(p->map(), p->d_pairs, (int) p->h_pairs.size(), prm, p->d_x, p->d_y, dz, p->n,
-                             p->ctx->stream);
+                             is_mixed(p) ? p->d_sig64 : nullptr, p->ctx->stream);
     prof_end(p, pi);
     EGS_HIP(hipGetLastError());
     p->factored = false;
@@ -413,10 +459,7 @@ void factor_panel(egs_problem_s *p, int k) {
     double *wk = p->winv(k);
     const double kb = real_rows(p, k);
     int pi = prof_begin(p, EGS_PROFILE_DIAGONAL, kb * kb * kb / 3.0);
-    for (int jb = 0; jb < nsb; ++jb) {
-        egs::launch_potrf_block(m, k, jb, wk, p->d_info, p->n, s);
-        if (jb + 1 < nsb) egs::launch_trsm_rows(m, k, (int64_t) k * nsb + jb + 1, nsb - jb - 1, jb, jb + 1, wk, s);
-    }
+    egs::launch_factor_diag(m, k, wk, p->d_dscr, p->d_info, p->n, s);
     prof_end(p, pi);
     if (k + 1 < nt) {
         double rows_below = 0.0;
@@ -528,6 +571,7 @@ int run_potrf(egs_problem_s *p) {
         EGS_HIP(hipEventRecord(p->ev_panel[k + 1], s));
     }
     if (nt >= 2) EGS_HIP(hipStreamWaitEvent(s, p->ev_bulk[nt - 2], 0));
+    if (is_mixed(p)) egs::launch_tinv(m, p->d_winv, p->d_tinv, s);
     EGS_HIP(hipGetLastError());
     return EGS_SUCCESS;
 }
@@ -552,7 +596,7 @@ void run_forward(egs_problem_s *p, double *partials = nullptr) {
     const int pi = prof_begin(p, EGS_PROFILE_SOLVE, (double) p->n * (double) p->n);
     for (int k = 0; k < p->nt; ++k) {
         egs::launch_trsv_diag(m, k, p->winv(k), p->d_work, partials, s);
-        egs::launch_trsv_panel(m, k, p->d_work, s);
+        egs::launch_fwd_panel(m, k, p->d_work + (int64_t) k * p->ts, p->d_work, s);
     }
     prof_end(p, pi);
 }
@@ -566,13 +610,47 @@ void run_backward(egs_problem_s *p) {
     }
 }
 
+// Preconditioner solves of mixed plans with the diagonal-tile inverses:
+// d_ybuf = L^{-1} d_work (d_work is consumed), then d_ybuf = L^{-T} d_work.
+void run_forward_inv(egs_problem_s *p) {
+    hipStream_t s = p->ctx->stream;
+    const egs::TileMap m = p->map();
+    const int pi = prof_begin(p, EGS_PROFILE_SOLVE, (double) p->n * (double) p->n);
+    for (int k = 0; k < p->nt; ++k) {
+        const int64_t o = (int64_t) k * p->ts;
+        egs::launch_fwd_diag_inv(p->d_tinv + o * p->ts, p->d_work + o, p->d_ybuf + o, p->ts, s);
+        egs::launch_fwd_panel(m, k, p->d_ybuf + o, p->d_work, s);
+    }
+    prof_end(p, pi);
+}
+
+void run_backward_inv(egs_problem_s *p) {
+    hipStream_t s = p->ctx->stream;
+    const egs::TileMap m = p->map();
+    const int pi = prof_begin(p, EGS_PROFILE_SOLVE, (double) p->n * (double) p->n);
+    for (int k = p->nt - 1; k >= 0; --k) {
+        const int64_t o = (int64_t) k * p->ts;
+        egs::launch_bwd_diag_inv(p->d_tinv + o * p->ts, p->d_work + o, p->d_ybuf + o, p->ts, s);
+        egs::launch_bwd_row(m, k, p->d_ybuf + o, p->d_work, s);
+    }
+    prof_end(p, pi);
+}
+
 // out = (L L^T)^{-1} in with the current factor (both vectors npad long).
 void apply_factor_inverse(egs_problem_s *p, const double *in, double *out) {
     hipStream_t s = p->ctx->stream;
-    hipMemcpyAsync(p->d_work, in, sizeof(double) * (size_t) p->npad, hipMemcpyDeviceToDevice, s);
+    const size_t bytes = sizeof(double) * (size_t) p->npad;
+    hipMemcpyAsync(p->d_work, in, bytes, hipMemcpyDeviceToDevice, s);
+    if (p->d_tinv) {
+        run_forward_inv(p);
+        hipMemcpyAsync(p->d_work, p->d_ybuf, bytes, hipMemcpyDeviceToDevice, s);
+        run_backward_inv(p);
+        hipMemcpyAsync(out, p->d_ybuf, bytes, hipMemcpyDeviceToDevice, s);
+        return;
+    }
     run_forward(p, p->d_mv);   // partials of a preconditioner solve are not needed
     run_backward(p);
-    hipMemcpyAsync(out, p->d_work, sizeof(double) * (size_t) p->npad, hipMemcpyDeviceToDevice, s);
+    hipMemcpyAsync(out, p->d_work, bytes, hipMemcpyDeviceToDevice, s);
 }
 
 int host_dot(egs_problem_s *p, const double *x, const double *y, double &out) {
@@ -590,6 +668,15 @@ int host_dot(egs_problem_s *p, const double *x, const double *y, double &out) {
 // q = Sigma v in FP64 (covariance regenerated from the locations at the
 // problem's current theta); q = base - Sigma v when base is given.
 int sigma_times(egs_problem_s *p, const double *v, double *q, const double *base) {
+    if (p->d_sig64) {
+        // FP64 tiles written by the last covariance generation (at cov_theta)
+        egs::launch_sigma_tile_matvec(p->d_sig64, p->d_pairs, (int) p->h_pairs.size(), p->ts, v, p->d_prow, p->d_pcol,
+                                      p->ctx->stream);
+        egs::launch_sigma_reduce(q, base, base ? -1.0 : 1.0, p->d_prow, p->d_pcol, p->nt, p->ts, p->n, p->npad,
+                                 p->ctx->stream);
+        EGS_HIP(hipGetLastError());
+        return EGS_SUCCESS;
+    }
     egs::MaternParams prm;
     int st = make_params(p->kernel, p->metric, p->cov_theta, prm);
     if (st) return st;
@@ -813,6 +900,7 @@ int egs_problem_create(egs_context_t ctx, int kernel, int64_t n_loc, const doubl
               hipMalloc(&p->d_store, (size_t) nt * nt) == hipSuccess &&
               hipMalloc(&p->d_pairs, sizeof(int2) * p->h_pairs.size()) == hipSuccess &&
               hipMalloc(&p->d_winv, sizeof(double) * (size_t) nt * p->nsb * egs::kBlk * egs::kBlk) == hipSuccess &&
+              hipMalloc(&p->d_dscr, sizeof(double) * (size_t) p->ts * p->ts) == hipSuccess &&
               hipMalloc(&p->d_obs, vb) == hipSuccess && hipMalloc(&p->d_work, vb) == hipSuccess &&
               hipMalloc(&p->d_aux, vb) == hipSuccess &&
               hipMalloc(&p->d_partials, sizeof(double) * 2 * (size_t) nt) == hipSuccess &&
@@ -871,6 +959,10 @@ int egs_problem_set_precision(egs_problem_t p, int mode, int band_fp64, int band
     p->band32 = band_fp32;
     int st2 = layout_tiles(p);
     if (st2) return st2;
+    if (!is_mixed(p)) {
+        free_sigma_copy(p);
+        free_tile_inverses(p);
+    }
     return is_mixed(p) ? setup_mixed(p) : EGS_SUCCESS;
 }
 
@@ -920,7 +1012,8 @@ int egs_solve_logdet_quad(egs_problem_t p, double *logdet, double *quad) {
     if (!p->has_obs) return fail(EGS_ERR_STATE, "the problem has no observations");
     hipStream_t s = p->ctx->stream;
     egs::launch_copy_pad(p->d_work, p->d_obs, p->n, p->npad, s);
-    run_forward(p);
+    if (is_mixed(p)) egs::launch_diag_logdet(p->map(), p->n, p->d_partials, s);   // quad from the refinement
+    else run_forward(p);
     EGS_HIP(hipGetLastError());
     EGS_HIP(hipStreamSynchronize(s));
     double ld, q;
@@ -949,7 +1042,8 @@ int egs_loglik(egs_problem_t p, const double *theta, double *loglik, double *log
     st = run_potrf(p);
     if (st) return st;
     EGS_HIP(hipEventRecord(p->ev[2], s));
-    run_forward(p);
+    if (is_mixed(p)) egs::launch_diag_logdet(p->map(), p->n, p->d_partials, s);   // quad from the refinement
+    else run_forward(p);
     EGS_HIP(hipGetLastError());
     EGS_HIP(hipEventRecord(p->ev[3], s));
     EGS_HIP(hipStreamSynchronize(s));

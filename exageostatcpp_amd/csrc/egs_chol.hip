@@ -279,4 +279,288 @@ void launch_update(const TileMap &map, const int2 *pairs, int npairs, int k, hip
     hipLaunchKernelGGL(update_kernel, dim3(nsb * nsb, npairs), dim3(kThreads), 0, s, map, pairs, k);
 }
 
+// ------------------------------------------------------------------------
+// Inverses of the diagonal tiles, X_k = L_kk^{-T} (upper triangular,
+// column-major, ts x ts), for the preconditioner solves of mixed plans.
+// Workgroup (rb, k) solves row block rb of X_k L_kk^T = I left-looking over
+// the 64-column blocks jb >= rb (blocks left of rb are zero), with the 64 x 64
+// inverses W of the diagonal blocks of L_kk.
+// ------------------------------------------------------------------------
+__global__ void __launch_bounds__(kThreads) tinv_kernel(TileMap map, const double *winv, double *tinv) {
+    __shared__ GemmSmem sm;
+    __shared__ double sT[kBlk][kLdsPad];
+    const int ts = map.ts;
+    const int nsb = ts / kBlk;
+    const int rb = blockIdx.x, k = blockIdx.y;
+    double *X = tinv + (int64_t) k * ts * ts + rb * kBlk;
+    const double *L = tile_f64(map, k, k);
+    const double *winv_k = winv + (int64_t) k * nsb * kBlk * kBlk;
+    const int lane = threadIdx.x & 63;
+    const int w = threadIdx.x >> 6;
+    const int mq = w & 1, nq = w >> 1;
+    const int lr = lane & 15, lk = lane >> 4;
+    for (int e = threadIdx.x; e < kBlk * ts; e += kThreads) {
+        const int r = e & 63, c = e >> 6;
+        X[r + (int64_t) c * ts] = (c == rb * kBlk + r) ? 1.0 : 0.0;
+    }
+    __syncthreads();
+    for (int jb = rb; jb < nsb; ++jb) {
+        Acc64 acc;
+        acc_zero(acc);
+        if (jb > rb)
+            wg_gemm_nt(acc, X + (int64_t) rb * kBlk * ts, ts, L + jb * kBlk + (int64_t) rb * kBlk * ts, ts,
+                       (jb - rb) * kBlk, sm);
+        double *Xj = X + (int64_t) jb * kBlk * ts;
+        acc_foreach(acc, [&](int r, int c, double v) { sT[r][c] = Xj[r + (int64_t) c * ts] - v; });
+        __syncthreads();
+        const double *W = winv_k + (int64_t) jb * kBlk * kBlk;
+        Acc64 acc2;
+        acc_zero(acc2);
+#pragma unroll 4
+        for (int kk = 0; kk < kBlk; kk += 4) {
+            const double a0 = sT[mq * 32 + lr][kk + lk];
+            const double a1 = sT[mq * 32 + 16 + lr][kk + lk];
+            const double b0 = W[(nq * 32 + lr) + (kk + lk) * kBlk];
+            const double b1 = W[(nq * 32 + 16 + lr) + (kk + lk) * kBlk];
+            acc2.v[0][0] = mfma_f64(b0, a0, acc2.v[0][0]);
+            acc2.v[0][1] = mfma_f64(b0, a1, acc2.v[0][1]);
+            acc2.v[1][0] = mfma_f64(b1, a0, acc2.v[1][0]);
+            acc2.v[1][1] = mfma_f64(b1, a1, acc2.v[1][1]);
+        }
+        acc_foreach(acc2, [&](int r, int c, double v) { Xj[r + (int64_t) c * ts] = v; });
+        __syncthreads();
+    }
+}
+
+void launch_tinv(const TileMap &map, const double *winv, double *tinv, hipStream_t s) {
+    const int nsb = map.ts / kBlk;
+    hipLaunchKernelGGL(tinv_kernel, dim3(nsb, map.nt), dim3(kThreads), 0, s, map, winv, tinv);
+}
+
+// ------------------------------------------------------------------------
+// Diagonal tile, right-looking in 64 x 64 blocks.  Step jb:
+//   potrf64_kernel      factors block (jb, jb) (already updated) and inverts it;
+//   diag_update_kernel  one workgroup per block (r, c), jb < c <= r: forms
+//                       L_r = A_r,jb W_jb^T and L_c = A_c,jb W_jb^T and applies
+//                       A_rc -= L_r L_c^T; the diagonal workgroups (r, r) also
+//                       write L_r to a scratch copy of the tile (the tile's own
+//                       block (r, jb) is still being read by other workgroups);
+//   diag_copy_kernel    at the end moves the solved blocks below the diagonal
+//                       from the scratch copy into the tile.
+// ------------------------------------------------------------------------
+namespace {
+__device__ __forceinline__ double pick4(const double (&d)[4], int q) {
+    return q == 0 ? d[0] : (q == 1 ? d[1] : (q == 2 ? d[2] : d[3]));
+}
+}  // namespace
+
+__global__ void __launch_bounds__(kThreads) potrf64_kernel(TileMap map, int k, int jb, double *winv_k, int *info,
+                                                           int64_t n) {
+    __shared__ double sA[kBlk][kLdsPad];
+    __shared__ double sW[kBlk][kLdsPad];
+    __shared__ double sTmp[3 * 16 * 16 + 48 * 16];
+    const int ts = map.ts;
+    const int t = threadIdx.x;
+    const int lane = t & 63, w = t >> 6;
+    double *Ajj = tile_f64(map, k, k) + (int64_t) jb * kBlk + (int64_t) jb * kBlk * ts;
+    for (int e = t; e < kBlk * kBlk; e += kThreads) {
+        const int r = e & 63, c = e >> 6;
+        sA[r][c] = Ajj[r + (int64_t) c * ts];
+        sW[r][c] = 0.0;
+    }
+    __syncthreads();
+    for (int b = 0; b < 4; ++b) {
+        const int o = 16 * b;
+        // (1) 16 x 16 diagonal block in the registers of wave 0:
+        //     lane l holds row i = l % 16, columns 4 (l / 16) .. 4 (l / 16) + 3
+        if (w == 0) {
+            const int i = lane & 15, jq = lane >> 4;
+            double d[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) d[q] = sA[o + i][o + 4 * jq + q];
+#pragma unroll
+            for (int j = 0; j < 16; ++j) {
+                const int jg = j >> 2, js = j & 3;
+                const double piv = __shfl(pick4(d, js), j + 16 * jg, 64);
+                if (lane == 0 && !(piv > 0.0)) {
+                    const int64_t col = (int64_t) k * ts + (int64_t) jb * kBlk + o + j;
+                    if (col < n) atomicCAS(info, 0, (int) (col + 1));
+                }
+                const double sq = sqrt(piv);
+                const double inv = 1.0 / sq;
+                if (jq == jg) {
+                    if (i == j) d[js] = sq;
+                    else if (i > j) d[js] *= inv;
+                }
+                const double lij = __shfl(pick4(d, js), i + 16 * jg, 64);
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const int c = 4 * jq + q;
+                    const double lcj = __shfl(pick4(d, js), c + 16 * jg, 64);
+                    if (c > j && i >= c) d[q] -= lij * lcj;
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < 4; ++q) sA[o + i][o + 4 * jq + q] = d[q];
+        }
+        __syncthreads();
+        // (2) inverse of the 16 x 16 factor: thread c < 16 solves column c
+        if (t < 16) {
+            const int c = t;
+            for (int r = c; r < 16; ++r) {
+                double acc = (r == c) ? 1.0 : 0.0;
+                for (int l = c; l < r; ++l) acc -= sA[o + r][o + l] * sW[o + l][o + c];
+                sW[o + r][o + c] = acc / sA[o + r][o + r];
+            }
+        }
+        __syncthreads();
+        const int m = 48 - o;   // rows below the 16 x 16 block
+        if (m > 0) {
+            // (3) L[r][o + c] = sum_{l <= c} A[r][o + l] Dinv[c][l] for the rows below
+            for (int e = t; e < m * 16; e += kThreads) {
+                const int r = o + 16 + (e >> 4), c = e & 15;
+                double acc = 0.0;
+                for (int l = 0; l <= c; ++l) acc += sA[r][o + l] * sW[o + c][o + l];
+                sTmp[e] = acc;
+            }
+            __syncthreads();
+            for (int e = t; e < m * 16; e += kThreads) sA[o + 16 + (e >> 4)][o + (e & 15)] = sTmp[e];
+            __syncthreads();
+            // (4) trailing lower part: A[r][c] -= sum_l L[r][o + l] L[c][o + l]
+            for (int e = t; e < m * m; e += kThreads) {
+                const int r = o + 16 + e % m, c = o + 16 + e / m;
+                if (c > r) continue;
+                double acc = 0.0;
+#pragma unroll
+                for (int l = 0; l < 16; ++l) acc += sA[r][o + l] * sA[c][o + l];
+                sA[r][c] -= acc;
+            }
+            __syncthreads();
+        }
+    }
+    // off-diagonal 16 x 16 blocks of W = L^{-1}, by distance d = rB - cB:
+    // W_rc = -Dinv_r sum_{l = c}^{r - 1} L_rl W_lc
+    for (int d = 1; d < 4; ++d) {
+        const int nb = 4 - d;
+        for (int e = t; e < nb * 256; e += kThreads) {
+            const int cB = e >> 8, rB = cB + d;
+            const int rr = e & 15, cc = (e >> 4) & 15;
+            double acc = 0.0;
+            for (int l = 16 * cB; l < 16 * rB; ++l) acc += sA[16 * rB + rr][l] * sW[l][16 * cB + cc];
+            sTmp[e] = acc;
+        }
+        __syncthreads();
+        for (int e = t; e < nb * 256; e += kThreads) {
+            const int cB = e >> 8, rB = cB + d;
+            const int rr = e & 15, cc = (e >> 4) & 15;
+            double acc = 0.0;
+            for (int l = 0; l <= rr; ++l) acc += sW[16 * rB + rr][16 * rB + l] * sTmp[(cB << 8) + (cc << 4) + l];
+            sW[16 * rB + rr][16 * cB + cc] = -acc;
+        }
+        __syncthreads();
+    }
+    double *W = winv_k + (int64_t) jb * kBlk * kBlk;
+    for (int e = t; e < kBlk * kBlk; e += kThreads) {
+        const int r = e & 63, c = e >> 6;
+        Ajj[r + (int64_t) c * ts] = (r >= c) ? sA[r][c] : 0.0;
+        W[r + c * kBlk] = (r >= c) ? sW[r][c] : 0.0;
+    }
+}
+
+// acc = X (64 x 64, column-major, ldx) W^T, W the 64 x 64 lower inverse (ld 64).
+__device__ __forceinline__ void times_wt(Acc64 &acc, const double *X, int64_t ldx, const double *W, GemmSmem &sm) {
+    acc_zero(acc);
+    wg_gemm_nt(acc, X, ldx, W, kBlk, kBlk, sm);
+}
+
+__global__ void __launch_bounds__(kThreads) diag_update_kernel(TileMap map, int k, int jb, const double *winv_k,
+                                                               double *scratch) {
+    __shared__ GemmSmem sm;
+    __shared__ double sLr[kBlk][kLdsPad];
+    __shared__ double sLc[kBlk][kLdsPad];
+    const int ts = map.ts;
+    const int nsb = ts / kBlk;
+    // blockIdx.x enumerates the pairs jb < c <= r < nsb column by column
+    int idx = blockIdx.x, c = jb + 1;
+    while (idx >= nsb - c) {
+        idx -= nsb - c;
+        ++c;
+    }
+    const int r = c + idx;
+    double *T = tile_f64(map, k, k);
+    const double *W = winv_k + (int64_t) jb * kBlk * kBlk;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int mq = wv & 1, nq = wv >> 1;
+    const int lr = lane & 15, lk = lane >> 4;
+    Acc64 acc;
+    times_wt(acc, T + r * kBlk + (int64_t) jb * kBlk * ts, ts, W, sm);
+    acc_foreach(acc, [&](int rr, int cc, double v) { sLr[rr][cc] = v; });
+    if (c != r) {
+        times_wt(acc, T + c * kBlk + (int64_t) jb * kBlk * ts, ts, W, sm);
+        acc_foreach(acc, [&](int rr, int cc, double v) { sLc[rr][cc] = v; });
+    }
+    __syncthreads();
+    const double (*Lc)[kLdsPad] = (c != r) ? sLc : sLr;
+    // A_rc -= L_r L_c^T
+    Acc64 upd;
+    acc_zero(upd);
+#pragma unroll 4
+    for (int kk = 0; kk < kBlk; kk += 4) {
+        const double a0 = sLr[mq * 32 + lr][kk + lk];
+        const double a1 = sLr[mq * 32 + 16 + lr][kk + lk];
+        const double b0 = Lc[nq * 32 + lr][kk + lk];
+        const double b1 = Lc[nq * 32 + 16 + lr][kk + lk];
+        upd.v[0][0] = mfma_f64(b0, a0, upd.v[0][0]);
+        upd.v[0][1] = mfma_f64(b0, a1, upd.v[0][1]);
+        upd.v[1][0] = mfma_f64(b1, a0, upd.v[1][0]);
+        upd.v[1][1] = mfma_f64(b1, a1, upd.v[1][1]);
+    }
+    double *C = T + r * kBlk + (int64_t) c * kBlk * ts;
+    double cv[16];
+    int q = 0;
+    acc_foreach(upd, [&](int rr, int cc, double) { cv[q++] = C[rr + (int64_t) cc * ts]; });
+    q = 0;
+    acc_foreach(upd, [&](int rr, int cc, double v) { C[rr + (int64_t) cc * ts] = cv[q++] - v; });
+    if (c == r) {
+        double *S = scratch + r * kBlk + (int64_t) jb * kBlk * ts;
+        for (int e = threadIdx.x; e < kBlk * kBlk; e += kThreads) {
+            const int rr = e & 63, cc = e >> 6;
+            S[rr + (int64_t) cc * ts] = sLr[rr][cc];
+        }
+    }
+}
+
+// Blocks (r, c), r > c, of the scratch copy back into diagonal tile k.
+__global__ void __launch_bounds__(kThreads) diag_copy_kernel(TileMap map, int k, const double *scratch) {
+    const int ts = map.ts;
+    const int nsb = ts / kBlk;
+    int idx = blockIdx.x, c = 0;
+    while (idx >= nsb - 1 - c) {
+        idx -= nsb - 1 - c;
+        ++c;
+    }
+    const int r = c + 1 + idx;
+    double *T = tile_f64(map, k, k);
+    const int64_t base = r * kBlk + (int64_t) c * kBlk * ts;
+    for (int e = threadIdx.x; e < kBlk * kBlk; e += kThreads) {
+        const int rr = e & 63, cc = e >> 6;
+        T[base + rr + (int64_t) cc * ts] = scratch[base + rr + (int64_t) cc * ts];
+    }
+}
+
+void launch_factor_diag(const TileMap &map, int k, double *winv_k, double *scratch, int *info, int64_t n,
+                        hipStream_t s) {
+    const int nsb = map.ts / kBlk;
+    for (int jb = 0; jb < nsb; ++jb) {
+        hipLaunchKernelGGL(potrf64_kernel, dim3(1), dim3(kThreads), 0, s, map, k, jb, winv_k, info, n);
+        const int rem = nsb - jb - 1;
+        if (rem > 0)
+            hipLaunchKernelGGL(diag_update_kernel, dim3((unsigned) (rem * (rem + 1) / 2)), dim3(kThreads), 0, s, map,
+                               k, jb, winv_k, scratch);
+    }
+    if (nsb > 1)
+        hipLaunchKernelGGL(diag_copy_kernel, dim3((unsigned) (nsb * (nsb - 1) / 2)), dim3(kThreads), 0, s, map, k,
+                           scratch);
+}
+
 }  // namespace egs

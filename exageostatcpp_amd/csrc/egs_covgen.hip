@@ -21,6 +21,7 @@ struct CovTilesArgs {
     Locs loc;
     int64_t n;              // true matrix dimension; rows/cols >= n are identity padding
     int three_d;
+    double *sig64;          // optional FP64 copy of every lower tile (tile i*(i+1)/2 + j, ts*ts each)
 };
 
 __global__ void __launch_bounds__(kThreads) covgen_tiles_kernel(CovTilesArgs a) {
@@ -63,6 +64,7 @@ __global__ void __launch_bounds__(kThreads) covgen_tiles_kernel(CovTilesArgs a) 
     const int rvar = (int) (gr & 1);
     const uint8_t store = a.map.store[ij.x * a.map.nt + ij.y];
     char *tile = a.map.base + a.map.off[ij.x * a.map.nt + ij.y];
+    const int64_t sig_base = ((int64_t) ij.x * (ij.x + 1) / 2 + ij.y) * ts * ts;
     for (int cc = t / kBlk; cc < kBlk; cc += kThreads / kBlk) {
         const int64_t gc = c0 + cc;
         double v;
@@ -77,6 +79,7 @@ __global__ void __launch_bounds__(kThreads) covgen_tiles_kernel(CovTilesArgs a) 
         const int64_t idx = (int64_t) (sbr * kBlk + r) + (int64_t) (sbc * kBlk + cc) * ts;
         if (store == kStoreF64) reinterpret_cast<double *>(tile)[idx] = v;
         else reinterpret_cast<float *>(tile)[idx] = (float) v;
+        if (a.sig64) a.sig64[sig_base + idx] = v;
     }
 }
 
@@ -102,7 +105,7 @@ __global__ void __launch_bounds__(kThreads) covgen_rect_kernel(CovRectArgs a) {
 }
 
 void launch_covgen_tiles(const TileMap &map, const int2 *pairs, int npairs, const MaternParams &prm,
-                         const double *x, const double *y, const double *z, int64_t n, hipStream_t s) {
+                         const double *x, const double *y, const double *z, int64_t n, double *sig64, hipStream_t s) {
     if (npairs <= 0) return;
     CovTilesArgs a;
     a.map = map;
@@ -111,6 +114,7 @@ void launch_covgen_tiles(const TileMap &map, const int2 *pairs, int npairs, cons
     a.loc = Locs{x, y, z};
     a.n = n;
     a.three_d = z != nullptr;
+    a.sig64 = sig64;
     const int nsb = map.ts / kBlk;
     dim3 grid(nsb * nsb, npairs);
     hipLaunchKernelGGL(covgen_tiles_kernel, grid, dim3(kThreads), 0, s, a);

@@ -12,8 +12,10 @@
 namespace egs {
 
 // ---- covariance generation (egs_covgen.hip)
+// sig64 (may be null): also writes every generated tile in FP64 to
+// sig64 + (i*(i+1)/2 + j) * ts * ts, column-major.
 void launch_covgen_tiles(const TileMap &map, const int2 *pairs, int npairs, const MaternParams &prm,
-                         const double *x, const double *y, const double *z, int64_t n, hipStream_t s);
+                         const double *x, const double *y, const double *z, int64_t n, double *sig64, hipStream_t s);
 void launch_covgen_rect(double *out, int rows, int cols, int64_t row_off, int64_t col_off, const MaternParams &prm,
                         const double *x1, const double *y1, const double *z1, const double *x2, const double *y2,
                         const double *z2, hipStream_t s);
@@ -57,6 +59,30 @@ void launch_cov_gemv(double *out, int64_t rows, int64_t cols, const MaternParams
 void launch_cov_matvec(double *partial, const double *v, int64_t n, int64_t npad, const MaternParams &prm,
                        const double *x, const double *y, const double *z, hipStream_t s);
 int matvec_chunks(int64_t n);
+// Cholesky of diagonal tile k in place, right-looking in 64 x 64 blocks, with
+// the inverses of its diagonal blocks in winv_k; scratch holds ts * ts doubles.
+void launch_factor_diag(const TileMap &map, int k, double *winv_k, double *scratch, int *info, int64_t n,
+                        hipStream_t s);
+// X_k = L_kk^{-T} for every diagonal tile k (tinv: nt * ts * ts doubles).
+void launch_tinv(const TileMap &map, const double *winv, double *tinv, hipStream_t s);
+// Preconditioner solves with the diagonal-tile inverses (right-looking):
+// forward step k: y_k = X_k' z_k, then z_i -= L_ik y_k for i > k;
+// backward step k: y_k = X_k x_k, then x_j -= L_kj' y_k for j < k.
+void launch_fwd_diag_inv(const double *tinv_k, const double *z_k, double *y_k, int ts, hipStream_t s);
+void launch_fwd_panel(const TileMap &map, int k, const double *y_k, double *z, hipStream_t s);
+void launch_bwd_diag_inv(const double *tinv_k, const double *x_k, double *y_k, int ts, hipStream_t s);
+void launch_bwd_row(const TileMap &map, int k, const double *y_k, double *x, hipStream_t s);
+// logdet partials: partials[2k] = sum log L_ii over diagonal tile k (L_ii > 0).
+void launch_diag_logdet(const TileMap &map, int64_t n, double *partials, hipStream_t s);
+// Sigma v from the FP64 lower tiles written by launch_covgen_tiles: one
+// workgroup per tile (i, j) writes T_ij v_j to prow[t * ts ..] and, off the
+// diagonal, T_ij' v_i to pcol[t * ts ..] (t = i*(i+1)/2 + j).
+void launch_sigma_tile_matvec(const double *sig64, const int2 *pairs, int npairs, int ts, const double *v,
+                              double *prow, double *pcol, hipStream_t s);
+// out[g] = base[g] + scale * (Sigma v)[g] for g < n (0 for padding rows), summing
+// the tile partials of row block g / ts in a fixed order.
+void launch_sigma_reduce(double *out, const double *base, double scale, const double *prow, const double *pcol,
+                         int nt, int ts, int64_t n, int64_t npad, hipStream_t s);
 // out = base + scale * (sum of the chunk partials), padding rows set to zero.
 void launch_matvec_reduce(double *out, const double *base, double scale, const double *partial, int nchunks,
                           int64_t n, int64_t npad, hipStream_t s);

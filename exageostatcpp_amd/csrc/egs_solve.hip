@@ -490,4 +490,282 @@ void launch_copy_pad(double *dst, const double *src, int64_t n, int64_t npad, hi
                        dst, src, n, npad);
 }
 
+// ------------------------------------------------------------------------
+// Sigma v from the FP64 lower tiles (refinement of mixed plans).  One
+// workgroup per tile (i, j); the tile is read once: wave w walks the columns
+// c = w, w + 4, ..., lane l holds rows l + 64 m of a 512-row chunk.  The row
+// products T v_j go to prow, the column products T' v_i (off the diagonal) to
+// pcol; both are summed in a fixed order, so results are reproducible.
+// ------------------------------------------------------------------------
+constexpr int kRowsPerLane = 8;
+constexpr int kChunkRows = 64 * kRowsPerLane;
+
+__global__ void __launch_bounds__(kThreads) sigma_tile_matvec_kernel(const double *__restrict__ sig64,
+                                                                     const int2 *__restrict__ pairs, int ts,
+                                                                     const double *__restrict__ v,
+                                                                     double *__restrict__ prow,
+                                                                     double *__restrict__ pcol) {
+    __shared__ double s_acc[kChunkRows];
+    __shared__ double s_col[kMaxTs];
+    const int2 ij = pairs[blockIdx.x];
+    const int i = ij.x, j = ij.y;
+    const int64_t tid = (int64_t) i * (i + 1) / 2 + j;
+    const double *T = sig64 + tid * ts * ts;
+    const double *vi = v + (int64_t) i * ts;
+    const double *vj = v + (int64_t) j * ts;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const bool offdiag = i != j;
+    for (int c = threadIdx.x; c < ts; c += kThreads) s_col[c] = 0.0;
+    __syncthreads();
+    for (int r0 = 0; r0 < ts; r0 += kChunkRows) {
+        double acc[kRowsPerLane], vr[kRowsPerLane];
+#pragma unroll
+        for (int m = 0; m < kRowsPerLane; ++m) {
+            const int r = r0 + lane + 64 * m;
+            vr[m] = r < ts ? vi[r] : 0.0;
+            acc[m] = 0.0;
+        }
+        for (int c = w; c < ts; c += kThreads / 64) {
+            const double vc = vj[c];
+            const double *col = T + (int64_t) c * ts + r0 + lane;
+            double dot = 0.0;
+#pragma unroll
+            for (int m = 0; m < kRowsPerLane; ++m) {
+                if (r0 + lane + 64 * m < ts) {
+                    const double t = __builtin_nontemporal_load(col + 64 * m);
+                    acc[m] += t * vc;
+                    dot += t * vr[m];
+                }
+            }
+            if (offdiag) {
+#pragma unroll
+                for (int off = 32; off > 0; off >>= 1) dot += __shfl_xor(dot, off, 64);
+                if (lane == 0) s_col[c] += dot;
+            }
+        }
+        // the four waves' row partials, added in wave order
+        for (int q = 0; q < kThreads / 64; ++q) {
+            if (w == q) {
+#pragma unroll
+                for (int m = 0; m < kRowsPerLane; ++m) {
+                    const int e = lane + 64 * m;
+                    s_acc[e] = (q == 0 ? 0.0 : s_acc[e]) + acc[m];
+                }
+            }
+            __syncthreads();
+        }
+        for (int e = threadIdx.x; e < kChunkRows && r0 + e < ts; e += kThreads) prow[tid * ts + r0 + e] = s_acc[e];
+        __syncthreads();
+    }
+    if (offdiag)
+        for (int c = threadIdx.x; c < ts; c += kThreads) pcol[tid * ts + c] = s_col[c];
+}
+
+__global__ void __launch_bounds__(kThreads) sigma_reduce_kernel(double *out, const double *base, double scale,
+                                                                const double *__restrict__ prow,
+                                                                const double *__restrict__ pcol, int nt, int ts,
+                                                                int64_t n, int64_t npad) {
+    const int64_t g = (int64_t) blockIdx.x * kThreads + threadIdx.x;
+    if (g >= npad) return;
+    const int i = (int) (g / ts), r = (int) (g % ts);
+    double acc = 0.0;
+    for (int j = 0; j <= i; ++j) acc += prow[((int64_t) i * (i + 1) / 2 + j) * ts + r];
+    for (int j = i + 1; j < nt; ++j) acc += pcol[((int64_t) j * (j + 1) / 2 + i) * ts + r];
+    out[g] = g < n ? (base ? base[g] : 0.0) + scale * acc : 0.0;
+}
+
+void launch_sigma_tile_matvec(const double *sig64, const int2 *pairs, int npairs, int ts, const double *v,
+                              double *prow, double *pcol, hipStream_t s) {
+    if (npairs <= 0) return;
+    hipLaunchKernelGGL(sigma_tile_matvec_kernel, dim3((unsigned) npairs), dim3(kThreads), 0, s, sig64, pairs, ts, v,
+                       prow, pcol);
+}
+
+void launch_sigma_reduce(double *out, const double *base, double scale, const double *prow, const double *pcol,
+                         int nt, int ts, int64_t n, int64_t npad, hipStream_t s) {
+    if (npad <= 0) return;
+    hipLaunchKernelGGL(sigma_reduce_kernel, dim3((unsigned) ((npad + kThreads - 1) / kThreads)), dim3(kThreads), 0, s,
+                       out, base, scale, prow, pcol, nt, ts, n, npad);
+}
+
+// ------------------------------------------------------------------------
+// Preconditioner solves of mixed plans with X_k = L_kk^{-T} (launch_tinv).
+// Both directions are right-looking: a GEMV with the diagonal-tile inverse,
+// then a GEMV of the tile column (forward) or tile row (backward) that
+// updates the not-yet-solved blocks in place.  Every output element is
+// written by one wave or one workgroup, sums run in a fixed order.
+// ------------------------------------------------------------------------
+
+// y[c] = sum_{r <= c} X[r][c] z[r]: one wave per column c.
+__global__ void __launch_bounds__(kThreads) fwd_diag_inv_kernel(const double *__restrict__ X,
+                                                                const double *__restrict__ z,
+                                                                double *__restrict__ y, int ts) {
+    const int lane = threadIdx.x & 63;
+    const int c = blockIdx.x * (kThreads / 64) + (threadIdx.x >> 6);
+    if (c >= ts) return;
+    const double *col = X + (int64_t) c * ts;
+    double s0 = 0.0, s1 = 0.0;
+    int r = lane;
+    for (; r + 64 <= c; r += 128) {
+        s0 += col[r] * z[r];
+        s1 += col[r + 64] * z[r + 64];
+    }
+    for (; r <= c; r += 64) s0 += col[r] * z[r];
+    double s = s0 + s1;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+    if (lane == 0) y[c] = s;
+}
+
+// z_i -= L_ik y_k for i > k; one workgroup per 64 rows, four independent
+// partial sums per lane over the columns of its wave.
+template <typename TA>
+__device__ __forceinline__ double panel_row_dot(const TA *__restrict__ A, const double *sy, int ts, int w) {
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+    int c = w;
+    for (; c + 12 < ts; c += 16) {
+        a0 += (double) A[(int64_t) c * ts] * sy[c];
+        a1 += (double) A[(int64_t) (c + 4) * ts] * sy[c + 4];
+        a2 += (double) A[(int64_t) (c + 8) * ts] * sy[c + 8];
+        a3 += (double) A[(int64_t) (c + 12) * ts] * sy[c + 12];
+    }
+    for (; c < ts; c += 4) a0 += (double) A[(int64_t) c * ts] * sy[c];
+    return (a0 + a1) + (a2 + a3);
+}
+
+__global__ void __launch_bounds__(kThreads) fwd_panel_kernel(TileMap map, int k, const double *__restrict__ yk,
+                                                             double *__restrict__ z) {
+    __shared__ double sy[kMaxTs];
+    __shared__ double sp[4][kBlk];
+    const int ts = map.ts;
+    const int nsb = ts / kBlk;
+    const int t = threadIdx.x;
+    const int lane = t & 63, w = t >> 6;
+    const int i = k + 1 + (int) (blockIdx.x / nsb);
+    const int rb = (int) (blockIdx.x % nsb);
+    for (int e = t; e < ts; e += kThreads) sy[e] = yk[e];
+    __syncthreads();
+    double s;
+    if (map.store[i * map.nt + k] == kStoreF32) s = panel_row_dot(tile_f32(map, i, k) + rb * kBlk + lane, sy, ts, w);
+    else s = panel_row_dot(tile_f64(map, i, k) + rb * kBlk + lane, sy, ts, w);
+    sp[w][lane] = s;
+    __syncthreads();
+    if (t < kBlk) z[(int64_t) i * ts + rb * kBlk + t] -= ((sp[0][t] + sp[1][t]) + sp[2][t]) + sp[3][t];
+}
+
+// y[r] = sum_{c >= r} X[r][c] x[c]: one workgroup of 16 waves per 64 rows,
+// wave g takes the columns c = rb*64 + g + 16 m.
+__global__ void __launch_bounds__(kSolveThreads) bwd_diag_inv_kernel(const double *__restrict__ X,
+                                                                     const double *__restrict__ x,
+                                                                     double *__restrict__ y, int ts) {
+    __shared__ double sp[kGroups][kBlk];
+    const int t = threadIdx.x;
+    const int lane = t & 63, g = t >> 6;
+    const int rb = blockIdx.x;
+    const int r = rb * kBlk + lane;
+    const double *Xr = X + r;
+    double s0 = 0.0, s1 = 0.0;
+    int c = rb * kBlk + g;
+    for (; c + kGroups < ts; c += 2 * kGroups) {
+        s0 += (c >= r ? Xr[(int64_t) c * ts] : 0.0) * x[c];
+        s1 += Xr[(int64_t) (c + kGroups) * ts] * x[c + kGroups] * (c + kGroups >= r ? 1.0 : 0.0);
+    }
+    for (; c < ts; c += kGroups) s0 += (c >= r ? Xr[(int64_t) c * ts] : 0.0) * x[c];
+    sp[g][lane] = s0 + s1;
+    __syncthreads();
+    if (t < kBlk) {
+        double a = 0.0;
+        for (int q = 0; q < kGroups; ++q) a += sp[q][t];
+        y[rb * kBlk + t] = a;
+    }
+}
+
+// x_j[c] -= sum_r L_kj[r][c] y_k[r] for j < k: workgroup (j, 64-column block),
+// wave w reduces 16 columns, lanes stride the rows.
+__global__ void __launch_bounds__(kThreads) bwd_row_kernel(TileMap map, int k, const double *__restrict__ yk,
+                                                           double *__restrict__ x) {
+    __shared__ double sy[kMaxTs];
+    const int ts = map.ts;
+    const int nsb = ts / kBlk;
+    const int j = (int) (blockIdx.x / nsb);
+    const int cb = (int) (blockIdx.x % nsb);
+    const int t = threadIdx.x;
+    const int lane = t & 63, w = t >> 6;
+    for (int e = t; e < ts; e += kThreads) sy[e] = yk[e];
+    __syncthreads();
+    const bool f32 = map.store[k * map.nt + j] == kStoreF32;
+    const double *A = tile_f64(map, k, j);
+    const float *Af = tile_f32(map, k, j);
+    for (int cc = 0; cc < 16; ++cc) {
+        const int c = cb * kBlk + w * 16 + cc;
+        double s0 = 0.0, s1 = 0.0;
+        int r0 = lane;
+        if (f32) {
+            const float *col = Af + (int64_t) c * ts;
+            for (; r0 + 64 < ts; r0 += 128) {
+                s0 += (double) col[r0] * sy[r0];
+                s1 += (double) col[r0 + 64] * sy[r0 + 64];
+            }
+            for (; r0 < ts; r0 += 64) s0 += (double) col[r0] * sy[r0];
+        } else {
+            const double *col = A + (int64_t) c * ts;
+            for (; r0 + 64 < ts; r0 += 128) {
+                s0 += col[r0] * sy[r0];
+                s1 += col[r0 + 64] * sy[r0 + 64];
+            }
+            for (; r0 < ts; r0 += 64) s0 += col[r0] * sy[r0];
+        }
+        double s = s0 + s1;
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+        if (lane == 0) x[(int64_t) j * ts + c] -= s;
+    }
+}
+
+// partials[2k] = sum of log L_ii (L_ii > 0) over diagonal tile k, partials[2k+1] = 0.
+__global__ void __launch_bounds__(kThreads) diag_logdet_kernel(TileMap map, double *partials) {
+    __shared__ double red[kThreads];
+    const int k = blockIdx.x;
+    const int ts = map.ts;
+    const double *L = tile_f64(map, k, k);
+    double ld = 0.0;
+    for (int e = threadIdx.x; e < ts; e += kThreads) {
+        const double d = L[e + (int64_t) e * ts];
+        if (d > 0.0) ld += log(d);
+    }
+    ld = wg_sum(ld, red);
+    if (threadIdx.x == 0) {
+        partials[2 * k] = ld;
+        partials[2 * k + 1] = 0.0;
+    }
+}
+
+void launch_fwd_diag_inv(const double *tinv_k, const double *z_k, double *y_k, int ts, hipStream_t s) {
+    const int per = kThreads / 64;
+    hipLaunchKernelGGL(fwd_diag_inv_kernel, dim3((unsigned) ((ts + per - 1) / per)), dim3(kThreads), 0, s, tinv_k, z_k,
+                       y_k, ts);
+}
+
+void launch_fwd_panel(const TileMap &map, int k, const double *y_k, double *z, hipStream_t s) {
+    const int rows = map.nt - k - 1;
+    if (rows <= 0) return;
+    const int nsb = map.ts / kBlk;
+    hipLaunchKernelGGL(fwd_panel_kernel, dim3((unsigned) (rows * nsb)), dim3(kThreads), 0, s, map, k, y_k, z);
+}
+
+void launch_bwd_diag_inv(const double *tinv_k, const double *x_k, double *y_k, int ts, hipStream_t s) {
+    hipLaunchKernelGGL(bwd_diag_inv_kernel, dim3((unsigned) (ts / kBlk)), dim3(kSolveThreads), 0, s, tinv_k, x_k, y_k,
+                       ts);
+}
+
+void launch_bwd_row(const TileMap &map, int k, const double *y_k, double *x, hipStream_t s) {
+    if (k <= 0) return;
+    const int nsb = map.ts / kBlk;
+    hipLaunchKernelGGL(bwd_row_kernel, dim3((unsigned) (k * nsb)), dim3(kThreads), 0, s, map, k, y_k, x);
+}
+
+void launch_diag_logdet(const TileMap &map, int64_t, double *partials, hipStream_t s) {
+    hipLaunchKernelGGL(diag_logdet_kernel, dim3((unsigned) map.nt), dim3(kThreads), 0, s, map, partials);
+}
+
 }  // namespace egs

@@ -195,6 +195,36 @@ int egs_bobyqa(int n, double *x, const double *lb, const double *ub, double ftol
 int egs_data_modeling(egs_problem_t problem, const double *lb, const double *ub, const double *start, double tolerance,
                       int max_iterations, double *theta, double *loglik, int *nevals);
 
+/* ---- serving ------------------------------------------------------------------
+ * A request loop over many spatial regions on one GPU.  The reference has no
+ * serving layer: it evaluates one problem per process through
+ * ExaGeoStatDataModeling / ExaGeoStatPrediction (ExaGeoStat.cpp:91-125); the
+ * server runs those two operations (egs_loglik, egs_predict) for registered
+ * regions concurrently.  `workers` contexts (1..16) each own a pair of streams;
+ * a region is pinned to one worker, so its requests run in submission order
+ * and a prediction at the theta of the region's last evaluation reuses the
+ * factor kept in HBM.  Regions are made resident on demand within
+ * hbm_budget_bytes, releasing the least recently used idle ones. */
+typedef struct egs_server_s *egs_server_t;
+int egs_server_create(int device, int64_t hbm_budget_bytes, int workers, egs_server_t *server);
+int egs_server_destroy(egs_server_t server);
+/* Copies the locations and observations (n_loc * P values); *region is its id. */
+int egs_server_add_region(egs_server_t server, int kernel, int64_t n_loc, const double *x, const double *y,
+                          const double *z, const double *obs, int metric, int dts, int *region);
+int egs_server_submit_loglik(egs_server_t server, int region, const double *theta, int64_t *ticket);
+int egs_server_submit_predict(egs_server_t server, int region, const double *theta, int64_t n_miss, const double *xm,
+                              const double *ym, int64_t *ticket);
+/* Blocks until the request is done; *value is the log-likelihood (or -1 for a
+ * prediction), zmiss (n_miss * P values) the predictions, *latency_ms the time
+ * from submission to completion.  A failed request returns its error. */
+int egs_server_wait(egs_server_t server, int64_t ticket, double *value, double *zmiss, int64_t zmiss_len,
+                    double *latency_ms);
+/* Completed requests, latency percentiles (ms), completions per second since
+ * the first submission, and the HBM held by resident regions. */
+int egs_server_stats(egs_server_t server, int64_t *completed, double *p50_ms, double *p99_ms, double *per_second,
+                     int64_t *resident_bytes);
+int egs_server_reset_stats(egs_server_t server);
+
 #ifdef __cplusplus
 }
 #endif

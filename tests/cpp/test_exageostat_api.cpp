@@ -113,6 +113,46 @@ static void cpu_cases() {
         double bad_lb[2] = {1, 1}, bad_ub[2] = {0, 2};
         REQUIRE(egs_bobyqa(2, x, bad_lb, bad_ub, 0.0, 10, 0, rosen, nullptr, &fmin, &nev, &reason) == EGS_ERR_INVALID_ARG);
     });
+    run("CSV write then read round trip (CSVLoader.cpp:36-234)", [] {
+        using exageostat::dataLoader::csv::CSVLoader;
+        char tmpl[] = "/tmp/egs_csv_XXXXXX";
+        REQUIRE(mkdtemp(tmpl) != nullptr);
+        std::string dir = tmpl;
+        Locations<double> l(16, Dimension2D);
+        l.SetLocationX(*kX16, 16);
+        l.SetLocationY(*kY16, 16);
+        std::vector<double> z2(32);
+        for (int i = 0; i < 32; ++i) z2[i] = kZ16[i % 16] * (i % 2 ? -1.0 : 1.0);
+        std::string path = dir;
+        CSVLoader<double>::GetInstance()->WriteData(*z2.data(), 32, 2, path, l);   // bivariate: P = 2
+        REQUIRE(path == dir + "/synthetic_ds");
+        Configurations c;
+        c.SetDataPath(path + "/SYN_16_1");
+        std::vector<double> z, x, y, zz;
+        CSVLoader<double>::GetInstance()->ReadData(c, z, x, y, zz, 2);
+        REQUIRE(c.GetProblemSize() == 32 && x.size() == 16 && zz.empty());
+        for (int i = 0; i < 16; ++i) REQUIRE(std::fabs(x[i] - kX16[i]) < 1e-14 && std::fabs(y[i] - kY16[i]) < 1e-14);
+        for (int i = 0; i < 32; ++i) REQUIRE(std::fabs(z[i] - z2[i]) < 1e-14);
+        // two values per line cannot feed P = 3
+        std::vector<double> a, b, d, e;
+        Configurations c1;
+        c1.SetDataPath(path + "/SYN_16_1");
+        bool threw = false;
+        try {
+            CSVLoader<double>::GetInstance()->ReadData(c1, a, b, d, e, 3);
+        } catch (const std::runtime_error &err) {
+            threw = std::string(err.what()).find("isn't consistent") != std::string::npos;
+        }
+        REQUIRE(threw);
+        Configurations c2;
+        threw = false;
+        try {
+            CSVLoader<double>::GetInstance()->ReadData(c2, a, b, d, e, 1);
+        } catch (const std::runtime_error &err) {
+            threw = std::string(err.what()) == "Please enter the path to data file.";
+        }
+        REQUIRE(threw);
+    });
     run("prediction without an estimated theta throws", [] {
         Configurations c;
         c.SetIsMSPE(true);
