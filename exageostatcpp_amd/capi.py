@@ -203,7 +203,7 @@ class Problem:
         _check(lib().egs_problem_device_bytes(self.handle, ctypes.byref(b)))
         return b.value
 
-    def set_precision(self, mode, band_fp64=2, band_fp32=10):
+    def set_precision(self, mode, band_fp64=1, band_fp32=1):
         _check(lib().egs_problem_set_precision(self.handle, mode, band_fp64, band_fp32))
 
     def set_observations(self, z):

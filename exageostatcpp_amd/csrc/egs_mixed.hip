@@ -15,6 +15,8 @@
 // consecutive K values of one row, one 16-byte load).  The split costs no
 // arithmetic in the GEMM.  Diagonal tiles, their factorization, the
 // log-determinant and every residual stay FP64.
+#include <cstdlib>
+
 #include "egs_common.hpp"
 #include "egs_kernels.hpp"
 
@@ -325,6 +327,156 @@ __global__ void __launch_bounds__(kNT, 1) update192_bf16_kernel(TileMap map, con
     }
 }
 
+// ------------------------------------------------------------------------
+// Split-BF16 update with v_mfma_f32_32x32x16_bf16: four waves per 192 x 192
+// block, each owning a 96 x 96 quadrant as 3 x 3 tiles of 32 x 32 (144 FP32
+// accumulators per lane).  Per 16-deep K step a wave reads 12 operand
+// fragments for 27 MFMAs, half the LDS bytes per flop of the 16 x 16 tiling.
+// Output slot i of lane l holds row (l % 32) of the A operand and column
+// 8 (i / 4) + 4 (l / 32) + i % 4 of the B operand (measured on gfx950).
+// ------------------------------------------------------------------------
+constexpr int kNTw = 256;
+constexpr int kKCw = 32;                  // K per chunk: two MFMA steps
+constexpr int kPlanesW = kKCw / 8;        // 16-byte vectors per row per chunk
+constexpr int kPlaneW = kBM + 4;          // rows per LDS plane, padded
+constexpr int kVecsW = kBM * kPlanesW;    // 768 vectors per operand part per chunk
+constexpr int kPerThreadW = kVecsW / kNTw;   // 3
+
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+__device__ __forceinline__ f32x16 mfma_bf16_32(const bf16x8 &x, const bf16x8 &y, f32x16 acc) {
+    return __builtin_amdgcn_mfma_f32_32x32x16_bf16(x, y, acc, 0, 0, 0);
+}
+
+template <bool kSplit>
+__global__ void __launch_bounds__(kNTw, 1) update192_bf16w_kernel(TileMap map, const __bf16 *p16h, const __bf16 *p16l,
+                                                                  const int2 *pairs, int k) {
+    constexpr int kParts = kSplit ? 2 : 1;
+    __shared__ bf16x8 sA[2][kParts][kPlanesW][kPlaneW];
+    __shared__ bf16x8 sB[2][kParts][kPlanesW][kPlaneW];
+    const int ts = map.ts;
+    const int nsb = ts / kBM;
+    const int nsub = nsb * nsb;
+    const int bid = xcd_remap((int) blockIdx.x, (int) gridDim.x);
+    const int pair = bid / nsub;
+    const int sub = bid % nsub;
+    const int sbm = sub % nsb, sbn = sub / nsb;
+    const int2 ij = pairs[pair];
+    const int64_t tsz = (int64_t) ts * ts;
+    const int64_t aoff = (int64_t) (ij.x - k - 1) * tsz + (int64_t) sbm * kBM * ts;
+    const int64_t boff = (int64_t) (ij.y - k - 1) * tsz + (int64_t) sbn * kBM * ts;
+    const __bf16 *src[4] = {p16h + aoff, p16l + aoff, p16h + boff, p16l + boff};
+    const int t = threadIdx.x;
+    const int lane = t & 63, w = t >> 6;
+    const int wm = w & 1, wn = w >> 1;
+    const int lr = lane & 31, lk = lane >> 5;
+
+    // staging: vector e = t + 256 q of a chunk is row e / 4, plane e % 4
+    uint32_t goff[kPerThreadW];
+#pragma unroll
+    for (int q = 0; q < kPerThreadW; ++q) {
+        const int e = t + kNTw * q;
+        goff[q] = (uint32_t) ((e >> 2) * ts + (e & 3) * 8);
+    }
+
+    f32x16 acc[3][3];
+#pragma unroll
+    for (int mi = 0; mi < 3; ++mi)
+#pragma unroll
+        for (int ni = 0; ni < 3; ++ni)
+#pragma unroll
+            for (int v = 0; v < 16; ++v) acc[mi][ni][v] = 0.f;
+
+    constexpr int kDepth = 2;
+    bf16x8 rg[kDepth][kPerThreadW][2 * kParts];
+    auto issue = [&](int set, int chunk) {
+        const uint32_t kofs = (uint32_t) (chunk * kKCw);
+#pragma unroll
+        for (int q = 0; q < kPerThreadW; ++q)
+#pragma unroll
+            for (int p = 0; p < kParts; ++p) {
+                rg[set][q][p] = *reinterpret_cast<const bf16x8 *>(src[p] + goff[q] + kofs);
+                rg[set][q][kParts + p] = *reinterpret_cast<const bf16x8 *>(src[2 + p] + goff[q] + kofs);
+            }
+    };
+    auto stage = [&](int set, int buf) {
+#pragma unroll
+        for (int q = 0; q < kPerThreadW; ++q) {
+            const int e = t + kNTw * q;
+#pragma unroll
+            for (int p = 0; p < kParts; ++p) {
+                sA[buf][p][e & 3][e >> 2] = rg[set][q][p];
+                sB[buf][p][e & 3][e >> 2] = rg[set][q][kParts + p];
+            }
+        }
+    };
+    const int nchunks = ts / kKCw;
+#pragma unroll
+    for (int u = 0; u < kDepth; ++u)
+        if (u < nchunks) issue(u, u);
+    stage(0, 0);
+    __syncthreads();
+    for (int c0 = 0; c0 < nchunks; c0 += kDepth) {
+#pragma unroll
+        for (int u = 0; u < kDepth; ++u) {
+            const int c = c0 + u;
+            const int buf = c & 1;
+            if (c + kDepth < nchunks) issue(u, c + kDepth);
+#pragma unroll
+            for (int ks = 0; ks < kKCw / 16; ++ks) {
+                const int pl = 2 * ks + lk;
+                bf16x8 ah[3], bh[3];
+#pragma unroll
+                for (int mi = 0; mi < 3; ++mi) ah[mi] = sA[buf][0][pl][wm * 96 + mi * 32 + lr];
+#pragma unroll
+                for (int ni = 0; ni < 3; ++ni) bh[ni] = sB[buf][0][pl][wn * 96 + ni * 32 + lr];
+                if (kSplit) {
+                    bf16x8 al[3], bl[3];
+#pragma unroll
+                    for (int mi = 0; mi < 3; ++mi) al[mi] = sA[buf][kParts - 1][pl][wm * 96 + mi * 32 + lr];
+#pragma unroll
+                    for (int ni = 0; ni < 3; ++ni) bl[ni] = sB[buf][kParts - 1][pl][wn * 96 + ni * 32 + lr];
+#pragma unroll
+                    for (int mi = 0; mi < 3; ++mi)
+#pragma unroll
+                        for (int ni = 0; ni < 3; ++ni) {
+                            acc[mi][ni] = mfma_bf16_32(bh[ni], ah[mi], acc[mi][ni]);
+                            acc[mi][ni] = mfma_bf16_32(bh[ni], al[mi], acc[mi][ni]);
+                            acc[mi][ni] = mfma_bf16_32(bl[ni], ah[mi], acc[mi][ni]);
+                        }
+                } else {
+#pragma unroll
+                    for (int mi = 0; mi < 3; ++mi)
+#pragma unroll
+                        for (int ni = 0; ni < 3; ++ni) acc[mi][ni] = mfma_bf16_32(bh[ni], ah[mi], acc[mi][ni]);
+                }
+            }
+            if (c + 1 < nchunks) stage((u + 1) % kDepth, buf ^ 1);
+            __syncthreads();
+        }
+    }
+    // C -= acc: lane row (A side) wm*96 + mi*32 + lr, column (B side)
+    // wn*96 + ni*32 + 8 (v / 4) + 4 lk + v % 4; one 3 x 16 batch per mi
+    float *C = tile_f32(map, ij.x, ij.y);
+    const uint32_t cbase = (uint32_t) ((sbm * kBM + wm * 96 + lr) + (sbn * kBM + wn * 96 + 4 * lk) * ts);
+#pragma unroll
+    for (int mi = 0; mi < 3; ++mi) {
+        float cv[3][16];
+#pragma unroll
+        for (int ni = 0; ni < 3; ++ni)
+#pragma unroll
+            for (int v = 0; v < 16; ++v)
+                cv[ni][v] = __builtin_nontemporal_load(
+                    &C[cbase + (uint32_t) (mi * 32 + (ni * 32 + 8 * (v >> 2) + (v & 3)) * ts)]);
+#pragma unroll
+        for (int ni = 0; ni < 3; ++ni)
+#pragma unroll
+            for (int v = 0; v < 16; ++v)
+                __builtin_nontemporal_store(cv[ni][v] - acc[mi][ni][v],
+                                            &C[cbase + (uint32_t) (mi * 32 + (ni * 32 + 8 * (v >> 2) + (v & 3)) * ts)]);
+    }
+}
+
 }  // namespace
 
 void launch_panel_convert(const TileMap &map, int k, double *p64, float *p32, __bf16 *p16h, __bf16 *p16l,
@@ -348,6 +500,15 @@ void launch_update192_bf16(const TileMap &map, const __bf16 *p16h, const __bf16 
     if (npairs <= 0) return;
     const int nsb = map.ts / kBM;
     const dim3 grid((unsigned) npairs * (unsigned) (nsb * nsb));
+    static const bool wide = [] {
+        const char *v = std::getenv("EGS_BF16_WIDE");
+        return v && v[0] == '1';
+    }();
+    if (wide) {
+        if (split) hipLaunchKernelGGL(update192_bf16w_kernel<true>, grid, dim3(kNTw), 0, s, map, p16h, p16l, pairs, k);
+        else hipLaunchKernelGGL(update192_bf16w_kernel<false>, grid, dim3(kNTw), 0, s, map, p16h, p16l, pairs, k);
+        return;
+    }
     if (split) hipLaunchKernelGGL(update192_bf16_kernel<true>, grid, dim3(kNT), 0, s, map, p16h, p16l, pairs, k);
     else hipLaunchKernelGGL(update192_bf16_kernel<false>, grid, dim3(kNT), 0, s, map, p16h, p16l, pairs, k);
 }

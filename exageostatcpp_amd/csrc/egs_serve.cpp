@@ -145,9 +145,7 @@ struct egs_server_s {
                         if (st == EGS_SUCCESS)
                             st = egs_problem_set_observations(reg->prob, reg->obs.data(), (int64_t) reg->obs.size());
                         if (st == EGS_SUCCESS) {
-                            int64_t b = need;
-                            egs_problem_device_bytes(reg->prob, &b);
-                            reg->footprint = std::max(b, need);
+                            reg->footprint = need;   // the estimate bounds the problem's device bytes
                             resident += reg->footprint;
                         } else {
                             res.error = egs_last_error();
