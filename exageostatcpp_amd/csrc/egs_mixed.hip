@@ -3,7 +3,7 @@
 // Plan: tile (i, j) with d = i - j is
 //   d <= b64        : stored FP64, updated with FP64 MFMA (egs_update.hip);
 //   b64 < d <= b32  : stored FP32, updated with v_mfma_f32_16x16x4_f32;
-//   d > b32         : stored FP32, updated with v_mfma_f32_16x16x32_bf16 on
+//   d > b32         : stored FP32, updated with v_mfma_f32_32x32x16_bf16 on
 //                     BF16 operands with FP32 accumulation.  By default each
 //                     FP32 operand is split into hi = bf16(x) and lo = bf16(x - hi)
 //                     and the product is hi*hi + hi*lo + lo*hi (three BF16
@@ -15,8 +15,6 @@
 // consecutive K values of one row, one 16-byte load).  The split costs no
 // arithmetic in the GEMM.  Diagonal tiles, their factorization, the
 // log-determinant and every residual stay FP64.
-#include <cstdlib>
-
 #include "egs_common.hpp"
 #include "egs_kernels.hpp"
 
@@ -30,9 +28,6 @@ constexpr int kNT = 512;
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4v __attribute__((ext_vector_type(4)));
 
-__device__ __forceinline__ f32x4 mfma_bf16(const bf16x8 &x, const bf16x8 &y, f32x4 acc) {
-    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(x, y, acc, 0, 0, 0);
-}
 
 // ------------------------------------------------------------------------
 // Panel conversion: tile (i, k) of panel k -> P64, P32 (column-major) and
@@ -155,148 +150,6 @@ __global__ void __launch_bounds__(kNT, 1) update192_f32_kernel(TileMap map, cons
                 for (int mi = 0; mi < 6; ++mi)
 #pragma unroll
                     for (int ni = 0; ni < 3; ++ni) acc[mi][ni] = mfma_f32(b[ni], a[mi], acc[mi][ni]);
-            }
-            if (c + 1 < nchunks) stage((u + 1) % kDepth, buf ^ 1);
-            __syncthreads();
-        }
-    }
-    // C -= acc in two batches of 36 values (one memory round trip each)
-    float *C = tile_f32(map, ij.x, ij.y);
-    const uint32_t cbase = (uint32_t) ((sbm * kBM + wm * 96 + lr) + (sbn * kBM + wn * 48 + 4 * lk) * ts);
-#pragma unroll
-    for (int m0 = 0; m0 < 6; m0 += 3) {
-        float cv[3][3][4];
-#pragma unroll
-        for (int mi = 0; mi < 3; ++mi)
-#pragma unroll
-            for (int ni = 0; ni < 3; ++ni)
-#pragma unroll
-                for (int v = 0; v < 4; ++v)
-                    cv[mi][ni][v] = __builtin_nontemporal_load(
-                        &C[cbase + (uint32_t) ((m0 + mi) * 16 + (ni * 16 + v) * ts)]);
-#pragma unroll
-        for (int mi = 0; mi < 3; ++mi)
-#pragma unroll
-            for (int ni = 0; ni < 3; ++ni)
-#pragma unroll
-                for (int v = 0; v < 4; ++v)
-                    __builtin_nontemporal_store(cv[mi][ni][v] - acc[m0 + mi][ni][v],
-                                                &C[cbase + (uint32_t) ((m0 + mi) * 16 + (ni * 16 + v) * ts)]);
-    }
-}
-
-// ------------------------------------------------------------------------
-// BF16-class update: C_ij (FP32) -= A B^T from the pre-split BF16 panels.
-// K chunk 32 = one v_mfma_f32_16x16x32_bf16 step.  LDS holds each chunk as 4
-// planes (K values 8p .. 8p+7) of 16-byte row vectors: the 16 lanes that read
-// one plane read 16 consecutive rows, 256 contiguous bytes, and the plane stride
-// of 196 rows puts the 4 planes a group of staging threads writes on different
-// banks.  Lane l supplies row l%16, K values 8*(l/16) .. +7 of both operands;
-// the FP32 MFMA result element v of lane l is D[4*(l/16) + v][l%16].
-// ------------------------------------------------------------------------
-constexpr int kKC16 = 32;
-constexpr int kVecRow = kKC16 / 8;        // 16-byte vectors per row per chunk (planes)
-constexpr int kPlane = kBM + 4;           // rows per LDS plane, padded
-constexpr int kVecs = kBM * kVecRow;      // 768 vectors per operand part per chunk
-
-template <bool kSplit>
-__global__ void __launch_bounds__(kNT, 1) update192_bf16_kernel(TileMap map, const __bf16 *p16h, const __bf16 *p16l,
-                                                                const int2 *pairs, int k) {
-    constexpr int kParts = kSplit ? 2 : 1;
-    __shared__ bf16x8 sA[2][kParts][kVecRow][kPlane];
-    __shared__ bf16x8 sB[2][kParts][kVecRow][kPlane];
-    const int ts = map.ts;
-    const int nsb = ts / kBM;
-    const int nsub = nsb * nsb;
-    const int bid = xcd_remap((int) blockIdx.x, (int) gridDim.x);
-    const int pair = bid / nsub;
-    const int sub = bid % nsub;
-    const int sbm = sub % nsb, sbn = sub / nsb;
-    const int2 ij = pairs[pair];
-    const int64_t tsz = (int64_t) ts * ts;
-    // row-major tiles: element (row, col) at row * ts + col
-    const int64_t aoff = (int64_t) (ij.x - k - 1) * tsz + (int64_t) sbm * kBM * ts;
-    const int64_t boff = (int64_t) (ij.y - k - 1) * tsz + (int64_t) sbn * kBM * ts;
-    const __bf16 *src[4] = {p16h + aoff, p16l + aoff, p16h + boff, p16l + boff};
-    const int t = threadIdx.x;
-    const int lane = t & 63, w = t >> 6;
-    const int wm = w & 1, wn = w >> 1;
-    const int lr = lane & 15, lk = lane >> 4;
-
-    // staging: per operand part, vector e (< 768) is row e / 4, vector e % 4;
-    // threads take e = t and, for t < 256, e = 512 + t.
-    const bool two = t < kVecs - kNT;
-    const int e0 = t, e1 = kNT + t;
-    const uint32_t g0 = (uint32_t) ((e0 >> 2) * ts + (e0 & 3) * 8);
-    const uint32_t g1 = (uint32_t) ((e1 >> 2) * ts + (e1 & 3) * 8);
-
-    f32x4 acc[6][3];
-#pragma unroll
-    for (int mi = 0; mi < 6; ++mi)
-#pragma unroll
-        for (int ni = 0; ni < 3; ++ni) acc[mi][ni] = f32x4{0.f, 0.f, 0.f, 0.f};
-
-    constexpr int kDepth = 3;   // chunks of loads in flight (register ring)
-    bf16x8 r0[kDepth][2 * kParts], r1[kDepth][2 * kParts];
-    auto issue = [&](int set, int chunk) {
-        const uint32_t kofs = (uint32_t) (chunk * kKC16);
-#pragma unroll
-        for (int p = 0; p < kParts; ++p) {
-            r0[set][p] = *reinterpret_cast<const bf16x8 *>(src[p] + g0 + kofs);
-            r0[set][kParts + p] = *reinterpret_cast<const bf16x8 *>(src[2 + p] + g0 + kofs);
-            if (two) {
-                r1[set][p] = *reinterpret_cast<const bf16x8 *>(src[p] + g1 + kofs);
-                r1[set][kParts + p] = *reinterpret_cast<const bf16x8 *>(src[2 + p] + g1 + kofs);
-            }
-        }
-    };
-    auto stage = [&](int set, int buf) {
-#pragma unroll
-        for (int p = 0; p < kParts; ++p) {
-            sA[buf][p][e0 & 3][e0 >> 2] = r0[set][p];
-            sB[buf][p][e0 & 3][e0 >> 2] = r0[set][kParts + p];
-            if (two) {
-                sA[buf][p][e1 & 3][e1 >> 2] = r1[set][p];
-                sB[buf][p][e1 & 3][e1 >> 2] = r1[set][kParts + p];
-            }
-        }
-    };
-    const int nchunks = ts / kKC16;
-#pragma unroll
-    for (int u = 0; u < kDepth; ++u)
-        if (u < nchunks) issue(u, u);
-    stage(0, 0);
-    __syncthreads();
-    for (int c0 = 0; c0 < nchunks; c0 += kDepth) {
-#pragma unroll
-        for (int u = 0; u < kDepth; ++u) {
-            const int c = c0 + u;
-            const int buf = c & 1;
-            if (c + kDepth < nchunks) issue(u, c + kDepth);
-            bf16x8 ah[6], bh[3];
-#pragma unroll
-            for (int mi = 0; mi < 6; ++mi) ah[mi] = sA[buf][0][lk][wm * 96 + mi * 16 + lr];
-#pragma unroll
-            for (int ni = 0; ni < 3; ++ni) bh[ni] = sB[buf][0][lk][wn * 48 + ni * 16 + lr];
-            if (kSplit) {
-                bf16x8 al[6], bl[3];
-#pragma unroll
-                for (int mi = 0; mi < 6; ++mi) al[mi] = sA[buf][kParts - 1][lk][wm * 96 + mi * 16 + lr];
-#pragma unroll
-                for (int ni = 0; ni < 3; ++ni) bl[ni] = sB[buf][kParts - 1][lk][wn * 48 + ni * 16 + lr];
-#pragma unroll
-                for (int mi = 0; mi < 6; ++mi)
-#pragma unroll
-                    for (int ni = 0; ni < 3; ++ni) {
-                        acc[mi][ni] = mfma_bf16(bh[ni], ah[mi], acc[mi][ni]);
-                        acc[mi][ni] = mfma_bf16(bh[ni], al[mi], acc[mi][ni]);
-                        acc[mi][ni] = mfma_bf16(bl[ni], ah[mi], acc[mi][ni]);
-                    }
-            } else {
-#pragma unroll
-                for (int mi = 0; mi < 6; ++mi)
-#pragma unroll
-                    for (int ni = 0; ni < 3; ++ni) acc[mi][ni] = mfma_bf16(bh[ni], ah[mi], acc[mi][ni]);
             }
             if (c + 1 < nchunks) stage((u + 1) % kDepth, buf ^ 1);
             __syncthreads();
@@ -500,17 +353,8 @@ void launch_update192_bf16(const TileMap &map, const __bf16 *p16h, const __bf16 
     if (npairs <= 0) return;
     const int nsb = map.ts / kBM;
     const dim3 grid((unsigned) npairs * (unsigned) (nsb * nsb));
-    static const bool wide = [] {
-        const char *v = std::getenv("EGS_BF16_WIDE");
-        return v && v[0] == '1';
-    }();
-    if (wide) {
-        if (split) hipLaunchKernelGGL(update192_bf16w_kernel<true>, grid, dim3(kNTw), 0, s, map, p16h, p16l, pairs, k);
-        else hipLaunchKernelGGL(update192_bf16w_kernel<false>, grid, dim3(kNTw), 0, s, map, p16h, p16l, pairs, k);
-        return;
-    }
-    if (split) hipLaunchKernelGGL(update192_bf16_kernel<true>, grid, dim3(kNT), 0, s, map, p16h, p16l, pairs, k);
-    else hipLaunchKernelGGL(update192_bf16_kernel<false>, grid, dim3(kNT), 0, s, map, p16h, p16l, pairs, k);
+    if (split) hipLaunchKernelGGL(update192_bf16w_kernel<true>, grid, dim3(kNTw), 0, s, map, p16h, p16l, pairs, k);
+    else hipLaunchKernelGGL(update192_bf16w_kernel<false>, grid, dim3(kNTw), 0, s, map, p16h, p16l, pairs, k);
 }
 
 }  // namespace egs
