@@ -180,6 +180,37 @@ def serve_phase(egs, args):
             "p50_ms": st["p50_ms"], "p99_ms": st["p99_ms"], "completed_per_s": len(tickets) / wall}
 
 
+def bivariate_gcd(egs, args):
+    """The bivariate parsimonious Matérn workload: 25,000 locations uniform on
+    the sphere (latitude, longitude in degrees), haversine distance in km,
+    theta = (1, 1, 637.1 km, 0.5, 1.0, 0.5), dts 960 (matrix dimension 50,000);
+    FP64 and mixed evaluations per second."""
+    import numpy as np
+
+    n_loc = 25000 if args.n == 100000 else args.n
+    rng = np.random.default_rng(0)
+    u, v = rng.random(n_loc), rng.random(n_loc)
+    lat, lon = np.degrees(np.arcsin(2.0 * u - 1.0)), 360.0 * v - 180.0
+    theta = [1.0, 1.0, 637.1, 0.5, 1.0, 0.5]
+    ctx = egs.Context(0)
+    prob = egs.Problem(ctx, "BivariateMaternParsimonious", lat, lon, metric=1, dts=args.dts)
+    prob.generate_observations(theta, seed=0)
+    out = {"workload": "bivariate parsimonious Matern, %d locations on the sphere, great-circle km, theta=%s, "
+                       "dts=%d (dimension %d)" % (n_loc, theta, args.dts, 2 * n_loc)}
+    fp64 = timed(egs, ctx, prob, theta, args.steps, args.warmup, "bivariate fp64")
+    prob.set_precision(egs.PRECISION_MIXED, args.band_fp64, args.band_fp32)
+    mixed = timed(egs, ctx, prob, theta, args.steps, args.warmup, "bivariate mixed")
+    out["fp64"] = {"evals_per_s": fp64["evals_per_s"], "ms_per_step": fp64["ms_per_step"], "loglik": fp64["loglik"],
+                   "tflops": (2 * n_loc) ** 3 / 3.0 / (fp64["ms_per_step"] * 1e-3) / 1e12}
+    out["mixed"] = {"evals_per_s": mixed["evals_per_s"], "ms_per_step": mixed["ms_per_step"],
+                    "loglik": mixed["loglik"],
+                    "rel_llh_err_vs_fp64": abs(mixed["loglik"] - fp64["loglik"]) / abs(fp64["loglik"]),
+                    "refinement": mixed["refinement"], "speedup_vs_fp64": mixed["evals_per_s"] / fp64["evals_per_s"]}
+    prob.close()
+    ctx.close()
+    print(json.dumps(out))
+
+
 def mle_compare(egs, args):
     """Maximum-likelihood fits of one data set with the FP64 and the mixed plan
     (BOBYQA from the lower bounds, as ExaGeoStatDataModeling does)."""
@@ -240,6 +271,8 @@ def main():
                     help="fit theta with the FP64 and the mixed plan on the same data and report both estimates "
                          "(defaults to n=40000, theta=(1, 0.05, 0.5), bounds [0.01, 5], ftol 1e-6)")
     ap.add_argument("--mle-ftol", type=float, default=1e-6)
+    ap.add_argument("--bivariate", action="store_true",
+                    help="the bivariate great-circle workload (25,000 locations on the sphere) instead of the default")
     ap.add_argument("--theta", type=float, nargs=3, default=None)
     ap.add_argument("--maxeval", type=int, default=500)
     ap.add_argument("--no-serve", action="store_true", help="skip the serving phase")
@@ -269,6 +302,8 @@ def main():
 
     if args.mle:
         return mle_compare(egs, args)
+    if args.bivariate:
+        return bivariate_gcd(egs, args)
 
     theta = [1.0, 0.05, 0.5]
     t_setup = time.perf_counter()
