@@ -404,14 +404,21 @@ __global__ void __launch_bounds__(kThreads) potrf64_kernel(TileMap map, int k, i
             for (int q = 0; q < 4; ++q) sA[o + i][o + 4 * jq + q] = d[q];
         }
         __syncthreads();
-        // (2) inverse of the 16 x 16 factor: thread c < 16 solves column c
+        // (2) inverse of the 16 x 16 factor: thread c < 16 solves column c,
+        //     keeping the column in registers
         if (t < 16) {
             const int c = t;
-            for (int r = c; r < 16; ++r) {
+            double wcol[16];
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
                 double acc = (r == c) ? 1.0 : 0.0;
-                for (int l = c; l < r; ++l) acc -= sA[o + r][o + l] * sW[o + l][o + c];
-                sW[o + r][o + c] = acc / sA[o + r][o + r];
+#pragma unroll
+                for (int l = 0; l < r; ++l)
+                    if (l >= c) acc -= sA[o + r][o + l] * wcol[l];
+                wcol[r] = (r < c) ? 0.0 : acc / sA[o + r][o + r];
             }
+#pragma unroll
+            for (int r = 0; r < 16; ++r) sW[o + r][o + c] = wcol[r];
         }
         __syncthreads();
         const int m = 48 - o;   // rows below the 16 x 16 block

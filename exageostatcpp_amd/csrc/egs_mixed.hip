@@ -201,7 +201,7 @@ __device__ __forceinline__ f32x16 mfma_bf16_32(const bf16x8 &x, const bf16x8 &y,
     return __builtin_amdgcn_mfma_f32_32x32x16_bf16(x, y, acc, 0, 0, 0);
 }
 
-template <bool kSplit>
+template <bool kSplit, int kDepth>
 __global__ void __launch_bounds__(kNTw, 1) update192_bf16w_kernel(TileMap map, const __bf16 *p16h, const __bf16 *p16l,
                                                                   const int2 *pairs, int k) {
     constexpr int kParts = kSplit ? 2 : 1;
@@ -240,7 +240,6 @@ __global__ void __launch_bounds__(kNTw, 1) update192_bf16w_kernel(TileMap map, c
 #pragma unroll
             for (int v = 0; v < 16; ++v) acc[mi][ni][v] = 0.f;
 
-    constexpr int kDepth = 2;
     bf16x8 rg[kDepth][kPerThreadW][2 * kParts];
     auto issue = [&](int set, int chunk) {
         const uint32_t kofs = (uint32_t) (chunk * kKCw);
@@ -353,8 +352,11 @@ void launch_update192_bf16(const TileMap &map, const __bf16 *p16h, const __bf16 
     if (npairs <= 0) return;
     const int nsb = map.ts / kBM;
     const dim3 grid((unsigned) npairs * (unsigned) (nsb * nsb));
-    if (split) hipLaunchKernelGGL(update192_bf16w_kernel<true>, grid, dim3(kNTw), 0, s, map, p16h, p16l, pairs, k);
-    else hipLaunchKernelGGL(update192_bf16w_kernel<false>, grid, dim3(kNTw), 0, s, map, p16h, p16l, pairs, k);
+    // two chunks of loads in flight: a third measured no faster
+    if (split)
+        hipLaunchKernelGGL((update192_bf16w_kernel<true, 2>), grid, dim3(kNTw), 0, s, map, p16h, p16l, pairs, k);
+    else
+        hipLaunchKernelGGL((update192_bf16w_kernel<false, 2>), grid, dim3(kNTw), 0, s, map, p16h, p16l, pairs, k);
 }
 
 }  // namespace egs
