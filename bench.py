@@ -9,8 +9,8 @@ HBM, factors it, and reduces log|Sigma| and z' Sigma^-1 z; inputs are resident
 in HBM when the timed region starts.
 
 The headline value is the mixed-precision plan (tiles within band 1 of the
-diagonal FP64, the rest split-BF16 products with FP32 accumulation and FP32
-storage, z' Sigma^-1 z refined to an FP64 residual <= 1e-12).  The same data is first
+diagonal FP64, the rest FP32 storage updated with row-scaled split-FP16
+products accumulated in FP32, z' Sigma^-1 z refined to an FP64 residual <= 1e-12).  The same data is first
 timed with the FP64 plan; the line reports both rates, the speedup and the
 relative log-likelihood error of the mixed plan against FP64.
 
@@ -31,7 +31,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "llh evals/s at n=100k, mixed vs FP64 GPU (rel llh err ≤1e-6); serve p50/p99"
-PEAK = {"fp64": 78.6, "fp32": 157.3, "bf16": 2500.0}
+PEAK = {"fp64": 78.6, "fp32": 157.3, "f16": 2500.0, "bf16": 2500.0}
 
 
 def log(msg):
@@ -61,6 +61,12 @@ def cpu_baseline(n_target, n_sample, dts, theta):
     }
 
 
+def operands(args):
+    """Operand format of the 16-bit update class: 'fp16-split' (row-scaled
+    hi/lo FP16, three products) or 'bf16' (one product)."""
+    return "bf16" if args.plain_bf16 else "fp16-split"
+
+
 def measured_traffic(kernel, args):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 counter
     summary (FETCH_SIZE + WRITE_SIZE, copy-calibrated), when it was taken on
@@ -73,7 +79,7 @@ def measured_traffic(kernel, args):
         return None
     same = (t.get("kernel") == kernel and t.get("n") == args.n and t.get("dts") == args.dts
             and t.get("band_fp64") == args.band_fp64 and t.get("band_fp32") == args.band_fp32
-            and t.get("bf16_products") == (1 if args.plain_bf16 else 3))
+            and t.get("operands") == operands(args))
     return t["fetch_bytes_per_launch"] + t["write_bytes_per_launch"] if same else None
 
 
@@ -101,7 +107,7 @@ def timed(egs, ctx, prob, theta, steps, warmup, label):
     prob.loglik(theta)
     ctx.synchronize()
     kinds = (("update_fp64", egs.capi.PROFILE_UPDATE), ("update_fp32", egs.capi.PROFILE_UPDATE_F32),
-             ("update_bf16", egs.capi.PROFILE_UPDATE_BF16), ("covariance", egs.capi.PROFILE_COVARIANCE),
+             ("update_16", egs.capi.PROFILE_UPDATE_16), ("covariance", egs.capi.PROFILE_COVARIANCE),
              ("panel_trsm", egs.capi.PROFILE_PANEL_TRSM), ("diagonal", egs.capi.PROFILE_DIAGONAL),
              ("forward_solve", egs.capi.PROFILE_SOLVE), ("refine", egs.capi.PROFILE_REFINE))
     ks = {name: prob.kernel_stats(kid) for name, kid in kinds}
@@ -348,7 +354,7 @@ def main():
 
     # dominant kernel of the mixed run
     peaks = {"update_fp64": ("fp64", 1.0), "update_fp32": ("fp32", 1.0),
-             "update_bf16": ("bf16", 1.0 if args.plain_bf16 else 3.0)}
+             "update_16": ("bf16", 1.0) if args.plain_bf16 else ("f16", 3.0)}
     dom = max(peaks, key=lambda k: mixed["kernels"][k]["ms"])
     d = mixed["kernels"][dom]
     achieved = d["flops"] / (d["ms"] * 1e-3) / 1e12 if d["ms"] > 0 else 0.0
@@ -367,15 +373,15 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64+f32+bf16",
+        "dtype": "f64+f32+bf16" if args.plain_bf16 else "f64+f32+f16",
         "data": "synthetic: jittered grid locations (srand(0)), z = L(theta) e from the dlarnv stream, "
                 "generated on the GPU in FP64",
         "config": {"workload": "univariate Matern likelihood, n=%d, theta=(1.0, 0.05, 0.5), dts=%d, mixed plan "
                                "FP64 band <=%d, FP32 band <=%d, %s beyond" % (
                                    args.n, args.dts, args.band_fp64, args.band_fp32,
-                                   "BF16" if args.plain_bf16 else "split-BF16"),
+                                   "BF16" if args.plain_bf16 else "row-scaled split-FP16"),
                    "n": args.n, "dts": args.dts, "band_fp64": args.band_fp64, "band_fp32": args.band_fp32,
-                   "bf16_products": 1 if args.plain_bf16 else 3},
+                   "operands": operands(args)},
         "roofline": {
             "bound": "mfma",
             "kernel": dom,
@@ -387,7 +393,7 @@ def main():
             "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated)",
             "launches": d["launches"],
             "avg_launch_ms": d["ms"] / max(1, d["launches"]),
-            "note": "achieved counts algorithmic flops; frac counts the MFMA work (x%d for split BF16 products)"
+            "note": "achieved counts algorithmic flops; frac counts the MFMA work (x%d for split products)"
                     % int(mfma_per_flop),
         },
         "fp64": {

@@ -66,7 +66,7 @@ public:
     void SetBand(int aBand) { mBand = aBand; }
     int GetBand() const { return mBand; }
     // Tile-distance bands of the mixed plan: FP64 up to DenseBandDP, FP32 up
-    // to BandDenseSP, split-BF16 products beyond (keys "banddensedp"/"banddensesp";
+    // to BandDenseSP, row-scaled split-FP16 products beyond (keys "banddensedp"/"banddensesp";
     // defaults 1 and 1: no FP32 band, which measured both faster and at least as
     // accurate as an FP32 band at n = 100k).
     void SetDenseBandDP(int aBand) { mBandDP = aBand; }

@@ -35,7 +35,7 @@ PROFILE_PANEL_TRSM = 2
 PROFILE_DIAGONAL = 3
 PROFILE_SOLVE = 4
 PROFILE_UPDATE_F32 = 5
-PROFILE_UPDATE_BF16 = 6
+PROFILE_UPDATE_16 = 6
 PROFILE_REFINE = 7
 
 _dp = ctypes.POINTER(ctypes.c_double)

@@ -91,7 +91,8 @@ struct egs_problem_s {
     // the bulk stream still reads panel k
     double *d_p64[2] = {nullptr, nullptr};
     float *d_p32[2] = {nullptr, nullptr};
-    __bf16 *d_p16h[2] = {nullptr, nullptr}, *d_p16l[2] = {nullptr, nullptr};
+    uint16_t *d_p16h[2] = {nullptr, nullptr}, *d_p16l[2] = {nullptr, nullptr};   // FP16 or BF16 bits
+    float *d_pinv[2] = {nullptr, nullptr};   // row scales of the FP16 parts (inverse, powers of two)
     double *d_cg = nullptr;                  // refinement vectors x, r, p, q, w (npad each)
     // FP64 copy of the lower tiles of Sigma written with every covariance
     // generation of a mixed plan (the refinement's Sigma v reads it), and the
@@ -244,6 +245,8 @@ void free_problem_buffers(egs_problem_s *p) {
         hipFree(p->d_p32[b]);
         hipFree(p->d_p16h[b]);
         hipFree(p->d_p16l[b]);
+        hipFree(p->d_pinv[b]);
+        p->d_pinv[b] = nullptr;
         p->d_p64[b] = nullptr;
         p->d_p32[b] = nullptr;
         p->d_p16h[b] = p->d_p16l[b] = nullptr;
@@ -325,7 +328,7 @@ uint8_t tile_store(const egs_problem_s *p, int i, int j) {
     return (i - j) <= p->band64 ? egs::kStoreF64 : egs::kStoreF32;
 }
 
-// Update class of tile (i, j) under a mixed plan: 0 FP64, 1 FP32, 2 BF16.
+// Update class of tile (i, j) under a mixed plan: 0 FP64, 1 FP32, 2 16-bit products.
 int tile_class(const egs_problem_s *p, int i, int j) {
     const int d = i - j;
     return d <= p->band64 ? 0 : (d <= p->band32 ? 1 : 2);
@@ -360,10 +363,12 @@ int setup_mixed(egs_problem_s *p) {
             hipMalloc(&p->d_p64[1], panel * sizeof(double)) != hipSuccess ||
             hipMalloc(&p->d_p32[0], panel * sizeof(float)) != hipSuccess ||
             hipMalloc(&p->d_p32[1], panel * sizeof(float)) != hipSuccess ||
-            hipMalloc(&p->d_p16h[0], panel * sizeof(__bf16)) != hipSuccess ||
-            hipMalloc(&p->d_p16h[1], panel * sizeof(__bf16)) != hipSuccess ||
-            hipMalloc(&p->d_p16l[0], panel * sizeof(__bf16)) != hipSuccess ||
-            hipMalloc(&p->d_p16l[1], panel * sizeof(__bf16)) != hipSuccess ||
+            hipMalloc(&p->d_p16h[0], panel * sizeof(uint16_t)) != hipSuccess ||
+            hipMalloc(&p->d_p16h[1], panel * sizeof(uint16_t)) != hipSuccess ||
+            hipMalloc(&p->d_p16l[0], panel * sizeof(uint16_t)) != hipSuccess ||
+            hipMalloc(&p->d_p16l[1], panel * sizeof(uint16_t)) != hipSuccess ||
+            hipMalloc(&p->d_pinv[0], sizeof(float) * (size_t) p->npad) != hipSuccess ||
+            hipMalloc(&p->d_pinv[1], sizeof(float) * (size_t) p->npad) != hipSuccess ||
             hipMalloc(&p->d_cg, 5 * sizeof(double) * (size_t) p->npad) != hipSuccess ||
             hipMalloc(&p->d_mv, sizeof(double) * (size_t) egs::matvec_chunks(p->n) * p->npad) != hipSuccess ||
             hipMalloc(&p->d_dot, sizeof(double) * egs::dot_blocks()) != hipSuccess)
@@ -477,8 +482,12 @@ void factor_panel(egs_problem_s *p, int k) {
                                       true);
         }
         prof_end(p, pi);
-        if (is_mixed(p))
-            egs::launch_panel_convert(m, k, p->d_p64[k & 1], p->d_p32[k & 1], p->d_p16h[k & 1], p->d_p16l[k & 1], s);
+        if (is_mixed(p)) {
+            const bool split = p->mode == EGS_PRECISION_MIXED;
+            if (split) egs::launch_panel_rowscale(m, k, p->d_pinv[k & 1], s);
+            egs::launch_panel_convert(m, k, p->d_p64[k & 1], p->d_p32[k & 1], p->d_p16h[k & 1], p->d_p16l[k & 1],
+                                      split ? p->d_pinv[k & 1] : nullptr, s);
+        }
     }
 }
 
@@ -502,7 +511,7 @@ double update_flops_cls(const egs_problem_s *p, int c, int j0, int j1) {
 // each class list.
 void launch_mixed_update(egs_problem_s *p, int k, int jcol_lo, int jcol_hi, hipStream_t st) {
     const egs::TileMap m = p->map();
-    static const int prof_id[3] = {EGS_PROFILE_UPDATE, EGS_PROFILE_UPDATE_F32, EGS_PROFILE_UPDATE_BF16};
+    static const int prof_id[3] = {EGS_PROFILE_UPDATE, EGS_PROFILE_UPDATE_F32, EGS_PROFILE_UPDATE_16};
     for (int c = 0; c < 3; ++c) {
         // class-c tiles with column in [jcol_lo, jcol_hi): list entries [count[jcol_hi], count[jcol_lo])
         const int beg = p->cls_count[c][jcol_hi];
@@ -514,8 +523,8 @@ void launch_mixed_update(egs_problem_s *p, int k, int jcol_lo, int jcol_hi, hipS
         if (c == 0) egs::launch_update192_panel(m, p->d_p64[slot], pr, end - beg, k, st);
         else if (c == 1) egs::launch_update192_f32(m, p->d_p32[slot], pr, end - beg, k, st);
         else
-            egs::launch_update192_bf16(m, p->d_p16h[slot], p->d_p16l[slot], pr, end - beg, k,
-                                       p->mode == EGS_PRECISION_MIXED, st);
+            egs::launch_update192_16(m, p->d_p16h[slot], p->d_p16l[slot], p->d_pinv[slot], pr, end - beg, k,
+                                     p->mode == EGS_PRECISION_MIXED, st);
         prof_end(p, pi, st);
     }
 }

@@ -92,11 +92,17 @@ void launch_dot(const double *x, const double *y, int64_t n, double *partial, hi
 // y = a x + b y
 void launch_axpby(double *y, const double *x, double a, double b, int64_t n, hipStream_t s);
 // Mixed-precision factorization pieces (egs_mixed.hip).
-void launch_panel_convert(const TileMap &map, int k, double *p64, float *p32, __bf16 *p16h, __bf16 *p16l,
-                          hipStream_t s);
+// inv_scale[slot * ts + r] = 2^(e+1) for the largest |x| = f 2^e of row r of panel slot.
+void launch_panel_rowscale(const TileMap &map, int k, float *inv_scale, hipStream_t s);
+// Panel k to FP64 and FP32 column-major copies and 16-bit hi/lo row-major parts:
+// FP16 of the row-scaled values when inv_scale is given, BF16 of the values otherwise.
+void launch_panel_convert(const TileMap &map, int k, double *p64, float *p32, uint16_t *p16h, uint16_t *p16l,
+                          const float *inv_scale, hipStream_t s);
 void launch_update192_f32(const TileMap &map, const float *p32, const int2 *pairs, int npairs, int k, hipStream_t s);
-void launch_update192_bf16(const TileMap &map, const __bf16 *p16h, const __bf16 *p16l, const int2 *pairs, int npairs,
-                           int k, bool split, hipStream_t s);
+// C -= A B^T from the 16-bit parts: split (hi*hi + hi*lo + lo*hi, FP16, row scales
+// undone with inv_scale) or a single BF16 product (split false, inv_scale unused).
+void launch_update192_16(const TileMap &map, const uint16_t *p16h, const uint16_t *p16l, const float *inv_scale,
+                         const int2 *pairs, int npairs, int k, bool split, hipStream_t s);
 // FP64 update reading panel k from a contiguous FP64 copy (tile i at p64 + (i-k-1) ts^2).
 void launch_update192_panel(const TileMap &map, const double *p64, const int2 *pairs, int npairs, int k,
                             hipStream_t s);

@@ -3,18 +3,24 @@
 // Plan: tile (i, j) with d = i - j is
 //   d <= b64        : stored FP64, updated with FP64 MFMA (egs_update.hip);
 //   b64 < d <= b32  : stored FP32, updated with v_mfma_f32_16x16x4_f32;
-//   d > b32         : stored FP32, updated with v_mfma_f32_32x32x16_bf16 on
-//                     BF16 operands with FP32 accumulation.  By default each
-//                     FP32 operand is split into hi = bf16(x) and lo = bf16(x - hi)
-//                     and the product is hi*hi + hi*lo + lo*hi (three BF16
-//                     MFMAs, ~16 significant bits); a single hi*hi product is the
-//                     plain-BF16 option.
+//   d > b32         : stored FP32, updated with 32x32x16 MFMAs on 16-bit
+//                     operands with FP32 accumulation.  By default every row of
+//                     the panel is scaled by a power of two so that its largest
+//                     entry lies in [1/2, 1), and each scaled value is split into
+//                     hi = fp16(x) and lo = fp16(x - hi); the product is
+//                     hi*hi + hi*lo + lo*hi (three FP16 MFMAs, ~22 significant
+//                     bits per operand) and the epilogue undoes the two row
+//                     scales exactly.  A single BF16 product of the unscaled
+//                     values is the plain-BF16 option.
 // After panel k is solved, panel_convert_kernel writes it once in the layouts
 // the three update kernels read: FP64 column-major, FP32 column-major, and the
-// BF16 hi and lo parts row-major (K-contiguous: an MFMA operand is 8
+// 16-bit hi and lo parts row-major (K-contiguous: an MFMA operand is 8
 // consecutive K values of one row, one 16-byte load).  The split costs no
 // arithmetic in the GEMM.  Diagonal tiles, their factorization, the
 // log-determinant and every residual stay FP64.
+#include <cstdint>
+#include <type_traits>
+
 #include "egs_common.hpp"
 #include "egs_kernels.hpp"
 
@@ -26,15 +32,50 @@ constexpr int kBM = 192;
 constexpr int kNT = 512;
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef _Float16 half8 __attribute__((ext_vector_type(8)));
 typedef float f32x4v __attribute__((ext_vector_type(4)));
 
 
 // ------------------------------------------------------------------------
-// Panel conversion: tile (i, k) of panel k -> P64, P32 (column-major) and
-// P32T (row-major) at slot i - k - 1.  One workgroup per 64 x 64 block.
+// Row scales of panel k: inv_scale[slot * ts + r] = 2^(e+1) for the largest
+// |x| = f 2^e (1 <= f < 2) of row r of tile (k+1+slot, k), 1 for a zero row.
+// One workgroup per 64 rows; four column groups reduced through LDS.
+// ------------------------------------------------------------------------
+__global__ void __launch_bounds__(kThreads) panel_rowscale_kernel(TileMap map, int k, float *inv_scale) {
+    __shared__ float sm[4][kBlk];
+    const int ts = map.ts;
+    const int nsb = ts / kBlk;
+    const int slot = blockIdx.x / nsb;
+    const int rb = blockIdx.x % nsb;
+    const int i = k + 1 + slot;
+    const int t = threadIdx.x;
+    const int r = t & 63, g = t >> 6;
+    const int64_t row = (int64_t) rb * kBlk + r;
+    float m = 0.0f;
+    if (map.store[i * map.nt + k] == kStoreF32) {
+        const float *A = tile_f32(map, i, k) + row;
+        for (int c = g; c < ts; c += 4) m = fmaxf(m, fabsf(A[(int64_t) c * ts]));
+    } else {
+        const double *A = tile_f64(map, i, k) + row;
+        for (int c = g; c < ts; c += 4) m = fmaxf(m, fabsf((float) A[(int64_t) c * ts]));
+    }
+    sm[g][r] = m;
+    __syncthreads();
+    if (t < kBlk) {
+        const float mx = fmaxf(fmaxf(sm[0][t], sm[1][t]), fmaxf(sm[2][t], sm[3][t]));
+        inv_scale[(int64_t) slot * ts + row - r + t] = (mx > 0.0f && mx < 3.0e38f) ? ldexpf(1.0f, ilogbf(mx) + 1) : 1.0f;
+    }
+}
+
+// ------------------------------------------------------------------------
+// Panel conversion: tile (i, k) of panel k -> P64, P32 (column-major) and the
+// 16-bit hi/lo parts (row-major) at slot i - k - 1.  With inv_scale, the parts
+// are FP16 of the row-scaled values; without, BF16 of the values.  One
+// workgroup per 64 x 64 block.
 // ------------------------------------------------------------------------
 __global__ void __launch_bounds__(kThreads) panel_convert_kernel(TileMap map, int k, double *p64, float *p32,
-                                                                 __bf16 *p16h, __bf16 *p16l) {
+                                                                 uint16_t *p16h, uint16_t *p16l,
+                                                                 const float *inv_scale) {
     __shared__ float sT[kBlk][kBlk + 1];
     const int ts = map.ts;
     const int nsb = ts / kBlk;
@@ -61,10 +102,19 @@ __global__ void __launch_bounds__(kThreads) panel_convert_kernel(TileMap map, in
     for (int rr = t >> 6; rr < kBlk; rr += kThreads / kBlk) {
         const int cc = t & 63;
         const float x = sT[cc][rr];
-        const __bf16 hi = (__bf16) x;
         const int64_t o = slot_off + (int64_t) (br * kBlk + rr) * ts + bc * kBlk + cc;
-        p16h[o] = hi;
-        p16l[o] = (__bf16) (x - (float) hi);
+        if (inv_scale) {
+            const float xs = x / inv_scale[(int64_t) slot * ts + br * kBlk + rr];   // exact: a power of two
+            const _Float16 hi = (_Float16) xs;
+            const _Float16 lo = (_Float16) (xs - (float) hi);
+            p16h[o] = __builtin_bit_cast(uint16_t, hi);
+            p16l[o] = __builtin_bit_cast(uint16_t, lo);
+        } else {
+            const __bf16 hi = (__bf16) x;
+            const __bf16 lo = (__bf16) (x - (float) hi);
+            p16h[o] = __builtin_bit_cast(uint16_t, hi);
+            p16l[o] = __builtin_bit_cast(uint16_t, lo);
+        }
     }
 }
 
@@ -181,12 +231,13 @@ __global__ void __launch_bounds__(kNT, 1) update192_f32_kernel(TileMap map, cons
 }
 
 // ------------------------------------------------------------------------
-// Split-BF16 update with v_mfma_f32_32x32x16_bf16: four waves per 192 x 192
-// block, each owning a 96 x 96 quadrant as 3 x 3 tiles of 32 x 32 (144 FP32
+// 16-bit-operand update with 32x32x16 MFMAs: four waves per 192 x 192 block,
+// each owning a 96 x 96 quadrant as 3 x 3 tiles of 32 x 32 (144 FP32
 // accumulators per lane).  Per 16-deep K step a wave reads 12 operand
-// fragments for 27 MFMAs, half the LDS bytes per flop of the 16 x 16 tiling.
-// Output slot i of lane l holds row (l % 32) of the A operand and column
-// 8 (i / 4) + 4 (l / 32) + i % 4 of the B operand (measured on gfx950).
+// fragments for 27 MFMAs.  Output slot i of lane l holds row (l % 32) of the A
+// operand and column 8 (i / 4) + 4 (l / 32) + i % 4 of the B operand (measured
+// on gfx950).  T16 = _Float16 with row scales (the default split), or __bf16
+// without (the plain-BF16 option, kSplit false).
 // ------------------------------------------------------------------------
 constexpr int kNTw = 256;
 constexpr int kKCw = 32;                  // K per chunk: two MFMA steps
@@ -196,17 +247,24 @@ constexpr int kVecsW = kBM * kPlanesW;    // 768 vectors per operand part per ch
 constexpr int kPerThreadW = kVecsW / kNTw;   // 3
 
 typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef uint16_t u16x8 __attribute__((ext_vector_type(8)));
 
-__device__ __forceinline__ f32x16 mfma_bf16_32(const bf16x8 &x, const bf16x8 &y, f32x16 acc) {
-    return __builtin_amdgcn_mfma_f32_32x32x16_bf16(x, y, acc, 0, 0, 0);
+template <typename T16>
+__device__ __forceinline__ f32x16 mfma_32(const u16x8 &x, const u16x8 &y, f32x16 acc) {
+    if constexpr (std::is_same<T16, _Float16>::value)
+        return __builtin_amdgcn_mfma_f32_32x32x16_f16(__builtin_bit_cast(half8, x), __builtin_bit_cast(half8, y), acc,
+                                                      0, 0, 0);
+    else
+        return __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, x), __builtin_bit_cast(bf16x8, y),
+                                                       acc, 0, 0, 0);
 }
 
-template <bool kSplit, int kDepth>
-__global__ void __launch_bounds__(kNTw, 1) update192_bf16w_kernel(TileMap map, const __bf16 *p16h, const __bf16 *p16l,
-                                                                  const int2 *pairs, int k) {
+template <typename T16, bool kSplit, int kDepth>
+__global__ void __launch_bounds__(kNTw, 1) update192_w_kernel(TileMap map, const uint16_t *p16h, const uint16_t *p16l,
+                                                              const float *inv_scale, const int2 *pairs, int k) {
     constexpr int kParts = kSplit ? 2 : 1;
-    __shared__ bf16x8 sA[2][kParts][kPlanesW][kPlaneW];
-    __shared__ bf16x8 sB[2][kParts][kPlanesW][kPlaneW];
+    __shared__ u16x8 sA[2][kParts][kPlanesW][kPlaneW];
+    __shared__ u16x8 sB[2][kParts][kPlanesW][kPlaneW];
     const int ts = map.ts;
     const int nsb = ts / kBM;
     const int nsub = nsb * nsb;
@@ -218,7 +276,7 @@ __global__ void __launch_bounds__(kNTw, 1) update192_bf16w_kernel(TileMap map, c
     const int64_t tsz = (int64_t) ts * ts;
     const int64_t aoff = (int64_t) (ij.x - k - 1) * tsz + (int64_t) sbm * kBM * ts;
     const int64_t boff = (int64_t) (ij.y - k - 1) * tsz + (int64_t) sbn * kBM * ts;
-    const __bf16 *src[4] = {p16h + aoff, p16l + aoff, p16h + boff, p16l + boff};
+    const uint16_t *src[4] = {p16h + aoff, p16l + aoff, p16h + boff, p16l + boff};
     const int t = threadIdx.x;
     const int lane = t & 63, w = t >> 6;
     const int wm = w & 1, wn = w >> 1;
@@ -240,15 +298,15 @@ __global__ void __launch_bounds__(kNTw, 1) update192_bf16w_kernel(TileMap map, c
 #pragma unroll
             for (int v = 0; v < 16; ++v) acc[mi][ni][v] = 0.f;
 
-    bf16x8 rg[kDepth][kPerThreadW][2 * kParts];
+    u16x8 rg[kDepth][kPerThreadW][2 * kParts];
     auto issue = [&](int set, int chunk) {
         const uint32_t kofs = (uint32_t) (chunk * kKCw);
 #pragma unroll
         for (int q = 0; q < kPerThreadW; ++q)
 #pragma unroll
             for (int p = 0; p < kParts; ++p) {
-                rg[set][q][p] = *reinterpret_cast<const bf16x8 *>(src[p] + goff[q] + kofs);
-                rg[set][q][kParts + p] = *reinterpret_cast<const bf16x8 *>(src[2 + p] + goff[q] + kofs);
+                rg[set][q][p] = *reinterpret_cast<const u16x8 *>(src[p] + goff[q] + kofs);
+                rg[set][q][kParts + p] = *reinterpret_cast<const u16x8 *>(src[2 + p] + goff[q] + kofs);
             }
     };
     auto stage = [&](int set, int buf) {
@@ -277,13 +335,13 @@ __global__ void __launch_bounds__(kNTw, 1) update192_bf16w_kernel(TileMap map, c
 #pragma unroll
             for (int ks = 0; ks < kKCw / 16; ++ks) {
                 const int pl = 2 * ks + lk;
-                bf16x8 ah[3], bh[3];
+                u16x8 ah[3], bh[3];
 #pragma unroll
                 for (int mi = 0; mi < 3; ++mi) ah[mi] = sA[buf][0][pl][wm * 96 + mi * 32 + lr];
 #pragma unroll
                 for (int ni = 0; ni < 3; ++ni) bh[ni] = sB[buf][0][pl][wn * 96 + ni * 32 + lr];
                 if (kSplit) {
-                    bf16x8 al[3], bl[3];
+                    u16x8 al[3], bl[3];
 #pragma unroll
                     for (int mi = 0; mi < 3; ++mi) al[mi] = sA[buf][kParts - 1][pl][wm * 96 + mi * 32 + lr];
 #pragma unroll
@@ -292,25 +350,39 @@ __global__ void __launch_bounds__(kNTw, 1) update192_bf16w_kernel(TileMap map, c
                     for (int mi = 0; mi < 3; ++mi)
 #pragma unroll
                         for (int ni = 0; ni < 3; ++ni) {
-                            acc[mi][ni] = mfma_bf16_32(bh[ni], ah[mi], acc[mi][ni]);
-                            acc[mi][ni] = mfma_bf16_32(bh[ni], al[mi], acc[mi][ni]);
-                            acc[mi][ni] = mfma_bf16_32(bl[ni], ah[mi], acc[mi][ni]);
+                            acc[mi][ni] = mfma_32<T16>(bh[ni], ah[mi], acc[mi][ni]);
+                            acc[mi][ni] = mfma_32<T16>(bh[ni], al[mi], acc[mi][ni]);
+                            acc[mi][ni] = mfma_32<T16>(bl[ni], ah[mi], acc[mi][ni]);
                         }
                 } else {
 #pragma unroll
                     for (int mi = 0; mi < 3; ++mi)
 #pragma unroll
-                        for (int ni = 0; ni < 3; ++ni) acc[mi][ni] = mfma_bf16_32(bh[ni], ah[mi], acc[mi][ni]);
+                        for (int ni = 0; ni < 3; ++ni) acc[mi][ni] = mfma_32<T16>(bh[ni], ah[mi], acc[mi][ni]);
                 }
             }
             if (c + 1 < nchunks) stage((u + 1) % kDepth, buf ^ 1);
             __syncthreads();
         }
     }
-    // C -= acc: lane row (A side) wm*96 + mi*32 + lr, column (B side)
-    // wn*96 + ni*32 + 8 (v / 4) + 4 lk + v % 4; one 3 x 16 batch per mi
+    // C -= acc (times the two row scales' inverses, exact powers of two): lane
+    // row (A side) wm*96 + mi*32 + lr, column (B side) wn*96 + ni*32 + 8 (v / 4)
+    // + 4 lk + v % 4; one 3 x 16 batch per mi
     float *C = tile_f32(map, ij.x, ij.y);
     const uint32_t cbase = (uint32_t) ((sbm * kBM + wm * 96 + lr) + (sbn * kBM + wn * 96 + 4 * lk) * ts);
+    constexpr bool kScaled = std::is_same<T16, _Float16>::value;
+    float sa[3] = {1.0f, 1.0f, 1.0f};
+    f32x4v sb[3][4];
+    if constexpr (kScaled) {
+        const float *invA = inv_scale + (int64_t) (ij.x - k - 1) * ts + sbm * kBM + wm * 96 + lr;
+        const float *invB = inv_scale + (int64_t) (ij.y - k - 1) * ts + sbn * kBM + wn * 96 + 4 * lk;
+#pragma unroll
+        for (int mi = 0; mi < 3; ++mi) sa[mi] = invA[mi * 32];
+#pragma unroll
+        for (int ni = 0; ni < 3; ++ni)
+#pragma unroll
+            for (int g = 0; g < 4; ++g) sb[ni][g] = *reinterpret_cast<const f32x4v *>(invB + ni * 32 + 8 * g);
+    }
 #pragma unroll
     for (int mi = 0; mi < 3; ++mi) {
         float cv[3][16];
@@ -323,21 +395,31 @@ __global__ void __launch_bounds__(kNTw, 1) update192_bf16w_kernel(TileMap map, c
 #pragma unroll
         for (int ni = 0; ni < 3; ++ni)
 #pragma unroll
-            for (int v = 0; v < 16; ++v)
-                __builtin_nontemporal_store(cv[ni][v] - acc[mi][ni][v],
+            for (int v = 0; v < 16; ++v) {
+                float a = acc[mi][ni][v];
+                if constexpr (kScaled) a *= sa[mi] * sb[ni][v >> 2][v & 3];
+                __builtin_nontemporal_store(cv[ni][v] - a,
                                             &C[cbase + (uint32_t) (mi * 32 + (ni * 32 + 8 * (v >> 2) + (v & 3)) * ts)]);
+            }
     }
 }
 
 }  // namespace
 
-void launch_panel_convert(const TileMap &map, int k, double *p64, float *p32, __bf16 *p16h, __bf16 *p16l,
-                          hipStream_t s) {
+void launch_panel_rowscale(const TileMap &map, int k, float *inv_scale, hipStream_t s) {
+    const int rows = map.nt - k - 1;
+    if (rows <= 0) return;
+    const int nsb = map.ts / kBlk;
+    hipLaunchKernelGGL(panel_rowscale_kernel, dim3((unsigned) (rows * nsb)), dim3(kThreads), 0, s, map, k, inv_scale);
+}
+
+void launch_panel_convert(const TileMap &map, int k, double *p64, float *p32, uint16_t *p16h, uint16_t *p16l,
+                          const float *inv_scale, hipStream_t s) {
     const int rows = map.nt - k - 1;
     if (rows <= 0) return;
     const int nsb = map.ts / kBlk;
     hipLaunchKernelGGL(panel_convert_kernel, dim3((unsigned) (rows * nsb * nsb)), dim3(kThreads), 0, s, map, k, p64,
-                       p32, p16h, p16l);
+                       p32, p16h, p16l, inv_scale);
 }
 
 void launch_update192_f32(const TileMap &map, const float *p32, const int2 *pairs, int npairs, int k, hipStream_t s) {
@@ -347,16 +429,18 @@ void launch_update192_f32(const TileMap &map, const float *p32, const int2 *pair
                        p32, pairs, k);
 }
 
-void launch_update192_bf16(const TileMap &map, const __bf16 *p16h, const __bf16 *p16l, const int2 *pairs, int npairs,
-                           int k, bool split, hipStream_t s) {
+void launch_update192_16(const TileMap &map, const uint16_t *p16h, const uint16_t *p16l, const float *inv_scale,
+                         const int2 *pairs, int npairs, int k, bool split, hipStream_t s) {
     if (npairs <= 0) return;
     const int nsb = map.ts / kBM;
     const dim3 grid((unsigned) npairs * (unsigned) (nsb * nsb));
     // two chunks of loads in flight: a third measured no faster
     if (split)
-        hipLaunchKernelGGL((update192_bf16w_kernel<true, 2>), grid, dim3(kNTw), 0, s, map, p16h, p16l, pairs, k);
+        hipLaunchKernelGGL((update192_w_kernel<_Float16, true, 2>), grid, dim3(kNTw), 0, s, map, p16h, p16l,
+                           inv_scale, pairs, k);
     else
-        hipLaunchKernelGGL((update192_bf16w_kernel<false, 2>), grid, dim3(kNTw), 0, s, map, p16h, p16l, pairs, k);
+        hipLaunchKernelGGL((update192_w_kernel<__bf16, false, 2>), grid, dim3(kNTw), 0, s, map, p16h, p16l, nullptr,
+                           pairs, k);
 }
 
 }  // namespace egs

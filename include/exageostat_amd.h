@@ -40,9 +40,10 @@ extern "C" {
 #define EGS_METRIC_GREAT_CIRCLE 1
 
 /* Precision plans for the factorization.  MIXED: FP64 near the diagonal,
- * FP32 further out, and beyond that BF16 MFMA products of split operands
- * (hi*hi + hi*lo + lo*hi, FP32 accumulation).  MIXED_BF16: the same with a
- * single BF16 product.  Both refine z'Sigma^{-1}z with FP64 residuals. */
+ * FP32 further out, and beyond that FP16 MFMA products of split operands
+ * (each panel row scaled by a power of two, hi = fp16(x), lo = fp16(x - hi),
+ * hi*hi + hi*lo + lo*hi, FP32 accumulation).  MIXED_BF16: one BF16 product of
+ * the unscaled values instead.  Both refine z'Sigma^{-1}z with FP64 residuals. */
 #define EGS_PRECISION_FP64 0
 #define EGS_PRECISION_MIXED 1
 #define EGS_PRECISION_MIXED_BF16 2
@@ -88,9 +89,9 @@ int egs_problem_destroy(egs_problem_t problem);
 int egs_problem_device_bytes(egs_problem_t problem, int64_t *bytes);
 
 /* Precision plan: EGS_PRECISION_FP64, or a mixed plan with tiles whose tile
- * distance |i - j| <= band_fp64 in FP64, <= band_fp32 in FP32 and the rest in
- * BF16 products accumulated in FP32 (needs a tile size that is a multiple of
- * 192).  Under a mixed plan the factor's diagonal gives log|Sigma| and
+ * distance |i - j| <= band_fp64 in FP64, <= band_fp32 in FP32 and the rest
+ * stored in FP32 and updated with 16-bit products accumulated in FP32 (needs a
+ * tile size that is a multiple of 192).  Under a mixed plan the factor's diagonal gives log|Sigma| and
  * z'Sigma^{-1}z comes from conjugate gradients preconditioned by the factor,
  * with residuals b - Sigma x evaluated in FP64 until ||r|| <= 1e-12 ||b||. */
 int egs_problem_set_precision(egs_problem_t problem, int mode, int band_fp64, int band_fp32);
@@ -143,7 +144,7 @@ int egs_last_timings(egs_problem_t problem, double *t_covariance, double *t_fact
 #define EGS_PROFILE_DIAGONAL 3      /* diagonal-tile factorization (all its launches) */
 #define EGS_PROFILE_SOLVE 4         /* forward solve with log-det and dot product */
 #define EGS_PROFILE_UPDATE_F32 5    /* mixed plans: FP32-band update */
-#define EGS_PROFILE_UPDATE_BF16 6   /* mixed plans: BF16-band update */
+#define EGS_PROFILE_UPDATE_16 6      /* mixed plans: update with 16-bit products */
 #define EGS_PROFILE_REFINE 7        /* mixed plans: refinement of z'Sigma^{-1}z */
 #define EGS_PROFILE_KERNELS 8
 int egs_problem_profile(egs_problem_t problem, int enable);

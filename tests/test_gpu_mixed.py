@@ -1,4 +1,4 @@
-"""Mixed-precision plans (FP64 band / FP32 band / BF16 products) against the
+"""Mixed-precision plans (FP64 band / FP32 band / 16-bit products) against the
 FP64 oracle, through the C ABI."""
 import sys
 
@@ -25,7 +25,7 @@ def ctx():
 
 @pytest.fixture(scope="module")
 def data4k():
-    # 21 tiles of 192: every band class (FP64 <= 2, FP32 3..10, BF16 > 10) occurs
+    # 21 tiles of 192: every band class (FP64 <= 2, FP32 3..10, 16-bit > 10) occurs
     x, y = orc.generate_locations(4000, seed=0)
     th = [1.0, 0.05, 0.5]
     z = orc.generate_observations(0, (x, y), th, 192, seed=0)
@@ -35,7 +35,7 @@ def data4k():
 
 def test_mixed_loglik_within_1e6(ctx, data4k):
     x, y, th, z, ref = data4k
-    log("mixed (split BF16) loglik n=4000 dts=192")
+    log("mixed (split FP16) loglik n=4000 dts=192")
     p = egs.Problem(ctx, 0, x, y, dts=192)
     p.set_observations(z)
     p.set_precision(egs.PRECISION_MIXED, 2, 10)
@@ -47,6 +47,27 @@ def test_mixed_loglik_within_1e6(ctx, data4k):
     assert rf["residual"] <= 1e-12
     assert abs(q - ref["quad"]) <= 1e-9 * abs(ref["quad"])
     assert rel <= 1e-6
+
+
+@pytest.mark.parametrize("log2c", [-20, 40])
+def test_mixed_plan_is_equivariant_under_power_of_two_variance(ctx, data4k, log2c):
+    """Sigma(c sigma^2) = c Sigma(sigma^2) with c = 2^log2c: every step of the
+    mixed plan (FP64 and FP32 storage, row-scaled FP16 parts, refinement) then
+    scales exactly, so log|Sigma| shifts by n ln c and z'Sigma^{-1}z divides by
+    c.  At c = 2^40 the factor's entries (~2^20) exceed the FP16 range and at
+    c = 2^-20 the lo parts fall below it: a split without the row scales, or an
+    epilogue that does not undo them, breaks the identity."""
+    x, y, th, z, _ = data4k
+    n = len(x)
+    c = 2.0 ** log2c
+    p = egs.Problem(ctx, 0, x, y, dts=192)
+    p.set_observations(z)
+    p.set_precision(egs.PRECISION_MIXED, 1, 3)
+    _, ld1, q1 = p.loglik(th)
+    _, ldc, qc = p.loglik([th[0] * c, th[1], th[2]])
+    log("  2^%d: logdet shift %.12f (n ln c %.12f), quad ratio %.15f" % (log2c, ldc - ld1, n * np.log(c), qc * c / q1))
+    assert abs((ldc - ld1) - n * np.log(c)) <= 1e-10 * abs(n * np.log(c))
+    assert abs(qc * c - q1) <= 1e-12 * abs(q1)
 
 
 def test_mixed_plain_bf16_refines_quad(ctx, data4k):
@@ -105,7 +126,7 @@ def _mle_fixture():
 
 def test_data_modeling_fp64_and_mixed_theta_hat(ctx):
     """egs_data_modeling (BOBYQA on the GPU likelihood) at n = 1600: the FP64 fit
-    lands within 1e-4 of the independent optimum, the mixed fit (BF16 products
+    lands within 1e-4 of the independent optimum, the mixed fit (16-bit products
     from band 4 on) within 1e-3 of it."""
     g = _mle_fixture()
     x, y = orc.generate_locations(g["n"], seed=0)
