@@ -297,7 +297,8 @@ def main():
     ap.add_argument("--band-fp64", type=int, default=1)
     ap.add_argument("--band-fp32", type=int, default=1)
     ap.add_argument("--plain-bf16", action="store_true", help="single BF16 product instead of the split one")
-    ap.add_argument("--cpu-sample", type=int, default=6000)
+    ap.add_argument("--cpu-sample", type=int, default=14400,
+                    help="n of the oracle sample (about 10-20 s on 16 host cores)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--mle", action="store_true",
                     help="fit theta with the FP64 and the mixed plan on the same data and report both estimates "
