@@ -67,6 +67,11 @@ def operands(args):
     return "bf16" if args.plain_bf16 else "fp16-split"
 
 
+def tau_json(args):
+    """--single-tau for the JSON line (a string for infinity, which JSON lacks)."""
+    return "inf" if args.single_tau == float("inf") else args.single_tau
+
+
 def measured_traffic(kernel, args):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 counter
     summary (FETCH_SIZE + WRITE_SIZE, copy-calibrated), when it was taken on
@@ -79,6 +84,7 @@ def measured_traffic(kernel, args):
         return None
     same = (t.get("kernel") == kernel and t.get("n") == args.n and t.get("dts") == args.dts
             and t.get("band_fp64") == args.band_fp64 and t.get("band_fp32") == args.band_fp32
+            and t.get("single_band") == args.single_band and t.get("single_tau") == tau_json(args)
             and t.get("operands") == operands(args))
     return t["fetch_bytes_per_launch"] + t["write_bytes_per_launch"] if same else None
 
@@ -107,7 +113,8 @@ def timed(egs, ctx, prob, theta, steps, warmup, label):
     prob.loglik(theta)
     ctx.synchronize()
     kinds = (("update_fp64", egs.capi.PROFILE_UPDATE), ("update_fp32", egs.capi.PROFILE_UPDATE_F32),
-             ("update_16", egs.capi.PROFILE_UPDATE_16), ("covariance", egs.capi.PROFILE_COVARIANCE),
+             ("update_16", egs.capi.PROFILE_UPDATE_16), ("update_16_select", egs.capi.PROFILE_UPDATE_16_SELECT),
+             ("covariance", egs.capi.PROFILE_COVARIANCE),
              ("panel_trsm", egs.capi.PROFILE_PANEL_TRSM), ("diagonal", egs.capi.PROFILE_DIAGONAL),
              ("forward_solve", egs.capi.PROFILE_SOLVE), ("refine", egs.capi.PROFILE_REFINE))
     ks = {name: prob.kernel_stats(kid) for name, kid in kinds}
@@ -119,6 +126,7 @@ def timed(egs, ctx, prob, theta, steps, warmup, label):
         "phases_ms_per_step": {k: v / steps for k, v in phases.items()},
         "kernels": ks,
         "refinement": prob.refinement(),
+        "product_counts": prob.product_counts(),
     }
 
 
@@ -219,6 +227,7 @@ def bivariate_gcd(egs, args):
     out["mixed"] = []
     for b64, b32 in ((1, 1), (2, 2), (4, 4), (8, 8)):
         prob.set_precision(egs.PRECISION_MIXED, b64, b32)
+        prob.set_single_product(args.single_band, args.single_tau)
         try:
             mixed = timed(egs, ctx, prob, theta, args.steps, args.warmup, "bivariate mixed (%d, %d)" % (b64, b32))
         except egs.NotPositiveDefinite as e:
@@ -229,6 +238,7 @@ def bivariate_gcd(egs, args):
                              "ms_per_step": mixed["ms_per_step"], "loglik": mixed["loglik"],
                              "rel_llh_err_vs_fp64": abs(mixed["loglik"] - fp64["loglik"]) / abs(fp64["loglik"]),
                              "refinement": mixed["refinement"],
+        "block_updates_beyond_single_band": mixed["product_counts"],
                              "speedup_vs_fp64": mixed["evals_per_s"] / fp64["evals_per_s"]})
     prob.close()
     ctx.close()
@@ -257,6 +267,7 @@ def mle_compare(egs, args):
             fp64_prob = egs.Problem(ctx, "UnivariateMaternStationary", x, y, dts=args.dts)
             fp64_prob.set_observations(prob.get_observations())
             prob.set_precision(mode, args.band_fp64, args.band_fp32)
+            prob.set_single_product(args.single_band, args.single_tau)
         count, fallback = [0], [0]
 
         def objective(v):
@@ -279,7 +290,8 @@ def mle_compare(egs, args):
     d = [abs(a - b) for a, b in zip(out["fp64"]["theta_hat"], out["mixed"]["theta_hat"])]
     out["max_abs_theta_diff"] = max(d)
     out["fp64_loglik_at_mixed_theta_hat"] = fp64_prob.loglik(out["mixed"]["theta_hat"])[0]
-    out["mixed_plan"] = {"band_fp64": args.band_fp64, "band_fp32": args.band_fp32}
+    out["mixed_plan"] = {"band_fp64": args.band_fp64, "band_fp32": args.band_fp32, "single_band": args.single_band,
+                         "single_tau": tau_json(args)}
     fp64_prob.close()
     prob.close()
     ctx.close()
@@ -294,8 +306,13 @@ def main():
     ap.add_argument("--fp64-steps", type=int, default=None, help="timed FP64 steps (default: --steps)")
     ap.add_argument("--n", type=int, default=100000)
     ap.add_argument("--dts", type=int, default=960)
-    ap.add_argument("--band-fp64", type=int, default=1)
-    ap.add_argument("--band-fp32", type=int, default=1)
+    ap.add_argument("--band-fp64", type=int, default=2)
+    ap.add_argument("--band-fp32", type=int, default=2)
+    ap.add_argument("--single-band", type=int, default=0,
+                    help="blocks of tiles beyond this band may take one FP16 product instead of the split three "
+                         "(negative: never)")
+    ap.add_argument("--single-tau", type=float, default=2.0 ** -16,
+                    help="... when their panel rows' scales multiply to <= tau times the panel's largest squared")
     ap.add_argument("--plain-bf16", action="store_true", help="single BF16 product instead of the split one")
     ap.add_argument("--cpu-sample", type=int, default=14400,
                     help="n of the oracle sample (about 10-20 s on 16 host cores)")
@@ -350,16 +367,21 @@ def main():
     fp64 = timed(egs, ctx, prob, theta, fp64_steps, args.warmup, "fp64")
     mode = egs.PRECISION_MIXED_BF16 if args.plain_bf16 else egs.PRECISION_MIXED
     prob.set_precision(mode, args.band_fp64, args.band_fp32)
+    prob.set_single_product(args.single_band, args.single_tau)
     mixed = timed(egs, ctx, prob, theta, args.steps, args.warmup, "mixed")
     rel_err = abs(mixed["loglik"] - fp64["loglik"]) / abs(fp64["loglik"])
 
     # dominant kernel of the mixed run
     peaks = {"update_fp64": ("fp64", 1.0), "update_fp32": ("fp32", 1.0),
-             "update_16": ("bf16", 1.0) if args.plain_bf16 else ("f16", 3.0)}
+             "update_16": ("bf16", 1.0) if args.plain_bf16 else ("f16", 3.0), "update_16_select": ("f16", 1.0)}
     dom = max(peaks, key=lambda k: mixed["kernels"][k]["ms"])
     d = mixed["kernels"][dom]
     achieved = d["flops"] / (d["ms"] * 1e-3) / 1e12 if d["ms"] > 0 else 0.0
     prec, mfma_per_flop = peaks[dom]
+    pc = mixed["product_counts"]
+    if dom == "update_16_select" and pc["split"] + pc["single"] > 0:
+        # MFMAs per product: 3 for split blocks, 1 for single-product ones
+        mfma_per_flop = (3.0 * pc["split"] + pc["single"]) / (pc["split"] + pc["single"])
     dom64 = fp64["kernels"]["update_fp64"]
     ach64 = dom64["flops"] / (dom64["ms"] * 1e-3) / 1e12 if dom64["ms"] > 0 else 0.0
 
@@ -380,8 +402,13 @@ def main():
         "config": {"workload": "univariate Matern likelihood, n=%d, theta=(1.0, 0.05, 0.5), dts=%d, mixed plan "
                                "FP64 band <=%d, FP32 band <=%d, %s beyond" % (
                                    args.n, args.dts, args.band_fp64, args.band_fp32,
-                                   "BF16" if args.plain_bf16 else "row-scaled split-FP16"),
+                                   "BF16" if args.plain_bf16 else (
+                                       "row-scaled split-FP16" if args.single_band < 0 else
+                                       "row-scaled split-FP16, one FP16 product beyond band %d where the panel "
+                                       "rows' scales multiply to <= %g of the panel's largest squared"
+                                       % (args.single_band, args.single_tau))),
                    "n": args.n, "dts": args.dts, "band_fp64": args.band_fp64, "band_fp32": args.band_fp32,
+                   "single_band": args.single_band, "single_tau": tau_json(args),
                    "operands": operands(args)},
         "roofline": {
             "bound": "mfma",
@@ -394,8 +421,9 @@ def main():
             "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated)",
             "launches": d["launches"],
             "avg_launch_ms": d["ms"] / max(1, d["launches"]),
-            "note": "achieved counts algorithmic flops; frac counts the MFMA work (x%d for split products)"
-                    % int(mfma_per_flop),
+            "mfma_per_product": mfma_per_flop,
+            "note": "achieved counts algorithmic flops; frac counts the MFMA work (3 MFMAs per split product, "
+                    "1 per single one, weighted by the block counts)",
         },
         "fp64": {
             "evals_per_s": fp64["evals_per_s"], "ms_per_step": fp64["ms_per_step"], "steps": fp64_steps,
@@ -410,6 +438,7 @@ def main():
             "logdet_err_vs_fp64": mixed["logdet"] - fp64["logdet"],
             "quad_rel_err_vs_fp64": (mixed["quad"] - fp64["quad"]) / fp64["quad"],
             "refinement": mixed["refinement"],
+            "block_updates_beyond_single_band": mixed["product_counts"],
             "speedup_vs_fp64": mixed["evals_per_s"] / fp64["evals_per_s"],
             "phases_ms_per_step": mixed["phases_ms_per_step"],
             "kernels_per_step": kernel_table(mixed["kernels"], 1),

@@ -66,9 +66,11 @@ public:
     void SetBand(int aBand) { mBand = aBand; }
     int GetBand() const { return mBand; }
     // Tile-distance bands of the mixed plan: FP64 up to DenseBandDP, FP32 up
-    // to BandDenseSP, row-scaled split-FP16 products beyond (keys "banddensedp"/"banddensesp";
-    // defaults 1 and 1: no FP32 band, which measured both faster and at least as
-    // accurate as an FP32 band at n = 100k).
+    // to BandDenseSP, 16-bit products beyond (row-scaled split FP16, one FP16
+    // product for blocks with small panel rows; keys "banddensedp"/"banddensesp").
+    // Defaults 2 and 2: no FP32 band, which measured both faster and at least as
+    // accurate as an FP32 band at n = 100k; FP64 band 2 keeps the 40k fit's
+    // estimate within 1e-3 of the FP64 one, where band 1 did not.
     void SetDenseBandDP(int aBand) { mBandDP = aBand; }
     int GetDenseBandDP() const { return mBandDP; }
     void SetBandDenseSP(int aBand) { mBandSP = aBand; }
@@ -138,7 +140,7 @@ private:
     int mTimeSlot = 1;
     common::Computation mComputation = common::EXACT_DENSE;
     common::Precision mPrecision = common::DOUBLE;
-    int mCores = 1, mGpus = 0, mDts = 8, mBand = 0, mBandDP = 1, mBandSP = 1;
+    int mCores = 1, mGpus = 0, mDts = 8, mBand = 0, mBandDP = 2, mBandSP = 2;
     int mSeed = 0;
     std::vector<double> mInitialTheta, mLowerBounds, mUpperBounds, mEstimatedTheta, mStartingTheta;
     common::Dimension mDimension = common::Dimension2D;

@@ -37,6 +37,7 @@ PROFILE_SOLVE = 4
 PROFILE_UPDATE_F32 = 5
 PROFILE_UPDATE_16 = 6
 PROFILE_REFINE = 7
+PROFILE_UPDATE_16_SELECT = 8
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _i64 = ctypes.c_int64
@@ -67,6 +68,7 @@ EXPORTS = {
     "egs_problem_destroy": ([_vp], _int),
     "egs_problem_device_bytes": ([_vp, ctypes.POINTER(_i64)], _int),
     "egs_problem_set_precision": ([_vp, _int, _int, _int], _int),
+    "egs_problem_set_single_product": ([_vp, _int, ctypes.c_double], _int),
     "egs_problem_set_observations": ([_vp, _dp, _i64], _int),
     "egs_problem_get_observations": ([_vp, _dp, _i64], _int),
     "egs_loglik": ([_vp, _dp, _dp, _dp, _dp], _int),
@@ -76,6 +78,7 @@ EXPORTS = {
     "egs_get_matrix": ([_vp, _dp, _i64], _int),
     "egs_last_timings": ([_vp, _dp, _dp, _dp], _int),
     "egs_last_refinement": ([_vp, ctypes.POINTER(_int), _dp], _int),
+    "egs_last_product_counts": ([_vp, ctypes.POINTER(_i64), ctypes.POINTER(_i64)], _int),
     "egs_problem_profile": ([_vp, _int], _int),
     "egs_problem_kernel_stats": ([_vp, _int, ctypes.POINTER(_i64), _dp, _dp], _int),
     "egs_generate_locations": ([_i64, _int, _dp, _dp, _dp], _int),
@@ -203,8 +206,14 @@ class Problem:
         _check(lib().egs_problem_device_bytes(self.handle, ctypes.byref(b)))
         return b.value
 
-    def set_precision(self, mode, band_fp64=1, band_fp32=1):
+    def set_precision(self, mode, band_fp64=2, band_fp32=2):
         _check(lib().egs_problem_set_precision(self.handle, mode, band_fp64, band_fp32))
+
+    def set_single_product(self, band, tau=2.0 ** -16):
+        """Blocks of tiles beyond `band` take one FP16 product instead of the
+        split three when their panel rows are small (row-scale product <= tau
+        times the panel's largest squared); band None or negative: never."""
+        _check(lib().egs_problem_set_single_product(self.handle, -1 if band is None else int(band), float(tau)))
 
     def set_observations(self, z):
         z, pz = _arr(z)
@@ -248,6 +257,13 @@ class Problem:
         it, res = _int(), ctypes.c_double()
         _check(lib().egs_last_refinement(self.handle, ctypes.byref(it), ctypes.byref(res)))
         return {"iterations": it.value, "residual": res.value}
+
+    def product_counts(self):
+        """Block updates of the last factorization with the split and with the
+        single FP16 product."""
+        sp, si = _i64(), _i64()
+        _check(lib().egs_last_product_counts(self.handle, ctypes.byref(sp), ctypes.byref(si)))
+        return {"split": sp.value, "single": si.value}
 
     def profile(self, enable=True):
         _check(lib().egs_problem_profile(self.handle, 1 if enable else 0))

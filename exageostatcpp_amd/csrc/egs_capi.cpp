@@ -75,17 +75,26 @@ struct Profiler {
     KernelStat stats[EGS_PROFILE_KERNELS];
 };
 
+// Update classes of a mixed plan (see tile_class).
+constexpr int kClasses = 4;
+
 struct egs_problem_s {
     egs_context_s *ctx = nullptr;
     int kernel = 0, p = 1, metric = 0, dim = 2;
     int dts = 0, ts = 0, nt = 0, nsb = 0;
     int64_t n_loc = 0, n = 0, npad = 0;
     int mode = EGS_PRECISION_FP64, band64 = 2, band32 = 10;
+    // One-product FP16 for blocks beyond band_single (-1: never) whose panel
+    // rows have bmax(A) bmax(B) <= tau_single pmax^2.  With 2^-16 such a
+    // block's product error, about 2^-12 K bmax(A) bmax(B), stays 16x under
+    // 2^-24 K pmax^2, the split product's for the panel's largest block.
+    int band_single = 0;
+    double tau_single = 0x1p-16;
     // mixed plans: per-class update lists (descending column) and, for class c,
     // cls_count[c][j] = number of its tiles in columns >= j
-    int2 *d_pairs_cls[3] = {nullptr, nullptr, nullptr};
-    std::vector<int2> h_pairs_cls[3];
-    std::vector<int> cls_count[3];
+    int2 *d_pairs_cls[kClasses] = {nullptr, nullptr, nullptr, nullptr};
+    std::vector<int2> h_pairs_cls[kClasses];
+    std::vector<int> cls_count[kClasses];
     // panel k as FP64 column-major, FP32 column-major and FP32 row-major, in
     // slot k % 2: with the lookahead the critical stream writes panel k+1 while
     // the bulk stream still reads panel k
@@ -93,6 +102,9 @@ struct egs_problem_s {
     float *d_p32[2] = {nullptr, nullptr};
     uint16_t *d_p16h[2] = {nullptr, nullptr}, *d_p16l[2] = {nullptr, nullptr};   // FP16 or BF16 bits
     float *d_pinv[2] = {nullptr, nullptr};   // row scales of the FP16 parts (inverse, powers of two)
+    float *d_bmax[2] = {nullptr, nullptr};   // largest row scale per 192-row block of a panel
+    unsigned *d_pmax = nullptr;              // largest row scale per panel (float bits), nt entries
+    unsigned long long *d_counts = nullptr;  // split / single-product block updates of the last factorization
     double *d_cg = nullptr;                  // refinement vectors x, r, p, q, w (npad each)
     // FP64 copy of the lower tiles of Sigma written with every covariance
     // generation of a mixed plan (the refinement's Sigma v reads it), and the
@@ -246,11 +258,17 @@ void free_problem_buffers(egs_problem_s *p) {
         hipFree(p->d_p16h[b]);
         hipFree(p->d_p16l[b]);
         hipFree(p->d_pinv[b]);
+        hipFree(p->d_bmax[b]);
         p->d_pinv[b] = nullptr;
+        p->d_bmax[b] = nullptr;
         p->d_p64[b] = nullptr;
         p->d_p32[b] = nullptr;
         p->d_p16h[b] = p->d_p16l[b] = nullptr;
     }
+    hipFree(p->d_pmax);
+    hipFree(p->d_counts);
+    p->d_pmax = nullptr;
+    p->d_counts = nullptr;
     hipFree(p->d_cg);
     hipFree(p->d_mv);
     hipFree(p->d_dot);
@@ -328,10 +346,15 @@ uint8_t tile_store(const egs_problem_s *p, int i, int j) {
     return (i - j) <= p->band64 ? egs::kStoreF64 : egs::kStoreF32;
 }
 
-// Update class of tile (i, j) under a mixed plan: 0 FP64, 1 FP32, 2 16-bit products.
+// Update class of tile (i, j) under a mixed plan: 0 FP64, 1 FP32, 2 16-bit
+// products (split FP16, or one BF16 product under MIXED_BF16), 3 one FP16
+// product (beyond band_single under MIXED, per block when tau_single is finite).
 int tile_class(const egs_problem_s *p, int i, int j) {
     const int d = i - j;
-    return d <= p->band64 ? 0 : (d <= p->band32 ? 1 : 2);
+    if (d <= p->band64) return 0;
+    if (d <= p->band32) return 1;
+    if (p->mode == EGS_PRECISION_MIXED && p->band_single >= 0 && d > p->band_single) return 3;
+    return 2;
 }
 
 bool is_mixed(const egs_problem_s *p) { return p->mode != EGS_PRECISION_FP64; }
@@ -339,15 +362,15 @@ bool is_mixed(const egs_problem_s *p) { return p->mode != EGS_PRECISION_FP64; }
 // Buffers of a mixed plan: class lists, panel copies, refinement vectors.
 int setup_mixed(egs_problem_s *p) {
     const int nt = p->nt;
-    for (int c = 0; c < 3; ++c) {
+    for (int c = 0; c < kClasses; ++c) {
         p->h_pairs_cls[c].clear();
         p->cls_count[c].assign((size_t) nt + 1, 0);
     }
     for (int j = nt - 1; j >= 0; --j) {
         for (int i = j; i < nt; ++i) p->h_pairs_cls[tile_class(p, i, j)].push_back(int2{i, j});
-        for (int c = 0; c < 3; ++c) p->cls_count[c][j] = (int) p->h_pairs_cls[c].size();
+        for (int c = 0; c < kClasses; ++c) p->cls_count[c][j] = (int) p->h_pairs_cls[c].size();
     }
-    for (int c = 0; c < 3; ++c) {
+    for (int c = 0; c < kClasses; ++c) {
         hipFree(p->d_pairs_cls[c]);
         p->d_pairs_cls[c] = nullptr;
         const size_t bytes = sizeof(int2) * std::max<size_t>(1, p->h_pairs_cls[c].size());
@@ -369,6 +392,10 @@ int setup_mixed(egs_problem_s *p) {
             hipMalloc(&p->d_p16l[1], panel * sizeof(uint16_t)) != hipSuccess ||
             hipMalloc(&p->d_pinv[0], sizeof(float) * (size_t) p->npad) != hipSuccess ||
             hipMalloc(&p->d_pinv[1], sizeof(float) * (size_t) p->npad) != hipSuccess ||
+            hipMalloc(&p->d_bmax[0], sizeof(float) * (size_t) (p->npad / 192)) != hipSuccess ||
+            hipMalloc(&p->d_bmax[1], sizeof(float) * (size_t) (p->npad / 192)) != hipSuccess ||
+            hipMalloc(&p->d_pmax, sizeof(unsigned) * (size_t) p->nt) != hipSuccess ||
+            hipMalloc(&p->d_counts, 2 * sizeof(unsigned long long)) != hipSuccess ||
             hipMalloc(&p->d_cg, 5 * sizeof(double) * (size_t) p->npad) != hipSuccess ||
             hipMalloc(&p->d_mv, sizeof(double) * (size_t) egs::matvec_chunks(p->n) * p->npad) != hipSuccess ||
             hipMalloc(&p->d_dot, sizeof(double) * egs::dot_blocks()) != hipSuccess)
@@ -484,7 +511,7 @@ void factor_panel(egs_problem_s *p, int k) {
         prof_end(p, pi);
         if (is_mixed(p)) {
             const bool split = p->mode == EGS_PRECISION_MIXED;
-            if (split) egs::launch_panel_rowscale(m, k, p->d_pinv[k & 1], s);
+            if (split) egs::launch_panel_rowscale(m, k, p->d_pinv[k & 1], p->d_bmax[k & 1], p->d_pmax, s);
             egs::launch_panel_convert(m, k, p->d_p64[k & 1], p->d_p32[k & 1], p->d_p16h[k & 1], p->d_p16l[k & 1],
                                       split ? p->d_pinv[k & 1] : nullptr, s);
         }
@@ -511,8 +538,9 @@ double update_flops_cls(const egs_problem_s *p, int c, int j0, int j1) {
 // each class list.
 void launch_mixed_update(egs_problem_s *p, int k, int jcol_lo, int jcol_hi, hipStream_t st) {
     const egs::TileMap m = p->map();
-    static const int prof_id[3] = {EGS_PROFILE_UPDATE, EGS_PROFILE_UPDATE_F32, EGS_PROFILE_UPDATE_16};
-    for (int c = 0; c < 3; ++c) {
+    static const int prof_id[kClasses] = {EGS_PROFILE_UPDATE, EGS_PROFILE_UPDATE_F32, EGS_PROFILE_UPDATE_16,
+                                          EGS_PROFILE_UPDATE_16_SELECT};
+    for (int c = 0; c < kClasses; ++c) {
         // class-c tiles with column in [jcol_lo, jcol_hi): list entries [count[jcol_hi], count[jcol_lo])
         const int beg = p->cls_count[c][jcol_hi];
         const int end = p->cls_count[c][jcol_lo];
@@ -522,9 +550,13 @@ void launch_mixed_update(egs_problem_s *p, int k, int jcol_lo, int jcol_hi, hipS
         const int slot = k & 1;
         if (c == 0) egs::launch_update192_panel(m, p->d_p64[slot], pr, end - beg, k, st);
         else if (c == 1) egs::launch_update192_f32(m, p->d_p32[slot], pr, end - beg, k, st);
-        else
-            egs::launch_update192_16(m, p->d_p16h[slot], p->d_p16l[slot], p->d_pinv[slot], pr, end - beg, k,
-                                     p->mode == EGS_PRECISION_MIXED, st);
+        else {
+            const egs::Operands op =
+                c == 3 ? (std::isinf(p->tau_single) ? egs::kSingleF16 : egs::kAdaptiveF16)
+                       : (p->mode == EGS_PRECISION_MIXED ? egs::kSplitF16 : egs::kSingleBF16);
+            egs::launch_update192_16(m, p->d_p16h[slot], p->d_p16l[slot], p->d_pinv[slot], p->d_bmax[slot],
+                                     p->d_pmax, (float) p->tau_single, pr, end - beg, k, op, p->d_counts, st);
+        }
         prof_end(p, pi, st);
     }
 }
@@ -553,6 +585,10 @@ int run_potrf(egs_problem_s *p) {
         p->ev_bulk.push_back(e2);
     }
     EGS_HIP(hipMemsetAsync(p->d_info, 0, sizeof(int), s));
+    if (p->mode == EGS_PRECISION_MIXED) {
+        egs::launch_zero_u32(p->d_pmax, nt, s);
+        egs::launch_zero_u32(reinterpret_cast<unsigned *>(p->d_counts), 4, s);
+    }
     factor_panel(p, 0);
     EGS_HIP(hipEventRecord(p->ev_panel[0], s));
     for (int k = 0; k + 1 < nt; ++k) {
@@ -975,6 +1011,16 @@ int egs_problem_set_precision(egs_problem_t p, int mode, int band_fp64, int band
     return is_mixed(p) ? setup_mixed(p) : EGS_SUCCESS;
 }
 
+int egs_problem_set_single_product(egs_problem_t p, int band, double tau) {
+    int st = check_problem(p);
+    if (st) return st;
+    if (std::isnan(tau) || tau < 0.0) return fail(EGS_ERR_INVALID_ARG, "tau must be >= 0 (got %g)", tau);
+    EGS_HIP(hipStreamSynchronize(p->ctx->stream));
+    p->band_single = band < 0 ? -1 : band;
+    p->tau_single = tau;
+    return is_mixed(p) ? setup_mixed(p) : EGS_SUCCESS;
+}
+
 int egs_problem_set_observations(egs_problem_t p, const double *z, int64_t n) {
     int st = check_problem(p);
     if (st) return st;
@@ -1115,6 +1161,19 @@ int egs_last_refinement(egs_problem_t p, int *iterations, double *residual) {
     if (st) return st;
     if (iterations) *iterations = p->refine_iters;
     if (residual) *residual = p->refine_resid;
+    return EGS_SUCCESS;
+}
+
+int egs_last_product_counts(egs_problem_t p, int64_t *split_blocks, int64_t *single_blocks) {
+    int st = check_problem(p);
+    if (st) return st;
+    unsigned long long c[2] = {0, 0};
+    if (p->mode == EGS_PRECISION_MIXED && p->d_counts) {
+        EGS_HIP(hipStreamSynchronize(p->ctx->stream));
+        EGS_HIP(hipMemcpy(c, p->d_counts, sizeof(c), hipMemcpyDeviceToHost));
+    }
+    if (split_blocks) *split_blocks = (int64_t) c[0];
+    if (single_blocks) *single_blocks = (int64_t) c[1];
     return EGS_SUCCESS;
 }
 

@@ -92,17 +92,29 @@ void launch_dot(const double *x, const double *y, int64_t n, double *partial, hi
 // y = a x + b y
 void launch_axpby(double *y, const double *x, double a, double b, int64_t n, hipStream_t s);
 // Mixed-precision factorization pieces (egs_mixed.hip).
-// inv_scale[slot * ts + r] = 2^(e+1) for the largest |x| = f 2^e of row r of panel slot.
-void launch_panel_rowscale(const TileMap &map, int k, float *inv_scale, hipStream_t s);
+// inv_scale[slot * ts + r] = 2^(e+1) for the largest |x| = f 2^e of row r of panel
+// slot; bmax[slot * (ts/192) + b] the largest of 192-row block b; pmax[k] raised
+// to the largest of the panel (bits of a float, must start at zero).
+void launch_panel_rowscale(const TileMap &map, int k, float *inv_scale, float *bmax, unsigned *pmax,
+                           hipStream_t s);
+void launch_zero_u32(unsigned *p, int n, hipStream_t s);
 // Panel k to FP64 and FP32 column-major copies and 16-bit hi/lo row-major parts:
 // FP16 of the row-scaled values when inv_scale is given, BF16 of the values otherwise.
 void launch_panel_convert(const TileMap &map, int k, double *p64, float *p32, uint16_t *p16h, uint16_t *p16l,
                           const float *inv_scale, hipStream_t s);
 void launch_update192_f32(const TileMap &map, const float *p32, const int2 *pairs, int npairs, int k, hipStream_t s);
-// C -= A B^T from the 16-bit parts: split (hi*hi + hi*lo + lo*hi, FP16, row scales
-// undone with inv_scale) or a single BF16 product (split false, inv_scale unused).
+// Operands of the 16-bit update: split FP16 (hi*hi + hi*lo + lo*hi of the
+// row-scaled parts), one FP16 product of the row-scaled hi parts, the choice
+// between the two per 192 x 192 block (one product when bmax(A rows) *
+// bmax(B rows) <= tau * pmax[k]^2), or one BF16 product of unscaled values.
+// FP16 forms undo the row scales with inv_scale.
+enum Operands { kSplitF16 = 0, kSingleF16 = 1, kAdaptiveF16 = 2, kSingleBF16 = 3 };
+// C -= A B^T from the 16-bit parts of panel k.  The single and per-block forms
+// add the number of 192 x 192 blocks they update with the split product to
+// counts[0] and with one product to counts[1].
 void launch_update192_16(const TileMap &map, const uint16_t *p16h, const uint16_t *p16l, const float *inv_scale,
-                         const int2 *pairs, int npairs, int k, bool split, hipStream_t s);
+                         const float *bmax, const unsigned *pmax, float tau, const int2 *pairs, int npairs, int k,
+                         Operands op, unsigned long long *counts, hipStream_t s);
 // FP64 update reading panel k from a contiguous FP64 copy (tile i at p64 + (i-k-1) ts^2).
 void launch_update192_panel(const TileMap &map, const double *p64, const int2 *pairs, int npairs, int k,
                             hipStream_t s);

@@ -39,32 +39,61 @@ typedef float f32x4v __attribute__((ext_vector_type(4)));
 // ------------------------------------------------------------------------
 // Row scales of panel k: inv_scale[slot * ts + r] = 2^(e+1) for the largest
 // |x| = f 2^e (1 <= f < 2) of row r of tile (k+1+slot, k), 1 for a zero row.
-// One workgroup per 64 rows; four column groups reduced through LDS.
+// Also the largest scale of each 192-row block, bmax[slot * (ts/192) + b],
+// and of the whole panel, pmax[k] (zero rows count 0; pmax is the bits of a
+// non-negative float, raised with atomicMax, and must start at zero).  One
+// workgroup per 192 rows; four column groups reduced through LDS.
 // ------------------------------------------------------------------------
-__global__ void __launch_bounds__(kThreads) panel_rowscale_kernel(TileMap map, int k, float *inv_scale) {
-    __shared__ float sm[4][kBlk];
+__global__ void __launch_bounds__(kThreads) panel_rowscale_kernel(TileMap map, int k, float *inv_scale, float *bmax,
+                                                                  unsigned *pmax) {
+    __shared__ float sm[4][kBM];
+    __shared__ float sw[kThreads / 64];
     const int ts = map.ts;
-    const int nsb = ts / kBlk;
-    const int slot = blockIdx.x / nsb;
-    const int rb = blockIdx.x % nsb;
+    const int nb = ts / kBM;
+    const int slot = blockIdx.x / nb;
+    const int b = blockIdx.x % nb;
     const int i = k + 1 + slot;
     const int t = threadIdx.x;
     const int r = t & 63, g = t >> 6;
-    const int64_t row = (int64_t) rb * kBlk + r;
-    float m = 0.0f;
-    if (map.store[i * map.nt + k] == kStoreF32) {
-        const float *A = tile_f32(map, i, k) + row;
-        for (int c = g; c < ts; c += 4) m = fmaxf(m, fabsf(A[(int64_t) c * ts]));
-    } else {
-        const double *A = tile_f64(map, i, k) + row;
-        for (int c = g; c < ts; c += 4) m = fmaxf(m, fabsf((float) A[(int64_t) c * ts]));
+    const bool f32 = map.store[i * map.nt + k] == kStoreF32;
+#pragma unroll
+    for (int rs = 0; rs < kBM / 64; ++rs) {
+        const int64_t row = (int64_t) b * kBM + rs * 64 + r;
+        float m = 0.0f;
+        if (f32) {
+            const float *A = tile_f32(map, i, k) + row;
+            for (int c = g; c < ts; c += 4) m = fmaxf(m, fabsf(A[(int64_t) c * ts]));
+        } else {
+            const double *A = tile_f64(map, i, k) + row;
+            for (int c = g; c < ts; c += 4) m = fmaxf(m, fabsf((float) A[(int64_t) c * ts]));
+        }
+        sm[g][rs * 64 + r] = m;
     }
-    sm[g][r] = m;
     __syncthreads();
-    if (t < kBlk) {
+    float mag = 0.0f;
+    if (t < kBM) {
         const float mx = fmaxf(fmaxf(sm[0][t], sm[1][t]), fmaxf(sm[2][t], sm[3][t]));
-        inv_scale[(int64_t) slot * ts + row - r + t] = (mx > 0.0f && mx < 3.0e38f) ? ldexpf(1.0f, ilogbf(mx) + 1) : 1.0f;
+        const bool finite = mx > 0.0f && mx < 3.0e38f;
+        const float sc = finite ? ldexpf(1.0f, ilogbf(mx) + 1) : 1.0f;
+        inv_scale[(int64_t) slot * ts + b * kBM + t] = sc;
+        mag = mx > 0.0f ? sc : 0.0f;
     }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) mag = fmaxf(mag, __shfl_xor(mag, off));
+    if (r == 0) sw[g] = mag;
+    __syncthreads();
+    if (t == 0) {
+        float bm = sw[0];
+#pragma unroll
+        for (int q = 1; q < kThreads / 64; ++q) bm = fmaxf(bm, sw[q]);
+        bmax[slot * nb + b] = bm;
+        atomicMax(pmax + k, __float_as_uint(bm));
+    }
+}
+
+__global__ void zero_u32_kernel(unsigned *p, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0u;
 }
 
 // ------------------------------------------------------------------------
@@ -259,12 +288,17 @@ __device__ __forceinline__ f32x16 mfma_32(const u16x8 &x, const u16x8 &y, f32x16
                                                        acc, 0, 0, 0);
 }
 
-template <typename T16, bool kSplit, int kDepth>
+// kMode: 0 split product, 1 single product, 2 chosen per block: single when
+// bmax(A rows) * bmax(B rows) <= tau * pmax(panel k)^2, split otherwise.
+enum { kModeSplit = 0, kModeSingle = 1, kModeAdaptive = 2 };
+
+template <typename T16, int kMode, int kDepth>
 __global__ void __launch_bounds__(kNTw, 1) update192_w_kernel(TileMap map, const uint16_t *p16h, const uint16_t *p16l,
-                                                              const float *inv_scale, const int2 *pairs, int k) {
-    constexpr int kParts = kSplit ? 2 : 1;
-    __shared__ u16x8 sA[2][kParts][kPlanesW][kPlaneW];
-    __shared__ u16x8 sB[2][kParts][kPlanesW][kPlaneW];
+                                                              const float *inv_scale, const float *bmax,
+                                                              const unsigned *pmax, float tau, const int2 *pairs,
+                                                              int k, unsigned long long *counts) {
+    __shared__ u16x8 sA[2][kMode == kModeSingle ? 1 : 2][kPlanesW][kPlaneW];
+    __shared__ u16x8 sB[2][kMode == kModeSingle ? 1 : 2][kPlanesW][kPlaneW];
     const int ts = map.ts;
     const int nsb = ts / kBM;
     const int nsub = nsb * nsb;
@@ -298,72 +332,97 @@ __global__ void __launch_bounds__(kNTw, 1) update192_w_kernel(TileMap map, const
 #pragma unroll
             for (int v = 0; v < 16; ++v) acc[mi][ni][v] = 0.f;
 
-    u16x8 rg[kDepth][kPerThreadW][2 * kParts];
-    auto issue = [&](int set, int chunk) {
-        const uint32_t kofs = (uint32_t) (chunk * kKCw);
-#pragma unroll
-        for (int q = 0; q < kPerThreadW; ++q)
-#pragma unroll
-            for (int p = 0; p < kParts; ++p) {
-                rg[set][q][p] = *reinterpret_cast<const u16x8 *>(src[p] + goff[q] + kofs);
-                rg[set][q][kParts + p] = *reinterpret_cast<const u16x8 *>(src[2 + p] + goff[q] + kofs);
-            }
-    };
-    auto stage = [&](int set, int buf) {
-#pragma unroll
-        for (int q = 0; q < kPerThreadW; ++q) {
-            const int e = t + kNTw * q;
-#pragma unroll
-            for (int p = 0; p < kParts; ++p) {
-                sA[buf][p][e & 3][e >> 2] = rg[set][q][p];
-                sB[buf][p][e & 3][e >> 2] = rg[set][q][kParts + p];
-            }
-        }
-    };
     const int nchunks = ts / kKCw;
+    // The K loop with P = 2 (hi and lo parts, three products) or P = 1 (hi
+    // parts, one product); part p of a chunk sits in LDS plane set p.
+    auto run = [&](auto split_tag) {
+        constexpr bool kSplit = decltype(split_tag)::value;
+        constexpr int P = kSplit ? 2 : 1;
+        u16x8 rg[kDepth][kPerThreadW][2 * P];
+        auto issue = [&](int set, int chunk) {
+            const uint32_t kofs = (uint32_t) (chunk * kKCw);
 #pragma unroll
-    for (int u = 0; u < kDepth; ++u)
-        if (u < nchunks) issue(u, u);
-    stage(0, 0);
-    __syncthreads();
-    for (int c0 = 0; c0 < nchunks; c0 += kDepth) {
+            for (int q = 0; q < kPerThreadW; ++q)
 #pragma unroll
-        for (int u = 0; u < kDepth; ++u) {
-            const int c = c0 + u;
-            const int buf = c & 1;
-            if (c + kDepth < nchunks) issue(u, c + kDepth);
+                for (int p = 0; p < P; ++p) {
+                    rg[set][q][p] = *reinterpret_cast<const u16x8 *>(src[p] + goff[q] + kofs);
+                    rg[set][q][P + p] = *reinterpret_cast<const u16x8 *>(src[2 + p] + goff[q] + kofs);
+                }
+        };
+        auto stage = [&](int set, int buf) {
 #pragma unroll
-            for (int ks = 0; ks < kKCw / 16; ++ks) {
-                const int pl = 2 * ks + lk;
-                u16x8 ah[3], bh[3];
+            for (int q = 0; q < kPerThreadW; ++q) {
+                const int e = t + kNTw * q;
 #pragma unroll
-                for (int mi = 0; mi < 3; ++mi) ah[mi] = sA[buf][0][pl][wm * 96 + mi * 32 + lr];
-#pragma unroll
-                for (int ni = 0; ni < 3; ++ni) bh[ni] = sB[buf][0][pl][wn * 96 + ni * 32 + lr];
-                if (kSplit) {
-                    u16x8 al[3], bl[3];
-#pragma unroll
-                    for (int mi = 0; mi < 3; ++mi) al[mi] = sA[buf][kParts - 1][pl][wm * 96 + mi * 32 + lr];
-#pragma unroll
-                    for (int ni = 0; ni < 3; ++ni) bl[ni] = sB[buf][kParts - 1][pl][wn * 96 + ni * 32 + lr];
-#pragma unroll
-                    for (int mi = 0; mi < 3; ++mi)
-#pragma unroll
-                        for (int ni = 0; ni < 3; ++ni) {
-                            acc[mi][ni] = mfma_32<T16>(bh[ni], ah[mi], acc[mi][ni]);
-                            acc[mi][ni] = mfma_32<T16>(bh[ni], al[mi], acc[mi][ni]);
-                            acc[mi][ni] = mfma_32<T16>(bl[ni], ah[mi], acc[mi][ni]);
-                        }
-                } else {
-#pragma unroll
-                    for (int mi = 0; mi < 3; ++mi)
-#pragma unroll
-                        for (int ni = 0; ni < 3; ++ni) acc[mi][ni] = mfma_32<T16>(bh[ni], ah[mi], acc[mi][ni]);
+                for (int p = 0; p < P; ++p) {
+                    sA[buf][p][e & 3][e >> 2] = rg[set][q][p];
+                    sB[buf][p][e & 3][e >> 2] = rg[set][q][P + p];
                 }
             }
-            if (c + 1 < nchunks) stage((u + 1) % kDepth, buf ^ 1);
-            __syncthreads();
+        };
+#pragma unroll
+        for (int u = 0; u < kDepth; ++u)
+            if (u < nchunks) issue(u, u);
+        stage(0, 0);
+        __syncthreads();
+        for (int c0 = 0; c0 < nchunks; c0 += kDepth) {
+#pragma unroll
+            for (int u = 0; u < kDepth; ++u) {
+                const int c = c0 + u;
+                const int buf = c & 1;
+                if (c + kDepth < nchunks) issue(u, c + kDepth);
+#pragma unroll
+                for (int ks = 0; ks < kKCw / 16; ++ks) {
+                    const int pl = 2 * ks + lk;
+                    u16x8 ah[3], bh[3];
+#pragma unroll
+                    for (int mi = 0; mi < 3; ++mi) ah[mi] = sA[buf][0][pl][wm * 96 + mi * 32 + lr];
+#pragma unroll
+                    for (int ni = 0; ni < 3; ++ni) bh[ni] = sB[buf][0][pl][wn * 96 + ni * 32 + lr];
+                    if constexpr (kSplit) {
+                        u16x8 al[3], bl[3];
+#pragma unroll
+                        for (int mi = 0; mi < 3; ++mi) al[mi] = sA[buf][P - 1][pl][wm * 96 + mi * 32 + lr];
+#pragma unroll
+                        for (int ni = 0; ni < 3; ++ni) bl[ni] = sB[buf][P - 1][pl][wn * 96 + ni * 32 + lr];
+#pragma unroll
+                        for (int mi = 0; mi < 3; ++mi)
+#pragma unroll
+                            for (int ni = 0; ni < 3; ++ni) {
+                                acc[mi][ni] = mfma_32<T16>(bh[ni], ah[mi], acc[mi][ni]);
+                                acc[mi][ni] = mfma_32<T16>(bh[ni], al[mi], acc[mi][ni]);
+                                acc[mi][ni] = mfma_32<T16>(bl[ni], ah[mi], acc[mi][ni]);
+                            }
+                    } else {
+#pragma unroll
+                        for (int mi = 0; mi < 3; ++mi)
+#pragma unroll
+                            for (int ni = 0; ni < 3; ++ni) acc[mi][ni] = mfma_32<T16>(bh[ni], ah[mi], acc[mi][ni]);
+                    }
+                }
+                if (c + 1 < nchunks) stage((u + 1) % kDepth, buf ^ 1);
+                __syncthreads();
+            }
         }
+    };
+    bool single = kMode == kModeSingle;
+    if constexpr (kMode == kModeAdaptive) {
+        // uniform over the workgroup
+        const float a = bmax[(ij.x - k - 1) * nsb + sbm];
+        const float b = bmax[(ij.y - k - 1) * nsb + sbn];
+        const float pm = __uint_as_float(pmax[k]);
+        single = a * b <= tau * pm * pm;
+    }
+    if (counts && t == 0) atomicAdd(counts + (single ? 1 : 0), 1ull);
+    if constexpr (kMode == kModeSplit) {
+        run(std::true_type{});
+    } else if constexpr (kMode == kModeSingle) {
+        run(std::false_type{});
+    } else {
+        if (single)
+            run(std::false_type{});
+        else
+            run(std::true_type{});
     }
     // C -= acc (times the two row scales' inverses, exact powers of two): lane
     // row (A side) wm*96 + mi*32 + lr, column (B side) wn*96 + ni*32 + 8 (v / 4)
@@ -406,11 +465,18 @@ __global__ void __launch_bounds__(kNTw, 1) update192_w_kernel(TileMap map, const
 
 }  // namespace
 
-void launch_panel_rowscale(const TileMap &map, int k, float *inv_scale, hipStream_t s) {
+void launch_panel_rowscale(const TileMap &map, int k, float *inv_scale, float *bmax, unsigned *pmax,
+                           hipStream_t s) {
     const int rows = map.nt - k - 1;
     if (rows <= 0) return;
-    const int nsb = map.ts / kBlk;
-    hipLaunchKernelGGL(panel_rowscale_kernel, dim3((unsigned) (rows * nsb)), dim3(kThreads), 0, s, map, k, inv_scale);
+    const int nb = map.ts / kBM;
+    hipLaunchKernelGGL(panel_rowscale_kernel, dim3((unsigned) (rows * nb)), dim3(kThreads), 0, s, map, k, inv_scale,
+                       bmax, pmax);
+}
+
+void launch_zero_u32(unsigned *p, int n, hipStream_t s) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(zero_u32_kernel, dim3((unsigned) ((n + 255) / 256)), dim3(256), 0, s, p, n);
 }
 
 void launch_panel_convert(const TileMap &map, int k, double *p64, float *p32, uint16_t *p16h, uint16_t *p16l,
@@ -430,17 +496,29 @@ void launch_update192_f32(const TileMap &map, const float *p32, const int2 *pair
 }
 
 void launch_update192_16(const TileMap &map, const uint16_t *p16h, const uint16_t *p16l, const float *inv_scale,
-                         const int2 *pairs, int npairs, int k, bool split, hipStream_t s) {
+                         const float *bmax, const unsigned *pmax, float tau, const int2 *pairs, int npairs, int k,
+                         Operands op, unsigned long long *counts, hipStream_t s) {
     if (npairs <= 0) return;
     const int nsb = map.ts / kBM;
     const dim3 grid((unsigned) npairs * (unsigned) (nsb * nsb));
     // two chunks of loads in flight: a third measured no faster
-    if (split)
-        hipLaunchKernelGGL((update192_w_kernel<_Float16, true, 2>), grid, dim3(kNTw), 0, s, map, p16h, p16l,
-                           inv_scale, pairs, k);
-    else
-        hipLaunchKernelGGL((update192_w_kernel<__bf16, false, 2>), grid, dim3(kNTw), 0, s, map, p16h, p16l, nullptr,
-                           pairs, k);
+    switch (op) {
+    case kSplitF16:
+        hipLaunchKernelGGL((update192_w_kernel<_Float16, kModeSplit, 2>), grid, dim3(kNTw), 0, s, map, p16h, p16l,
+                           inv_scale, bmax, pmax, tau, pairs, k, nullptr);
+        break;
+    case kSingleF16:
+        hipLaunchKernelGGL((update192_w_kernel<_Float16, kModeSingle, 2>), grid, dim3(kNTw), 0, s, map, p16h, p16l,
+                           inv_scale, bmax, pmax, tau, pairs, k, counts);
+        break;
+    case kAdaptiveF16:
+        hipLaunchKernelGGL((update192_w_kernel<_Float16, kModeAdaptive, 2>), grid, dim3(kNTw), 0, s, map, p16h,
+                           p16l, inv_scale, bmax, pmax, tau, pairs, k, counts);
+        break;
+    default:
+        hipLaunchKernelGGL((update192_w_kernel<__bf16, kModeSingle, 2>), grid, dim3(kNTw), 0, s, map, p16h, p16l,
+                           nullptr, nullptr, nullptr, 0.0f, pairs, k, nullptr);
+    }
 }
 
 }  // namespace egs

@@ -95,8 +95,25 @@ int egs_problem_device_bytes(egs_problem_t problem, int64_t *bytes);
  * z'Sigma^{-1}z comes from conjugate gradients preconditioned by the factor,
  * with residuals b - Sigma x evaluated in FP64 until ||r|| <= 1e-12 ||b||. */
 int egs_problem_set_precision(egs_problem_t problem, int mode, int band_fp64, int band_fp32);
+/* Under EGS_PRECISION_MIXED, a 192 x 192 block of a tile with tile distance
+ * |i - j| > band (and > band_fp32) is updated with one FP16 product of the
+ * row-scaled values instead of the split three (about 11 significant bits per
+ * operand instead of 22) when the panel rows it reads are small:
+ * smax(its A rows) * smax(its B rows) <= tau * smax(panel)^2, where smax is the
+ * largest power-of-two row scale (2^(e+1) for a largest entry f 2^e).
+ * tau = INFINITY takes every such block, tau = 0 none but all-zero ones;
+ * band < 0 keeps the split product everywhere.  Default: band 0, tau = 2^-16,
+ * which keeps a single-product block's error (about 2^-12 K smax(A) smax(B))
+ * 16x under the split product's for the panel's largest block
+ * (2^-24 K smax^2).
+ * Kept across egs_problem_set_precision calls. */
+int egs_problem_set_single_product(egs_problem_t problem, int band, double tau);
 /* Iterations and final relative FP64 residual of the last refinement. */
 int egs_last_refinement(egs_problem_t problem, int *iterations, double *residual);
+/* Block updates (192 x 192) of the tiles beyond the single-product band in the
+ * last factorization under EGS_PRECISION_MIXED: those done with the split FP16
+ * product and those done with one FP16 product (0 and 0 otherwise). */
+int egs_last_product_counts(egs_problem_t problem, int64_t *split_blocks, int64_t *single_blocks);
 
 /* Observations Z (length n_loc * P): the measurements descriptor Z and its
  * restore copy Z_COPY of ChameleonImplementation.cpp:113-125. */
@@ -146,7 +163,8 @@ int egs_last_timings(egs_problem_t problem, double *t_covariance, double *t_fact
 #define EGS_PROFILE_UPDATE_F32 5    /* mixed plans: FP32-band update */
 #define EGS_PROFILE_UPDATE_16 6      /* mixed plans: update with 16-bit products */
 #define EGS_PROFILE_REFINE 7        /* mixed plans: refinement of z'Sigma^{-1}z */
-#define EGS_PROFILE_KERNELS 8
+#define EGS_PROFILE_UPDATE_16_SELECT 8  /* mixed plans: update of the tiles beyond the single-product band */
+#define EGS_PROFILE_KERNELS 9
 int egs_problem_profile(egs_problem_t problem, int enable);
 int egs_problem_kernel_stats(egs_problem_t problem, int kernel, int64_t *launches, double *total_ms, double *flops);
 

@@ -149,3 +149,36 @@ def test_data_modeling_fp64_and_mixed_theta_hat(ctx):
     assert np.max(np.abs(thm - ref)) < 1e-3
     assert abs(llhm - g["loglik"]) < 1e-6 * abs(g["loglik"])
     pm.close()
+
+
+def test_single_product_selection_pins_both_ends(ctx, data4k):
+    """The per-block choice between the split and the single FP16 product:
+    tau = 0 picks the single product only for all-zero blocks, so the result
+    equals the split-only plan bit for bit; tau = infinity picks it for every
+    block beyond the band, so the result equals what the block counts say (all
+    single) and differs from the split; the default tau = 2^-16 keeps the split
+    plan's accuracy."""
+    x, y, th, z, ref = data4k
+    p = egs.Problem(ctx, 0, x, y, dts=192)
+    p.set_observations(z)
+    p.set_precision(egs.PRECISION_MIXED, 1, 3)
+    p.set_single_product(-1)
+    split_only = p.loglik(th)
+    assert p.product_counts() == {"split": 0, "single": 0}
+    p.set_single_product(0, 0.0)
+    tau0 = p.loglik(th)
+    c0 = p.product_counts()
+    p.set_single_product(0, float("inf"))
+    tau_inf = p.loglik(th)
+    cinf = p.product_counts()
+    p.set_single_product(0)
+    dflt = p.loglik(th)
+    cd = p.product_counts()
+    log("  split-only %r, tau 0 %r %s, tau inf %r %s, default %r %s" % (split_only[0], tau0[0], c0, tau_inf[0], cinf,
+                                                                          dflt[0], cd))
+    assert tau0 == split_only
+    assert c0["split"] > 0 and c0["split"] + c0["single"] == cinf["single"] and cinf["split"] == 0
+    assert cd["split"] + cd["single"] == cinf["single"] and cd["single"] > 0 and cd["split"] > 0
+    assert tau_inf[1] != split_only[1]
+    assert abs(dflt[0] - ref["llh"]) <= 1e-6 * abs(ref["llh"])
+    p.close()
