@@ -319,8 +319,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--mle", action="store_true",
                     help="fit theta with the FP64 and the mixed plan on the same data and report both estimates "
-                         "(defaults to n=40000, theta=(1, 0.05, 0.5), ftol 1e-7)")
-    ap.add_argument("--mle-ftol", type=float, default=1e-7)
+                         "(defaults to n=40000, theta=(1, 0.05, 0.5), ftol 1e-9, tighter than the reference's "
+                         "default 1e-8)")
+    ap.add_argument("--mle-ftol", type=float, default=1e-9)
     ap.add_argument("--bivariate", action="store_true",
                     help="the bivariate great-circle workload (25,000 locations on the sphere) instead of the default")
     ap.add_argument("--theta", type=float, nargs=3, default=None)
