@@ -314,7 +314,7 @@ def main():
     ap.add_argument("--single-tau", type=float, default=2.0 ** -16,
                     help="... when their panel rows' scales multiply to <= tau times the panel's largest squared")
     ap.add_argument("--plain-bf16", action="store_true", help="single BF16 product instead of the split one")
-    ap.add_argument("--cpu-sample", type=int, default=14400,
+    ap.add_argument("--cpu-sample", type=int, default=18240,
                     help="n of the oracle sample (about 10-20 s on 16 host cores)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--mle", action="store_true",
