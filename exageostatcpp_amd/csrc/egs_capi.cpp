@@ -77,6 +77,9 @@ struct Profiler {
 
 // Update classes of a mixed plan (see tile_class).
 constexpr int kClasses = 4;
+// Panel copies in flight: panel k is read by the bulk update of step k while
+// the critical stream writes panel k+1 and then panel k+2.
+constexpr int kPanelSlots = 3;
 
 struct egs_problem_s {
     egs_context_s *ctx = nullptr;
@@ -95,14 +98,14 @@ struct egs_problem_s {
     int2 *d_pairs_cls[kClasses] = {nullptr, nullptr, nullptr, nullptr};
     std::vector<int2> h_pairs_cls[kClasses];
     std::vector<int> cls_count[kClasses];
-    // panel k as FP64 column-major, FP32 column-major and FP32 row-major, in
-    // slot k % 2: with the lookahead the critical stream writes panel k+1 while
-    // the bulk stream still reads panel k
-    double *d_p64[2] = {nullptr, nullptr};
-    float *d_p32[2] = {nullptr, nullptr};
-    uint16_t *d_p16h[2] = {nullptr, nullptr}, *d_p16l[2] = {nullptr, nullptr};   // FP16 or BF16 bits
-    float *d_pinv[2] = {nullptr, nullptr};   // row scales of the FP16 parts (inverse, powers of two)
-    float *d_bmax[2] = {nullptr, nullptr};   // largest row scale per 192-row block of a panel
+    // panel k as FP64 column-major, FP32 column-major and 16-bit row-major, in
+    // slot k % kPanelSlots: the critical stream writes panel k+2 while the bulk
+    // stream may still read panel k (see run_potrf)
+    double *d_p64[kPanelSlots] = {};
+    float *d_p32[kPanelSlots] = {};
+    uint16_t *d_p16h[kPanelSlots] = {}, *d_p16l[kPanelSlots] = {};   // FP16 or BF16 bits
+    float *d_pinv[kPanelSlots] = {};   // row scales of the FP16 parts (inverse, powers of two)
+    float *d_bmax[kPanelSlots] = {};   // largest row scale per 192-row block of a panel
     unsigned *d_pmax = nullptr;              // largest row scale per panel (float bits), nt entries
     unsigned long long *d_counts = nullptr;  // split / single-product block updates of the last factorization
     double *d_cg = nullptr;                  // refinement vectors x, r, p, q, w (npad each)
@@ -252,7 +255,7 @@ void free_problem_buffers(egs_problem_s *p) {
         hipFree(q);
         q = nullptr;
     }
-    for (int b = 0; b < 2; ++b) {
+    for (int b = 0; b < kPanelSlots; ++b) {
         hipFree(p->d_p64[b]);
         hipFree(p->d_p32[b]);
         hipFree(p->d_p16h[b]);
@@ -381,20 +384,18 @@ int setup_mixed(egs_problem_s *p) {
                               hipMemcpyHostToDevice));
     }
     const size_t panel = (size_t) p->npad * p->ts;
-    if (!p->d_p64[0]) {
-        if (hipMalloc(&p->d_p64[0], panel * sizeof(double)) != hipSuccess ||
-            hipMalloc(&p->d_p64[1], panel * sizeof(double)) != hipSuccess ||
-            hipMalloc(&p->d_p32[0], panel * sizeof(float)) != hipSuccess ||
-            hipMalloc(&p->d_p32[1], panel * sizeof(float)) != hipSuccess ||
-            hipMalloc(&p->d_p16h[0], panel * sizeof(uint16_t)) != hipSuccess ||
-            hipMalloc(&p->d_p16h[1], panel * sizeof(uint16_t)) != hipSuccess ||
-            hipMalloc(&p->d_p16l[0], panel * sizeof(uint16_t)) != hipSuccess ||
-            hipMalloc(&p->d_p16l[1], panel * sizeof(uint16_t)) != hipSuccess ||
-            hipMalloc(&p->d_pinv[0], sizeof(float) * (size_t) p->npad) != hipSuccess ||
-            hipMalloc(&p->d_pinv[1], sizeof(float) * (size_t) p->npad) != hipSuccess ||
-            hipMalloc(&p->d_bmax[0], sizeof(float) * (size_t) (p->npad / 192)) != hipSuccess ||
-            hipMalloc(&p->d_bmax[1], sizeof(float) * (size_t) (p->npad / 192)) != hipSuccess ||
-            hipMalloc(&p->d_pmax, sizeof(unsigned) * (size_t) p->nt) != hipSuccess ||
+    bool slots_ok = true;
+    const bool need_slots = !p->d_p64[0];
+    for (int b = 0; b < kPanelSlots && need_slots; ++b)
+        slots_ok = slots_ok && hipMalloc(&p->d_p64[b], panel * sizeof(double)) == hipSuccess &&
+                   hipMalloc(&p->d_p32[b], panel * sizeof(float)) == hipSuccess &&
+                   hipMalloc(&p->d_p16h[b], panel * sizeof(uint16_t)) == hipSuccess &&
+                   hipMalloc(&p->d_p16l[b], panel * sizeof(uint16_t)) == hipSuccess &&
+                   hipMalloc(&p->d_pinv[b], sizeof(float) * (size_t) p->npad) == hipSuccess &&
+                   hipMalloc(&p->d_bmax[b], sizeof(float) * (size_t) (p->npad / 192)) == hipSuccess;
+    if (!slots_ok) return fail(EGS_ERR_OUT_OF_MEMORY, "cannot allocate the mixed-precision panel copies");
+    if (!p->d_cg) {
+        if (hipMalloc(&p->d_pmax, sizeof(unsigned) * (size_t) p->nt) != hipSuccess ||
             hipMalloc(&p->d_counts, 2 * sizeof(unsigned long long)) != hipSuccess ||
             hipMalloc(&p->d_cg, 5 * sizeof(double) * (size_t) p->npad) != hipSuccess ||
             hipMalloc(&p->d_mv, sizeof(double) * (size_t) egs::matvec_chunks(p->n) * p->npad) != hipSuccess ||
@@ -511,9 +512,10 @@ void factor_panel(egs_problem_s *p, int k) {
         prof_end(p, pi);
         if (is_mixed(p)) {
             const bool split = p->mode == EGS_PRECISION_MIXED;
-            if (split) egs::launch_panel_rowscale(m, k, p->d_pinv[k & 1], p->d_bmax[k & 1], p->d_pmax, s);
-            egs::launch_panel_convert(m, k, p->d_p64[k & 1], p->d_p32[k & 1], p->d_p16h[k & 1], p->d_p16l[k & 1],
-                                      split ? p->d_pinv[k & 1] : nullptr, s);
+            const int sl = k % kPanelSlots;
+            if (split) egs::launch_panel_rowscale(m, k, p->d_pinv[sl], p->d_bmax[sl], p->d_pmax, s);
+            egs::launch_panel_convert(m, k, p->d_p64[sl], p->d_p32[sl], p->d_p16h[sl], p->d_p16l[sl],
+                                      split ? p->d_pinv[sl] : nullptr, s);
         }
     }
 }
@@ -547,7 +549,7 @@ void launch_mixed_update(egs_problem_s *p, int k, int jcol_lo, int jcol_hi, hipS
         if (end <= beg) continue;
         const int pi = prof_begin(p, prof_id[c], update_flops_cls(p, c, jcol_lo, jcol_hi), st);
         const int2 *pr = p->d_pairs_cls[c] + beg;
-        const int slot = k & 1;
+        const int slot = k % kPanelSlots;
         if (c == 0) egs::launch_update192_panel(m, p->d_p64[slot], pr, end - beg, k, st);
         else if (c == 1) egs::launch_update192_f32(m, p->d_p32[slot], pr, end - beg, k, st);
         else {
@@ -595,15 +597,26 @@ int run_potrf(egs_problem_s *p) {
         const int beyond = nt - k - 2;                 // tile columns k+2 .. nt-1
         const int n_beyond = beyond * (beyond + 1) / 2;
         const int n_next = nt - k - 1;                 // tiles (i, k+1), i >= k+1
+        // Bulk step k: column k+2 first, then ev_bulk[k], then columns k+3 ..:
+        // the critical stream's step k+1 needs only column k+2, so it overlaps
+        // the rest of this step.  (The list holds columns in descending order:
+        // column k+2 is the last beyond - 1 ... entries of the prefix.)
         EGS_HIP(hipStreamWaitEvent(b, p->ev_panel[k], 0));
         if (is_mixed(p)) {
-            if (k + 2 < nt) launch_mixed_update(p, k, k + 2, nt, b);
+            if (k + 2 < nt) launch_mixed_update(p, k, k + 2, k + 3, b);
         } else if (n_beyond > 0) {
-            const int pi = prof_begin(p, EGS_PROFILE_UPDATE, update_flops(p, k + 2, nt), b);
-            egs::launch_update(m, p->d_pairs, n_beyond, k, b);
+            const int pi = prof_begin(p, EGS_PROFILE_UPDATE, update_flops(p, k + 2, k + 3), b);
+            egs::launch_update(m, p->d_pairs + (n_beyond - beyond), beyond, k, b);
             prof_end(p, pi, b);
         }
         EGS_HIP(hipEventRecord(p->ev_bulk[k], b));
+        if (is_mixed(p)) {
+            if (k + 3 < nt) launch_mixed_update(p, k, k + 3, nt, b);
+        } else if (n_beyond > beyond) {
+            const int pi = prof_begin(p, EGS_PROFILE_UPDATE, update_flops(p, k + 3, nt), b);
+            egs::launch_update(m, p->d_pairs, n_beyond - beyond, k, b);
+            prof_end(p, pi, b);
+        }
         if (k > 0) EGS_HIP(hipStreamWaitEvent(s, p->ev_bulk[k - 1], 0));
         if (is_mixed(p)) {
             launch_mixed_update(p, k, k + 1, k + 2, s);
@@ -615,7 +628,9 @@ int run_potrf(egs_problem_s *p) {
         factor_panel(p, k + 1);
         EGS_HIP(hipEventRecord(p->ev_panel[k + 1], s));
     }
-    if (nt >= 2) EGS_HIP(hipStreamWaitEvent(s, p->ev_bulk[nt - 2], 0));
+    // ev_bulk[nt - 1] (unused by the steps) marks the end of the bulk stream
+    EGS_HIP(hipEventRecord(p->ev_bulk[nt - 1], b));
+    EGS_HIP(hipStreamWaitEvent(s, p->ev_bulk[nt - 1], 0));
     if (is_mixed(p)) egs::launch_tinv(m, p->d_winv, p->d_tinv, s);
     EGS_HIP(hipGetLastError());
     return EGS_SUCCESS;

@@ -265,8 +265,11 @@ __global__ void __launch_bounds__(kNT, 1) update192_f32_kernel(TileMap map, cons
 // accumulators per lane).  Per 16-deep K step a wave reads 12 operand
 // fragments for 27 MFMAs.  Output slot i of lane l holds row (l % 32) of the A
 // operand and column 8 (i / 4) + 4 (l / 32) + i % 4 of the B operand (measured
-// on gfx950).  T16 = _Float16 with row scales (the default split), or __bf16
-// without (the plain-BF16 option, kSplit false).
+// on gfx950).  T16 = _Float16 with row scales, or __bf16 without (the
+// plain-BF16 option, single product).  The single-product and per-block
+// kernels fit 256 registers and 50 KB of LDS, so two workgroups share a CU
+// (measured 1.5x faster than one for the single product); the split-only
+// kernel double-buffers hi and lo (100 KB) and runs one per CU.
 // ------------------------------------------------------------------------
 constexpr int kNTw = 256;
 constexpr int kKCw = 32;                  // K per chunk: two MFMA steps
@@ -293,12 +296,15 @@ __device__ __forceinline__ f32x16 mfma_32(const u16x8 &x, const u16x8 &y, f32x16
 enum { kModeSplit = 0, kModeSingle = 1, kModeAdaptive = 2 };
 
 template <typename T16, int kMode, int kDepth>
-__global__ void __launch_bounds__(kNTw, 1) update192_w_kernel(TileMap map, const uint16_t *p16h, const uint16_t *p16l,
+__global__ void __launch_bounds__(kNTw, kMode == kModeSplit ? 1 : 2) update192_w_kernel(TileMap map, const uint16_t *p16h, const uint16_t *p16l,
                                                               const float *inv_scale, const float *bmax,
                                                               const unsigned *pmax, float tau, const int2 *pairs,
                                                               int k, unsigned long long *counts) {
-    __shared__ u16x8 sA[2][kMode == kModeSingle ? 1 : 2][kPlanesW][kPlaneW];
-    __shared__ u16x8 sB[2][kMode == kModeSingle ? 1 : 2][kPlanesW][kPlaneW];
+    // [buffer][part]: the split kernel double-buffers hi and lo; the others
+    // double-buffer one part, and the per-block kernel's split path uses the
+    // two buffers for hi and lo, single-buffered (50 KB: two workgroups per CU)
+    __shared__ u16x8 sA[2][kMode == kModeSplit ? 2 : 1][kPlanesW][kPlaneW];
+    __shared__ u16x8 sB[2][kMode == kModeSplit ? 2 : 1][kPlanesW][kPlaneW];
     const int ts = map.ts;
     const int nsb = ts / kBM;
     const int nsub = nsb * nsb;
@@ -380,19 +386,19 @@ __global__ void __launch_bounds__(kNTw, 1) update192_w_kernel(TileMap map, const
 #pragma unroll
                     for (int ni = 0; ni < 3; ++ni) bh[ni] = sB[buf][0][pl][wn * 96 + ni * 32 + lr];
                     if constexpr (kSplit) {
-                        u16x8 al[3], bl[3];
+                        u16x8 al[3];
 #pragma unroll
                         for (int mi = 0; mi < 3; ++mi) al[mi] = sA[buf][P - 1][pl][wm * 96 + mi * 32 + lr];
 #pragma unroll
-                        for (int ni = 0; ni < 3; ++ni) bl[ni] = sB[buf][P - 1][pl][wn * 96 + ni * 32 + lr];
+                        for (int ni = 0; ni < 3; ++ni) {
+                            const u16x8 bl = sB[buf][P - 1][pl][wn * 96 + ni * 32 + lr];
 #pragma unroll
-                        for (int mi = 0; mi < 3; ++mi)
-#pragma unroll
-                            for (int ni = 0; ni < 3; ++ni) {
+                            for (int mi = 0; mi < 3; ++mi) {
                                 acc[mi][ni] = mfma_32<T16>(bh[ni], ah[mi], acc[mi][ni]);
                                 acc[mi][ni] = mfma_32<T16>(bh[ni], al[mi], acc[mi][ni]);
-                                acc[mi][ni] = mfma_32<T16>(bl[ni], ah[mi], acc[mi][ni]);
+                                acc[mi][ni] = mfma_32<T16>(bl, ah[mi], acc[mi][ni]);
                             }
+                        }
                     } else {
 #pragma unroll
                         for (int mi = 0; mi < 3; ++mi)
@@ -402,6 +408,58 @@ __global__ void __launch_bounds__(kNTw, 1) update192_w_kernel(TileMap map, const
                 }
                 if (c + 1 < nchunks) stage((u + 1) % kDepth, buf ^ 1);
                 __syncthreads();
+            }
+        }
+    };
+    // Split product with hi and lo in the two LDS buffers, one chunk at a time
+    // (the per-block kernel): loads of chunk c+1 are in flight while chunk c
+    // is multiplied.
+    auto run_split_1buf = [&]() {
+        u16x8 rg[kPerThreadW][4];
+        auto issue = [&](int chunk) {
+            const uint32_t kofs = (uint32_t) (chunk * kKCw);
+#pragma unroll
+            for (int q = 0; q < kPerThreadW; ++q)
+#pragma unroll
+                for (int p = 0; p < 2; ++p) {
+                    rg[q][p] = *reinterpret_cast<const u16x8 *>(src[p] + goff[q] + kofs);
+                    rg[q][2 + p] = *reinterpret_cast<const u16x8 *>(src[2 + p] + goff[q] + kofs);
+                }
+        };
+        issue(0);
+        for (int c = 0; c < nchunks; ++c) {
+            __syncthreads();   // every wave is done with chunk c-1
+#pragma unroll
+            for (int q = 0; q < kPerThreadW; ++q) {
+                const int e = t + kNTw * q;
+#pragma unroll
+                for (int p = 0; p < 2; ++p) {
+                    sA[p][0][e & 3][e >> 2] = rg[q][p];
+                    sB[p][0][e & 3][e >> 2] = rg[q][2 + p];
+                }
+            }
+            if (c + 1 < nchunks) issue(c + 1);
+            __syncthreads();
+#pragma unroll
+            for (int ks = 0; ks < kKCw / 16; ++ks) {
+                const int pl = 2 * ks + lk;
+                u16x8 ah[3], al[3];
+#pragma unroll
+                for (int mi = 0; mi < 3; ++mi) {
+                    ah[mi] = sA[0][0][pl][wm * 96 + mi * 32 + lr];
+                    al[mi] = sA[1][0][pl][wm * 96 + mi * 32 + lr];
+                }
+#pragma unroll
+                for (int ni = 0; ni < 3; ++ni) {
+                    const u16x8 bh = sB[0][0][pl][wn * 96 + ni * 32 + lr];
+                    const u16x8 bl = sB[1][0][pl][wn * 96 + ni * 32 + lr];
+#pragma unroll
+                    for (int mi = 0; mi < 3; ++mi) {
+                        acc[mi][ni] = mfma_32<T16>(bh, ah[mi], acc[mi][ni]);
+                        acc[mi][ni] = mfma_32<T16>(bh, al[mi], acc[mi][ni]);
+                        acc[mi][ni] = mfma_32<T16>(bl, ah[mi], acc[mi][ni]);
+                    }
+                }
             }
         }
     };
@@ -422,7 +480,7 @@ __global__ void __launch_bounds__(kNTw, 1) update192_w_kernel(TileMap map, const
         if (single)
             run(std::false_type{});
         else
-            run(std::true_type{});
+            run_split_1buf();
     }
     // C -= acc (times the two row scales' inverses, exact powers of two): lane
     // row (A side) wm*96 + mi*32 + lr, column (B side) wn*96 + ni*32 + 8 (v / 4)
@@ -430,36 +488,30 @@ __global__ void __launch_bounds__(kNTw, 1) update192_w_kernel(TileMap map, const
     float *C = tile_f32(map, ij.x, ij.y);
     const uint32_t cbase = (uint32_t) ((sbm * kBM + wm * 96 + lr) + (sbn * kBM + wn * 96 + 4 * lk) * ts);
     constexpr bool kScaled = std::is_same<T16, _Float16>::value;
-    float sa[3] = {1.0f, 1.0f, 1.0f};
-    f32x4v sb[3][4];
-    if constexpr (kScaled) {
-        const float *invA = inv_scale + (int64_t) (ij.x - k - 1) * ts + sbm * kBM + wm * 96 + lr;
-        const float *invB = inv_scale + (int64_t) (ij.y - k - 1) * ts + sbn * kBM + wn * 96 + 4 * lk;
-#pragma unroll
-        for (int mi = 0; mi < 3; ++mi) sa[mi] = invA[mi * 32];
-#pragma unroll
-        for (int ni = 0; ni < 3; ++ni)
-#pragma unroll
-            for (int g = 0; g < 4; ++g) sb[ni][g] = *reinterpret_cast<const f32x4v *>(invB + ni * 32 + 8 * g);
-    }
+    const float *invA = kScaled ? inv_scale + (int64_t) (ij.x - k - 1) * ts + sbm * kBM + wm * 96 + lr : nullptr;
+    const float *invB = kScaled ? inv_scale + (int64_t) (ij.y - k - 1) * ts + sbn * kBM + wn * 96 + 4 * lk : nullptr;
 #pragma unroll
     for (int mi = 0; mi < 3; ++mi) {
-        float cv[3][16];
+        const float sa = kScaled ? invA[mi * 32] : 1.0f;
 #pragma unroll
-        for (int ni = 0; ni < 3; ++ni)
+        for (int ni = 0; ni < 3; ++ni) {
+            float cv[16];
+            f32x4v sb[4];
+#pragma unroll
+            for (int g = 0; g < 4; ++g)
+                sb[g] = kScaled ? *reinterpret_cast<const f32x4v *>(invB + ni * 32 + 8 * g) : f32x4v{1.f, 1.f, 1.f, 1.f};
 #pragma unroll
             for (int v = 0; v < 16; ++v)
-                cv[ni][v] = __builtin_nontemporal_load(
+                cv[v] = __builtin_nontemporal_load(
                     &C[cbase + (uint32_t) (mi * 32 + (ni * 32 + 8 * (v >> 2) + (v & 3)) * ts)]);
-#pragma unroll
-        for (int ni = 0; ni < 3; ++ni)
 #pragma unroll
             for (int v = 0; v < 16; ++v) {
                 float a = acc[mi][ni][v];
-                if constexpr (kScaled) a *= sa[mi] * sb[ni][v >> 2][v & 3];
-                __builtin_nontemporal_store(cv[ni][v] - a,
+                if constexpr (kScaled) a *= sa * sb[v >> 2][v & 3];
+                __builtin_nontemporal_store(cv[v] - a,
                                             &C[cbase + (uint32_t) (mi * 32 + (ni * 32 + 8 * (v >> 2) + (v & 3)) * ts)]);
             }
+        }
     }
 }
 
