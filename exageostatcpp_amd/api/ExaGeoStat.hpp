@@ -68,9 +68,10 @@ public:
     // Tile-distance bands of the mixed plan: FP64 up to DenseBandDP, FP32 up
     // to BandDenseSP, 16-bit products beyond (row-scaled split FP16, one FP16
     // product for blocks with small panel rows; keys "banddensedp"/"banddensesp").
-    // Defaults 2 and 2: no FP32 band, which measured both faster and at least as
-    // accurate as an FP32 band at n = 100k; FP64 band 2 keeps the 40k fit's
-    // estimate within 1e-3 of the FP64 one, where band 1 did not.
+    // Defaults 4 and 4: no FP32 band, which measured both faster and at least as
+    // accurate as an FP32 band at n = 100k.  FP64 storage up to band 4 keeps
+    // the 40k fit's estimate within 1e-4 of the FP64 one (band 1: 1.7e-3);
+    // only the diagonal tiles and their neighbours take FP64 products.
     void SetDenseBandDP(int aBand) { mBandDP = aBand; }
     int GetDenseBandDP() const { return mBandDP; }
     void SetBandDenseSP(int aBand) { mBandSP = aBand; }
@@ -140,7 +141,7 @@ private:
     int mTimeSlot = 1;
     common::Computation mComputation = common::EXACT_DENSE;
     common::Precision mPrecision = common::DOUBLE;
-    int mCores = 1, mGpus = 0, mDts = 8, mBand = 0, mBandDP = 2, mBandSP = 2;
+    int mCores = 1, mGpus = 0, mDts = 8, mBand = 0, mBandDP = 4, mBandSP = 4;
     int mSeed = 0;
     std::vector<double> mInitialTheta, mLowerBounds, mUpperBounds, mEstimatedTheta, mStartingTheta;
     common::Dimension mDimension = common::Dimension2D;

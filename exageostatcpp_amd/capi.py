@@ -69,6 +69,7 @@ EXPORTS = {
     "egs_problem_device_bytes": ([_vp, ctypes.POINTER(_i64)], _int),
     "egs_problem_set_precision": ([_vp, _int, _int, _int], _int),
     "egs_problem_set_single_product": ([_vp, _int, ctypes.c_double], _int),
+    "egs_problem_set_fp64_product_band": ([_vp, _int], _int),
     "egs_problem_set_observations": ([_vp, _dp, _i64], _int),
     "egs_problem_get_observations": ([_vp, _dp, _i64], _int),
     "egs_loglik": ([_vp, _dp, _dp, _dp, _dp], _int),
@@ -206,7 +207,7 @@ class Problem:
         _check(lib().egs_problem_device_bytes(self.handle, ctypes.byref(b)))
         return b.value
 
-    def set_precision(self, mode, band_fp64=2, band_fp32=2):
+    def set_precision(self, mode, band_fp64=4, band_fp32=4):
         _check(lib().egs_problem_set_precision(self.handle, mode, band_fp64, band_fp32))
 
     def set_single_product(self, band, tau=2.0 ** -16):
@@ -257,6 +258,11 @@ class Problem:
         it, res = _int(), ctypes.c_double()
         _check(lib().egs_last_refinement(self.handle, ctypes.byref(it), ctypes.byref(res)))
         return {"iterations": it.value, "residual": res.value}
+
+    def set_fp64_product_band(self, band):
+        """Tiles of the FP64 band beyond `band` take split FP16 products into
+        their FP64 storage (None or negative: FP64 products throughout)."""
+        _check(lib().egs_problem_set_fp64_product_band(self.handle, -1 if band is None else int(band)))
 
     def product_counts(self):
         """Block updates of the last factorization with the split and with the

@@ -76,7 +76,7 @@ struct Profiler {
 };
 
 // Update classes of a mixed plan (see tile_class).
-constexpr int kClasses = 4;
+constexpr int kClasses = 5;
 // Panel copies in flight: panel k is read by the bulk update of step k while
 // the critical stream writes panel k+1 and then panel k+2.
 constexpr int kPanelSlots = 3;
@@ -92,10 +92,15 @@ struct egs_problem_s {
     // block's product error, about 2^-12 K bmax(A) bmax(B), stays 16x under
     // 2^-24 K pmax^2, the split product's for the panel's largest block.
     int band_single = 0;
+    // FP64 tiles with tile distance > band_prod64 (within band64) take split
+    // FP16 products into their FP64 storage instead of FP64 products.  1: the
+    // diagonal tiles and their neighbours take FP64 products (0, only the
+    // diagonal, put 2.5e-6 into the bivariate 25k likelihood)
+    int band_prod64 = 1;
     double tau_single = 0x1p-16;
     // mixed plans: per-class update lists (descending column) and, for class c,
     // cls_count[c][j] = number of its tiles in columns >= j
-    int2 *d_pairs_cls[kClasses] = {nullptr, nullptr, nullptr, nullptr};
+    int2 *d_pairs_cls[kClasses] = {};
     std::vector<int2> h_pairs_cls[kClasses];
     std::vector<int> cls_count[kClasses];
     // panel k as FP64 column-major, FP32 column-major and 16-bit row-major, in
@@ -354,7 +359,7 @@ uint8_t tile_store(const egs_problem_s *p, int i, int j) {
 // product (beyond band_single under MIXED, per block when tau_single is finite).
 int tile_class(const egs_problem_s *p, int i, int j) {
     const int d = i - j;
-    if (d <= p->band64) return 0;
+    if (d <= p->band64) return (p->mode == EGS_PRECISION_MIXED && d > p->band_prod64) ? 4 : 0;
     if (d <= p->band32) return 1;
     if (p->mode == EGS_PRECISION_MIXED && p->band_single >= 0 && d > p->band_single) return 3;
     return 2;
@@ -541,7 +546,7 @@ double update_flops_cls(const egs_problem_s *p, int c, int j0, int j1) {
 void launch_mixed_update(egs_problem_s *p, int k, int jcol_lo, int jcol_hi, hipStream_t st) {
     const egs::TileMap m = p->map();
     static const int prof_id[kClasses] = {EGS_PROFILE_UPDATE, EGS_PROFILE_UPDATE_F32, EGS_PROFILE_UPDATE_16,
-                                          EGS_PROFILE_UPDATE_16_SELECT};
+                                          EGS_PROFILE_UPDATE_16_SELECT, EGS_PROFILE_UPDATE_16};
     for (int c = 0; c < kClasses; ++c) {
         // class-c tiles with column in [jcol_lo, jcol_hi): list entries [count[jcol_hi], count[jcol_lo])
         const int beg = p->cls_count[c][jcol_hi];
@@ -554,8 +559,9 @@ void launch_mixed_update(egs_problem_s *p, int k, int jcol_lo, int jcol_hi, hipS
         else if (c == 1) egs::launch_update192_f32(m, p->d_p32[slot], pr, end - beg, k, st);
         else {
             const egs::Operands op =
-                c == 3 ? (std::isinf(p->tau_single) ? egs::kSingleF16 : egs::kAdaptiveF16)
-                       : (p->mode == EGS_PRECISION_MIXED ? egs::kSplitF16 : egs::kSingleBF16);
+                c == 4 ? egs::kSplitF16C64
+                       : c == 3 ? (std::isinf(p->tau_single) ? egs::kSingleF16 : egs::kAdaptiveF16)
+                                : (p->mode == EGS_PRECISION_MIXED ? egs::kSplitF16 : egs::kSingleBF16);
             egs::launch_update192_16(m, p->d_p16h[slot], p->d_p16l[slot], p->d_pinv[slot], p->d_bmax[slot],
                                      p->d_pmax, (float) p->tau_single, pr, end - beg, k, op, p->d_counts, st);
         }
@@ -1023,6 +1029,14 @@ int egs_problem_set_precision(egs_problem_t p, int mode, int band_fp64, int band
         free_sigma_copy(p);
         free_tile_inverses(p);
     }
+    return is_mixed(p) ? setup_mixed(p) : EGS_SUCCESS;
+}
+
+int egs_problem_set_fp64_product_band(egs_problem_t p, int band) {
+    int st = check_problem(p);
+    if (st) return st;
+    EGS_HIP(hipStreamSynchronize(p->ctx->stream));
+    p->band_prod64 = band < 0 ? (1 << 30) : band;
     return is_mixed(p) ? setup_mixed(p) : EGS_SUCCESS;
 }
 

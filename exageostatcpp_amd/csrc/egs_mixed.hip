@@ -295,7 +295,7 @@ __device__ __forceinline__ f32x16 mfma_32(const u16x8 &x, const u16x8 &y, f32x16
 // bmax(A rows) * bmax(B rows) <= tau * pmax(panel k)^2, split otherwise.
 enum { kModeSplit = 0, kModeSingle = 1, kModeAdaptive = 2 };
 
-template <typename T16, int kMode, int kDepth>
+template <typename T16, int kMode, int kDepth, bool kC64 = false>
 __global__ void __launch_bounds__(kNTw, kMode == kModeSplit ? 1 : 2) update192_w_kernel(TileMap map, const uint16_t *p16h, const uint16_t *p16l,
                                                               const float *inv_scale, const float *bmax,
                                                               const unsigned *pmax, float tau, const int2 *pairs,
@@ -485,7 +485,13 @@ __global__ void __launch_bounds__(kNTw, kMode == kModeSplit ? 1 : 2) update192_w
     // C -= acc (times the two row scales' inverses, exact powers of two): lane
     // row (A side) wm*96 + mi*32 + lr, column (B side) wn*96 + ni*32 + 8 (v / 4)
     // + 4 lk + v % 4; one 3 x 16 batch per mi
-    float *C = tile_f32(map, ij.x, ij.y);
+    // C is FP32, or FP64 for tiles of the FP64 band updated with 16-bit products
+    using TC = typename std::conditional<kC64, double, float>::type;
+    TC *C;
+    if constexpr (kC64)
+        C = tile_f64(map, ij.x, ij.y);
+    else
+        C = tile_f32(map, ij.x, ij.y);
     const uint32_t cbase = (uint32_t) ((sbm * kBM + wm * 96 + lr) + (sbn * kBM + wn * 96 + 4 * lk) * ts);
     constexpr bool kScaled = std::is_same<T16, _Float16>::value;
     const float *invA = kScaled ? inv_scale + (int64_t) (ij.x - k - 1) * ts + sbm * kBM + wm * 96 + lr : nullptr;
@@ -495,7 +501,7 @@ __global__ void __launch_bounds__(kNTw, kMode == kModeSplit ? 1 : 2) update192_w
         const float sa = kScaled ? invA[mi * 32] : 1.0f;
 #pragma unroll
         for (int ni = 0; ni < 3; ++ni) {
-            float cv[16];
+            TC cv[16];
             f32x4v sb[4];
 #pragma unroll
             for (int g = 0; g < 4; ++g)
@@ -508,7 +514,7 @@ __global__ void __launch_bounds__(kNTw, kMode == kModeSplit ? 1 : 2) update192_w
             for (int v = 0; v < 16; ++v) {
                 float a = acc[mi][ni][v];
                 if constexpr (kScaled) a *= sa * sb[v >> 2][v & 3];
-                __builtin_nontemporal_store(cv[v] - a,
+                __builtin_nontemporal_store(cv[v] - (TC) a,
                                             &C[cbase + (uint32_t) (mi * 32 + (ni * 32 + 8 * (v >> 2) + (v & 3)) * ts)]);
             }
         }
@@ -566,6 +572,10 @@ void launch_update192_16(const TileMap &map, const uint16_t *p16h, const uint16_
     case kAdaptiveF16:
         hipLaunchKernelGGL((update192_w_kernel<_Float16, kModeAdaptive, 2>), grid, dim3(kNTw), 0, s, map, p16h,
                            p16l, inv_scale, bmax, pmax, tau, pairs, k, counts);
+        break;
+    case kSplitF16C64:
+        hipLaunchKernelGGL((update192_w_kernel<_Float16, kModeSplit, 2, true>), grid, dim3(kNTw), 0, s, map, p16h,
+                           p16l, inv_scale, bmax, pmax, tau, pairs, k, nullptr);
         break;
     default:
         hipLaunchKernelGGL((update192_w_kernel<__bf16, kModeSingle, 2>), grid, dim3(kNTw), 0, s, map, p16h, p16l,

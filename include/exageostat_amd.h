@@ -108,6 +108,12 @@ int egs_problem_set_precision(egs_problem_t problem, int mode, int band_fp64, in
  * (2^-24 K smax^2).
  * Kept across egs_problem_set_precision calls. */
 int egs_problem_set_single_product(egs_problem_t problem, int band, double tau);
+/* Under EGS_PRECISION_MIXED, tiles of the FP64 band (|i - j| <= band_fp64)
+ * with |i - j| > band keep FP64 storage but are updated with split FP16
+ * products accumulated in FP32 and added in FP64.  Default 1: the diagonal
+ * tiles and their neighbours take FP64 products.  Negative: every tile of the
+ * FP64 band does.  Kept across egs_problem_set_precision calls. */
+int egs_problem_set_fp64_product_band(egs_problem_t problem, int band);
 /* Iterations and final relative FP64 residual of the last refinement. */
 int egs_last_refinement(egs_problem_t problem, int *iterations, double *residual);
 /* Block updates (192 x 192) of the tiles beyond the single-product band in the

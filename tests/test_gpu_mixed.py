@@ -85,7 +85,9 @@ def test_mixed_plain_bf16_refines_quad(ctx, data4k):
 
 
 def test_mixed_fp64_only_band_equals_fp64(ctx):
-    # bands that cover every tile give the FP64 factorization
+    """An FP64 band that covers every tile, with FP64 products throughout, is
+    the FP64 factorization; with FP64 products on the diagonal tiles only (the
+    default) the other tiles keep FP64 storage but take split FP16 products."""
     x, y = orc.generate_locations(1500, seed=3)
     th = [1.0, 0.1, 1.0]
     z = orc.generate_observations(0, (x, y), th, 192, seed=1)
@@ -93,8 +95,13 @@ def test_mixed_fp64_only_band_equals_fp64(ctx):
     p = egs.Problem(ctx, 0, x, y, dts=192)
     p.set_observations(z)
     p.set_precision(egs.PRECISION_MIXED, 100, 100)
+    p.set_fp64_product_band(None)
     llh, _, _ = p.loglik(th)
     assert abs(llh - ref["llh"]) <= 1e-10 * abs(ref["llh"])
+    p.set_fp64_product_band(0)
+    llh0, _, _ = p.loglik(th)
+    log("  fp64 products everywhere %.12f, on the diagonal only %.12f, oracle %.12f" % (llh, llh0, ref["llh"]))
+    assert llh0 != llh and abs(llh0 - ref["llh"]) <= 1e-6 * abs(ref["llh"])
 
 
 def test_mixed_kriging_matches_oracle(ctx, data4k):
