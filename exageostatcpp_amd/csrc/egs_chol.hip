@@ -3,15 +3,15 @@
 // tile, TRSM of the panel below it, SYRK/GEMM of the trailing tiles).
 //
 // Step k of the factorization:
-//   * the diagonal tile is factored left-looking in 64 x 64 blocks: one
-//     workgroup per block updates it, factors it in LDS with a per-column
-//     pivot broadcast, and inverts it (potrf_block_kernel); the rows below it
-//     inside the tile are then solved against that block (trsm_rows_kernel);
+//   * the diagonal tile is factored right-looking in 64 x 64 blocks
+//     (potrf64_kernel, diag_update_kernel, diag_copy_kernel below), which also
+//     leaves the 64 x 64 inverses of its diagonal blocks;
 //   * the panel below the diagonal tile is solved by one workgroup per 64-row
 //     block, marching over the 64-column blocks of L_kk and applying the
 //     64 x 64 inverses (trsm_rows_kernel);
-//   * the trailing tiles are updated with FP64 MFMA (update_kernel), one
-//     workgroup per 64 x 64 output block with K = ts.
+//   * the trailing tiles are updated with FP64 MFMA: update192 (egs_update.hip)
+//     when ts is a multiple of 192, else update_kernel here, one workgroup per
+//     64 x 64 output block with K = ts.
 // Every contraction uses v_mfma_f64_16x16x4f64 through wg_gemm_nt: four waves
 // each own a 32 x 32 quadrant of the 64 x 64 output; K is staged through LDS in
 // chunks of 32 columns, double buffered, with the next chunk's global loads
@@ -125,66 +125,6 @@ __device__ __forceinline__ void acc_foreach(const Acc64 &acc, F f) {
 }
 
 // ------------------------------------------------------------------------
-// Diagonal 64 x 64 block: update, factor, invert.
-// ------------------------------------------------------------------------
-__global__ void __launch_bounds__(kThreads) potrf_block_kernel(TileMap map, int k, int jb, double *winv_k, int *info,
-                                                               int64_t n) {
-    __shared__ GemmSmem sm;
-    __shared__ double sT[kBlk][kLdsPad];
-    __shared__ double sW[kBlk][kLdsPad];
-    const int ts = map.ts;
-    const int t = threadIdx.x;
-    double *T = tile_f64(map, k, k);
-    double *Ajj = T + (int64_t) jb * kBlk + (int64_t) jb * kBlk * ts;
-
-    Acc64 acc;
-    acc_zero(acc);
-    if (jb > 0) wg_gemm_nt(acc, T + jb * kBlk, ts, T + jb * kBlk, ts, jb * kBlk, sm);
-    acc_foreach(acc, [&](int r, int c, double v) { sT[r][c] = Ajj[r + (int64_t) c * ts] - v; });
-    __syncthreads();
-
-    // Right-looking Cholesky of sT (lower).
-    for (int j = 0; j < kBlk; ++j) {
-        const double d = sT[j][j];
-        const double s = sqrt(d);
-        const double inv = 1.0 / s;
-        if (t == 0 && !(d > 0.0)) {
-            const int64_t col = (int64_t) k * ts + (int64_t) jb * kBlk + j;
-            if (col < n) atomicCAS(info, 0, (int) (col + 1));
-        }
-        for (int e = t; e < kBlk * kBlk; e += kThreads) {
-            const int r = e & 63, c = e >> 6;
-            if (c > j && r >= c) sT[r][c] -= (sT[r][j] * inv) * (sT[c][j] * inv);
-        }
-        __syncthreads();
-        if (t >= j && t < kBlk) sT[t][j] = (t == j) ? s : sT[t][j] * inv;
-        __syncthreads();
-    }
-
-    // Inverse of the lower factor, one column per thread.
-    if (t < kBlk) {
-        const int c = t;
-        for (int r = 0; r < kBlk; ++r) {
-            if (r < c) {
-                sW[r][c] = 0.0;
-                continue;
-            }
-            double acc_s = (r == c) ? 1.0 : 0.0;
-            for (int l = c; l < r; ++l) acc_s -= sT[r][l] * sW[l][c];
-            sW[r][c] = acc_s / sT[r][r];
-        }
-    }
-    __syncthreads();
-
-    double *W = winv_k + (int64_t) jb * kBlk * kBlk;
-    for (int e = t; e < kBlk * kBlk; e += kThreads) {
-        const int r = e & 63, c = e >> 6;
-        Ajj[r + (int64_t) c * ts] = (r >= c) ? sT[r][c] : 0.0;
-        W[r + c * kBlk] = sW[r][c];
-    }
-}
-
-// ------------------------------------------------------------------------
 // Row-block triangular solve against L_kk, column blocks [jb0, jb1).
 // ------------------------------------------------------------------------
 template <typename TX>
@@ -254,10 +194,6 @@ __global__ void __launch_bounds__(kThreads) update_kernel(TileMap map, const int
     acc_foreach(acc, [&](int r, int c, double) { cv[q++] = C[r + (int64_t) c * ts]; });
     q = 0;
     acc_foreach(acc, [&](int r, int c, double v) { C[r + (int64_t) c * ts] = cv[q++] - v; });
-}
-
-void launch_potrf_block(const TileMap &map, int k, int jb, double *winv_k, int *info, int64_t n, hipStream_t s) {
-    hipLaunchKernelGGL(potrf_block_kernel, dim3(1), dim3(kThreads), 0, s, map, k, jb, winv_k, info, n);
 }
 
 void launch_trsm_rows(const TileMap &map, int k, int64_t g0, int count, int jb0, int jb1, const double *winv_k,

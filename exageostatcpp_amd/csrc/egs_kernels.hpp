@@ -21,10 +21,6 @@ void launch_covgen_rect(double *out, int rows, int cols, int64_t row_off, int64_
                         const double *z2, hipStream_t s);
 
 // ---- tiled Cholesky (egs_chol.hip)
-// Factor 64 x 64 diagonal block jb of diagonal tile k (after its left-looking
-// update) and write its inverse to winv_k + jb * 4096.  The first failing
-// global column (1-based) is recorded in *info.
-void launch_potrf_block(const TileMap &map, int k, int jb, double *winv_k, int *info, int64_t n, hipStream_t s);
 // Triangular solve X L_kk^T = B for row blocks [g0, g0 + count) of tile column k
 // (global 64-row block g lives in tile g / nsb at row (g % nsb) * 64), column
 // blocks [jb0, jb1).
@@ -41,8 +37,6 @@ void launch_update192(const TileMap &map, const int2 *pairs, int npairs, int k, 
 // Forward solve of tile row k: y_k = L_kk^{-1} z_k in place, with per-tile
 // partials[2k] = sum log L_ii (L_ii > 0), partials[2k+1] = y_k . y_k.
 void launch_trsv_diag(const TileMap &map, int k, const double *winv_k, double *z, double *partials, hipStream_t s);
-// z_i -= L_ik y_k for every tile row i > k.
-void launch_trsv_panel(const TileMap &map, int k, double *z, hipStream_t s);
 // Backward solve with L^T of tile row k (after rows > k are solved):
 // partial sums of L_ik^T y_i into scratch, then the diagonal solve.
 void launch_trsv_t_partial(const TileMap &map, int k, const double *z, double *scratch, hipStream_t s);

@@ -115,35 +115,6 @@ __global__ void __launch_bounds__(kSolveThreads) trsv_diag_kernel(TileMap map, i
     }
 }
 
-// z_i -= L_ik y_k for the tile rows below k, one workgroup per 64 rows.
-__global__ void __launch_bounds__(kThreads) trsv_panel_kernel(TileMap map, int k, double *z) {
-    __shared__ double sy[kMaxTs];
-    __shared__ double sp[4][kBlk];
-    const int ts = map.ts;
-    const int nsb = ts / kBlk;
-    const int t = threadIdx.x;
-    const int r = t & 63, g = t >> 6;
-    const int i = k + 1 + (int) (blockIdx.x / nsb);
-    const int rb = (int) (blockIdx.x % nsb);
-    const double *yk = z + (int64_t) k * ts;
-    for (int e = t; e < ts; e += kThreads) sy[e] = yk[e];
-    __syncthreads();
-    double s = 0.0;
-    if (map.store[i * map.nt + k] == kStoreF32) {
-        const float *A = tile_f32(map, i, k) + rb * kBlk;
-        for (int c = g; c < ts; c += 4) s += (double) A[r + (int64_t) c * ts] * sy[c];
-    } else {
-        const double *A = tile_f64(map, i, k) + rb * kBlk;
-        for (int c = g; c < ts; c += 4) s += A[r + (int64_t) c * ts] * sy[c];
-    }
-    sp[g][r] = s;
-    __syncthreads();
-    if (t < kBlk) {
-        double *zi = z + (int64_t) i * ts + rb * kBlk;
-        zi[t] -= ((sp[0][t] + sp[1][t]) + sp[2][t]) + sp[3][t];
-    }
-}
-
 // scratch[(i - k - 1) * ts + c] = sum_r L_ik[r][c] y_i[r] for every tile row i > k.
 __global__ void __launch_bounds__(kThreads) trsv_t_partial_kernel(TileMap map, int k, const double *z,
                                                                   double *scratch) {
@@ -436,13 +407,6 @@ void launch_axpby(double *y, const double *x, double a, double b, int64_t n, hip
 
 void launch_trsv_diag(const TileMap &map, int k, const double *winv_k, double *z, double *partials, hipStream_t s) {
     hipLaunchKernelGGL(trsv_diag_kernel, dim3(1), dim3(kSolveThreads), 0, s, map, k, winv_k, z, partials);
-}
-
-void launch_trsv_panel(const TileMap &map, int k, double *z, hipStream_t s) {
-    const int nsb = map.ts / kBlk;
-    const int rows_below = map.nt - k - 1;
-    if (rows_below <= 0) return;
-    hipLaunchKernelGGL(trsv_panel_kernel, dim3(rows_below * nsb), dim3(kThreads), 0, s, map, k, z);
 }
 
 void launch_trsv_t_partial(const TileMap &map, int k, const double *z, double *scratch, hipStream_t s) {
