@@ -262,7 +262,7 @@ def mle_compare(egs, args):
     prob = egs.Problem(ctx, "UnivariateMaternStationary", x, y, dts=args.dts)
     prob.generate_observations(theta, seed=0)
     out = {"workload": "maximum-likelihood fit, n=%d, theta_true=%s, dts=%d, bounds %s..%s, start theta_true, "
-                       "ftol %g" % (n, theta, args.dts, lb, ub, args.mle_ftol)}
+                       "ftol %g, rho_end %g" % (n, theta, args.dts, lb, ub, args.mle_ftol, args.mle_rho_end)}
     fp64_prob = None
     for label, mode in (("fp64", None), ("mixed", egs.PRECISION_MIXED)):
         if mode is not None:
@@ -285,7 +285,8 @@ def mle_compare(egs, args):
             return -llh
 
         t0 = time.perf_counter()
-        th, fmin, nev, _ = egs.capi.bobyqa(objective, theta, lb, ub, ftol_abs=args.mle_ftol, maxeval=args.maxeval)
+        th, fmin, nev, _ = egs.capi.bobyqa(objective, theta, lb, ub, ftol_abs=args.mle_ftol, maxeval=args.maxeval,
+                                           rho_end=args.mle_rho_end)
         dt = time.perf_counter() - t0
         out[label] = {"theta_hat": list(th), "loglik": -fmin, "evaluations": nev, "fp64_fallbacks": fallback[0],
                       "seconds": dt}
@@ -328,6 +329,9 @@ def main():
                          "(defaults to n=40000, theta=(1, 0.05, 0.5), ftol 1e-9, tighter than the reference's "
                          "default 1e-8)")
     ap.add_argument("--mle-ftol", type=float, default=1e-9)
+    ap.add_argument("--mle-rho-end", type=float, default=0.0,
+                    help="BOBYQA's final trust-region radius (0: the solver's default); with --mle-ftol 0 the fits "
+                         "stop on the radius alone")
     ap.add_argument("--bivariate", action="store_true",
                     help="the bivariate great-circle workload (25,000 locations on the sphere) instead of the default")
     ap.add_argument("--theta", type=float, nargs=3, default=None)
