@@ -70,6 +70,7 @@ EXPORTS = {
     "egs_problem_set_precision": ([_vp, _int, _int, _int], _int),
     "egs_problem_set_single_product": ([_vp, _int, ctypes.c_double], _int),
     "egs_problem_set_fp64_product_band": ([_vp, _int], _int),
+    "egs_problem_set_graph": ([_vp, _int], _int),
     "egs_problem_set_observations": ([_vp, _dp, _i64], _int),
     "egs_problem_get_observations": ([_vp, _dp, _i64], _int),
     "egs_loglik": ([_vp, _dp, _dp, _dp, _dp], _int),
@@ -258,6 +259,10 @@ class Problem:
         it, res = _int(), ctypes.c_double()
         _check(lib().egs_last_refinement(self.handle, ctypes.byref(it), ctypes.byref(res)))
         return {"iterations": it.value, "residual": res.value}
+
+    def set_graph(self, enable=True):
+        """Replay FP64-plan evaluations from a captured hipGraph."""
+        _check(lib().egs_problem_set_graph(self.handle, 1 if enable else 0))
 
     def set_fp64_product_band(self, band):
         """Tiles of the FP64 band beyond `band` take split FP16 products into

@@ -150,6 +150,13 @@ struct egs_problem_s {
     std::vector<hipEvent_t> ev_panel, ev_bulk;   // per step: panel solved / bulk update done
     double t_cov = 0, t_fact = 0, t_solve = 0;
     Profiler prof;
+    // FP64 evaluations replayed from a captured graph (egs_problem_set_graph):
+    // the covariance parameters live in d_prm so that the graph serves every theta
+    bool use_graph = false;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    egs::MaternParams *d_prm = nullptr;
+    egs::MaternParams h_prm;
 
     egs::TileMap map() const {
         egs::TileMap m;
@@ -237,11 +244,23 @@ void free_tile_inverses(egs_problem_s *p) {
     p->d_tinv = p->d_ybuf = nullptr;
 }
 
+// Drop the captured evaluation graph (the launch sequence it recorded no
+// longer matches the problem's plan or buffers).
+void invalidate_graph(egs_problem_s *p) {
+    if (p->graph_exec) hipGraphExecDestroy(p->graph_exec);
+    if (p->graph) hipGraphDestroy(p->graph);
+    p->graph_exec = nullptr;
+    p->graph = nullptr;
+}
+
 void free_problem_buffers(egs_problem_s *p) {
     if (p->ctx) {
         auto &v = p->ctx->problems;
         v.erase(std::remove(v.begin(), v.end(), p), v.end());
     }
+    invalidate_graph(p);
+    hipFree(p->d_prm);
+    p->d_prm = nullptr;
     hipFree(p->d_x);
     hipFree(p->d_y);
     hipFree(p->d_z);
@@ -592,7 +611,8 @@ int run_potrf(egs_problem_s *p) {
         p->ev_panel.push_back(e1);
         p->ev_bulk.push_back(e2);
     }
-    EGS_HIP(hipMemsetAsync(p->d_info, 0, sizeof(int), s));
+    // a fill kernel, not a memset: this sequence is captured into a graph
+    egs::launch_zero_u32(reinterpret_cast<unsigned *>(p->d_info), 1, s);
     if (p->mode == EGS_PRECISION_MIXED) {
         egs::launch_zero_u32(p->d_pmax, nt, s);
         egs::launch_zero_u32(reinterpret_cast<unsigned *>(p->d_counts), 4, s);
@@ -634,9 +654,13 @@ int run_potrf(egs_problem_s *p) {
         factor_panel(p, k + 1);
         EGS_HIP(hipEventRecord(p->ev_panel[k + 1], s));
     }
-    // ev_bulk[nt - 1] (unused by the steps) marks the end of the bulk stream
-    EGS_HIP(hipEventRecord(p->ev_bulk[nt - 1], b));
-    EGS_HIP(hipStreamWaitEvent(s, p->ev_bulk[nt - 1], 0));
+    // ev_bulk[nt - 1] (unused by the steps) marks the end of the bulk stream;
+    // a one-tile matrix never uses it (and a capture must not wait on an event
+    // recorded outside it)
+    if (nt >= 2) {
+        EGS_HIP(hipEventRecord(p->ev_bulk[nt - 1], b));
+        EGS_HIP(hipStreamWaitEvent(s, p->ev_bulk[nt - 1], 0));
+    }
     if (is_mixed(p)) egs::launch_tinv(m, p->d_winv, p->d_tinv, s);
     EGS_HIP(hipGetLastError());
     return EGS_SUCCESS;
@@ -1020,6 +1044,7 @@ int egs_problem_set_precision(egs_problem_t p, int mode, int band_fp64, int band
         return fail(EGS_ERR_UNSUPPORTED, "mixed precision needs a dense tile size that is a multiple of 192 (got %d)",
                     p->ts);
     EGS_HIP(hipStreamSynchronize(p->ctx->stream));
+    invalidate_graph(p);
     p->mode = mode;
     p->band64 = band_fp64;
     p->band32 = band_fp32;
@@ -1032,10 +1057,20 @@ int egs_problem_set_precision(egs_problem_t p, int mode, int band_fp64, int band
     return is_mixed(p) ? setup_mixed(p) : EGS_SUCCESS;
 }
 
+int egs_problem_set_graph(egs_problem_t p, int enable) {
+    int st = check_problem(p);
+    if (st) return st;
+    EGS_HIP(hipStreamSynchronize(p->ctx->stream));
+    invalidate_graph(p);
+    p->use_graph = enable != 0;
+    return EGS_SUCCESS;
+}
+
 int egs_problem_set_fp64_product_band(egs_problem_t p, int band) {
     int st = check_problem(p);
     if (st) return st;
     EGS_HIP(hipStreamSynchronize(p->ctx->stream));
+    invalidate_graph(p);
     p->band_prod64 = band < 0 ? (1 << 30) : band;
     return is_mixed(p) ? setup_mixed(p) : EGS_SUCCESS;
 }
@@ -1045,6 +1080,7 @@ int egs_problem_set_single_product(egs_problem_t p, int band, double tau) {
     if (st) return st;
     if (std::isnan(tau) || tau < 0.0) return fail(EGS_ERR_INVALID_ARG, "tau must be >= 0 (got %g)", tau);
     EGS_HIP(hipStreamSynchronize(p->ctx->stream));
+    invalidate_graph(p);
     p->band_single = band < 0 ? -1 : band;
     p->tau_single = tau;
     return is_mixed(p) ? setup_mixed(p) : EGS_SUCCESS;
@@ -1112,12 +1148,96 @@ int egs_solve_logdet_quad(egs_problem_t p, double *logdet, double *quad) {
     return EGS_SUCCESS;
 }
 
+// One FP64 evaluation on the problem's streams, with the covariance parameters
+// read from d_prm: observations copied, covariance, factorization, forward solve
+// with the per-tile log-det and dot partials.  Everything it enqueues is
+// capturable (no host synchronization, no memset).
+int enqueue_fp64_eval(egs_problem_s *p) {
+    hipStream_t s = p->ctx->stream;
+    egs::launch_copy_pad(p->d_work, p->d_obs, p->n, p->npad, s);
+    const double *dz = (p->kernel == EGS_KERNEL_UNIVARIATE_MATERN_STATIONARY) ? p->d_z : nullptr;
+    egs::launch_covgen_tiles(p->map(), p->d_pairs, (int) p->h_pairs.size(), p->h_prm, p->d_x, p->d_y, dz, p->n,
+                             nullptr, s, p->d_prm);
+    const int st = run_potrf(p);
+    if (st) return st;
+    run_forward(p);
+    EGS_HIP(hipGetLastError());
+    return EGS_SUCCESS;
+}
+
+// Captures the problem's FP64 evaluation into a graph (once per plan); theta
+// reaches the graph through d_prm.
+int graph_capture(egs_problem_s *p) {
+    hipStream_t s = p->ctx->stream;
+    if (!p->d_prm && hipMalloc(&p->d_prm, sizeof(egs::MaternParams)) != hipSuccess)
+        return fail(EGS_ERR_OUT_OF_MEMORY, "cannot allocate the covariance parameters");
+    EGS_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed));
+    const int st = enqueue_fp64_eval(p);
+    hipGraph_t g = nullptr;
+    const hipError_t ec = hipStreamEndCapture(s, &g);
+    if (st || ec != hipSuccess) {
+        if (g) hipGraphDestroy(g);
+        return st ? st : fail(EGS_ERR_HIP, "graph capture failed: %s", hipGetErrorString(ec));
+    }
+    p->graph = g;
+    if (hipGraphInstantiate(&p->graph_exec, g, nullptr, nullptr, 0) != hipSuccess) {
+        invalidate_graph(p);
+        return fail(EGS_ERR_HIP, "graph instantiation failed");
+    }
+    return EGS_SUCCESS;
+}
+
+// Replays the captured evaluation for theta.
+int graph_launch(egs_problem_s *p, const double *theta) {
+    hipStream_t s = p->ctx->stream;
+    const int st = make_params(p->kernel, p->metric, theta, p->h_prm);
+    if (st) return st;
+    // h_prm is not touched again before the stream synchronizes
+    EGS_HIP(hipMemcpyAsync(p->d_prm, &p->h_prm, sizeof(egs::MaternParams), hipMemcpyHostToDevice, s));
+    EGS_HIP(hipGraphLaunch(p->graph_exec, s));
+    return EGS_SUCCESS;
+}
+
 int egs_loglik(egs_problem_t p, const double *theta, double *loglik, double *logdet, double *quad) {
     int st = check_problem(p);
     if (st) return st;
     if (!loglik) return fail(EGS_ERR_INVALID_ARG, "loglik is null");
     if (!p->has_obs) return fail(EGS_ERR_STATE, "the problem has no observations");
     hipStream_t s = p->ctx->stream;
+    if (p->use_graph && !is_mixed(p) && !p->prof.on && !p->graph_exec) {
+        // first use: capture; if the runtime refuses, evaluate directly from now on
+        st = graph_capture(p);
+        if (st == EGS_ERR_HIP) {
+            (void) hipGetLastError();
+            invalidate_graph(p);
+            p->use_graph = false;
+        } else if (st) {
+            return st;
+        }
+    }
+    if (p->use_graph && !is_mixed(p) && !p->prof.on) {
+        EGS_HIP(hipEventRecord(p->ev[0], s));
+        st = graph_launch(p, theta);
+        if (st) return st;
+        EGS_HIP(hipEventRecord(p->ev[3], s));
+        EGS_HIP(hipStreamSynchronize(s));
+        std::memcpy(p->cov_theta, theta, sizeof(double) * p->ntheta());
+        st = read_info(p);
+        if (st) return st;
+        p->factored = true;
+        std::memcpy(p->factored_theta, theta, sizeof(double) * p->ntheta());
+        float ms = 0;
+        hipEventElapsedTime(&ms, p->ev[0], p->ev[3]);
+        p->t_cov = p->t_solve = 0.0;   // one graph: only the total is timed
+        p->t_fact = ms;
+        double ld, q;
+        st = read_partials(p, ld, q);
+        if (st) return st;
+        *loglik = -0.5 * q - 0.5 * ld - (double) p->n / 2.0 * std::log(2.0 * kPi);
+        if (logdet) *logdet = ld;
+        if (quad) *quad = q;
+        return EGS_SUCCESS;
+    }
     EGS_HIP(hipEventRecord(p->ev[0], s));
     egs::launch_copy_pad(p->d_work, p->d_obs, p->n, p->npad, s);
     st = run_covariance(p, theta);

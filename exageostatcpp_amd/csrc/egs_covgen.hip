@@ -22,11 +22,14 @@ struct CovTilesArgs {
     int64_t n;              // true matrix dimension; rows/cols >= n are identity padding
     int three_d;
     double *sig64;          // optional FP64 copy of every lower tile (tile i*(i+1)/2 + j, ts*ts each)
+    const MaternParams *prm_dev;   // optional: read the parameters here instead of prm
 };
 
 __global__ void __launch_bounds__(kThreads) covgen_tiles_kernel(CovTilesArgs a) {
     __shared__ double s_rx[kBlk], s_ry[kBlk], s_rz[kBlk];
     __shared__ double s_cx[kBlk], s_cy[kBlk], s_cz[kBlk];
+    // parameters from device memory when given (a captured graph replays with new theta)
+    const MaternParams prm = a.prm_dev ? *a.prm_dev : a.prm;
     const int ts = a.map.ts;
     const int nsb = ts / kBlk;
     const int2 ij = a.pairs[blockIdx.y];
@@ -36,7 +39,7 @@ __global__ void __launch_bounds__(kThreads) covgen_tiles_kernel(CovTilesArgs a) 
     const int64_t c0 = (int64_t) ij.y * ts + sbc * kBlk;
     const int t = threadIdx.x;
     const bool three_d = a.three_d != 0;
-    const int shift = a.prm.kernel == 1 ? 1 : 0;
+    const int shift = prm.kernel == 1 ? 1 : 0;
     const int64_t nloc = a.n >> shift;
 
     // Stage the locations of this block's rows and columns.
@@ -72,9 +75,9 @@ __global__ void __launch_bounds__(kThreads) covgen_tiles_kernel(CovTilesArgs a) 
             v = (gr == gc) ? 1.0 : 0.0;
         } else {
             const int pair = shift ? (rvar * 2 + (int) (gc & 1)) : 0;
-            const double dist = distance(rx, ry, rz, s_cx[cc], s_cy[cc], s_cz[cc], a.prm.metric, three_d);
-            const double d = dist / a.prm.range;
-            v = (d == 0.0) ? a.prm.sig[pair] : matern_scaled(d, a.prm.con[pair], a.prm.sig[pair], a.prm.nu[pair]);
+            const double dist = distance(rx, ry, rz, s_cx[cc], s_cy[cc], s_cz[cc], prm.metric, three_d);
+            const double d = dist / prm.range;
+            v = (d == 0.0) ? prm.sig[pair] : matern_scaled(d, prm.con[pair], prm.sig[pair], prm.nu[pair]);
         }
         const int64_t idx = (int64_t) (sbr * kBlk + r) + (int64_t) (sbc * kBlk + cc) * ts;
         if (store == kStoreF64) reinterpret_cast<double *>(tile)[idx] = v;
@@ -105,7 +108,8 @@ __global__ void __launch_bounds__(kThreads) covgen_rect_kernel(CovRectArgs a) {
 }
 
 void launch_covgen_tiles(const TileMap &map, const int2 *pairs, int npairs, const MaternParams &prm,
-                         const double *x, const double *y, const double *z, int64_t n, double *sig64, hipStream_t s) {
+                         const double *x, const double *y, const double *z, int64_t n, double *sig64, hipStream_t s,
+                         const MaternParams *prm_dev) {
     if (npairs <= 0) return;
     CovTilesArgs a;
     a.map = map;
@@ -115,6 +119,7 @@ void launch_covgen_tiles(const TileMap &map, const int2 *pairs, int npairs, cons
     a.n = n;
     a.three_d = z != nullptr;
     a.sig64 = sig64;
+    a.prm_dev = prm_dev;
     const int nsb = map.ts / kBlk;
     dim3 grid(nsb * nsb, npairs);
     hipLaunchKernelGGL(covgen_tiles_kernel, grid, dim3(kThreads), 0, s, a);

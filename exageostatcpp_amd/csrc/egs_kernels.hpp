@@ -14,8 +14,11 @@ namespace egs {
 // ---- covariance generation (egs_covgen.hip)
 // sig64 (may be null): also writes every generated tile in FP64 to
 // sig64 + (i*(i+1)/2 + j) * ts * ts, column-major.
+// prm_dev, when given, holds the parameters in device memory (read instead of
+// prm, so that a captured launch can be replayed for another theta).
 void launch_covgen_tiles(const TileMap &map, const int2 *pairs, int npairs, const MaternParams &prm,
-                         const double *x, const double *y, const double *z, int64_t n, double *sig64, hipStream_t s);
+                         const double *x, const double *y, const double *z, int64_t n, double *sig64, hipStream_t s,
+                         const MaternParams *prm_dev = nullptr);
 void launch_covgen_rect(double *out, int rows, int cols, int64_t row_off, int64_t col_off, const MaternParams &prm,
                         const double *x1, const double *y1, const double *z1, const double *x2, const double *y2,
                         const double *z2, hipStream_t s);

@@ -114,6 +114,14 @@ int egs_problem_set_single_product(egs_problem_t problem, int band, double tau);
  * tiles and their neighbours take FP64 products.  Negative: every tile of the
  * FP64 band does.  Kept across egs_problem_set_precision calls. */
 int egs_problem_set_fp64_product_band(egs_problem_t problem, int band);
+/* FP64-plan evaluations (egs_loglik) replayed from a hipGraph: the first call
+ * captures the launch sequence (observation copy, covariance, factorization,
+ * forward solve), later calls copy the new covariance parameters to the device
+ * and replay it, saving the per-launch host cost of small problems.  Results
+ * are bit-identical to the direct path.  Any change of plan drops the graph;
+ * profiling and mixed plans use the direct path.  Off by default; no
+ * counterpart in the reference. */
+int egs_problem_set_graph(egs_problem_t problem, int enable);
 /* Iterations and final relative FP64 residual of the last refinement. */
 int egs_last_refinement(egs_problem_t problem, int *iterations, double *residual);
 /* Block updates (192 x 192) of the tiles beyond the single-product band in the
