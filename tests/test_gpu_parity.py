@@ -230,6 +230,34 @@ def test_mspe_fixture(ctx, fixtures):
     assert abs(mspe - f["mspe"]) < f["rel_tol"] * f["mspe"]
 
 
+def test_kriging_at_40k_interpolates_and_is_linear(ctx):
+    """At the size of the reference's 40k configuration (200 x 200 grid, dts
+    960), where the oracle is too slow to run: kriging at observed locations
+    returns the observations (the interpolation identity), predictions scale
+    with the observations, and z' Sigma^-1 z scales with their square."""
+    n = 40000
+    th = [1.0, 0.1, 0.5]
+    x, y = egs.generate_locations(n, seed=0)
+    p = egs.Problem(ctx, 0, x, y, dts=960)
+    z = p.generate_observations(th, seed=0)
+    log("kriging n=40000: interpolation identity")
+    idx = np.arange(0, n, n // 64)[:64]
+    zm, _ = p.predict(th, x[idx], y[idx])
+    err = np.max(np.abs(zm - z[idx]))
+    log("  max |z_pred - z| at observed points %.3e (max |z| %.3f)" % (err, np.max(np.abs(z))))
+    assert err < 1e-8 * np.max(np.abs(z))
+    rng = np.random.default_rng(1)
+    xm, ym = rng.random(100), rng.random(100)
+    z1, _ = p.predict(th, xm, ym)
+    _, _, q1 = p.loglik(th)
+    p.set_observations(4.0 * z)
+    z4, _ = p.predict(th, xm, ym)
+    _, _, q4 = p.loglik(th)
+    assert np.max(np.abs(z4 - 4.0 * z1)) <= 1e-12 * np.max(np.abs(z4))
+    assert abs(q4 - 16.0 * q1) <= 1e-12 * abs(q4)
+    p.close()
+
+
 @pytest.mark.parametrize("n_obs,dts,theta", [(2000, 320, [1.0, 0.1, 1.0]), (1100, 960, [0.9, 0.05, 0.5])])
 def test_kriging_matches_oracle(ctx, n_obs, dts, theta):
     log("kriging n_obs=%d" % n_obs)
