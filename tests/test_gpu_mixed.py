@@ -189,3 +189,27 @@ def test_single_product_selection_pins_both_ends(ctx, data4k):
     assert tau_inf[1] != split_only[1]
     assert abs(dflt[0] - ref["llh"]) <= 1e-6 * abs(ref["llh"])
     p.close()
+
+
+def test_default_mixed_plan_at_100k_within_1e6_of_fp64(ctx):
+    """The benchmark workload at full size (n = 100,000, theta = (1, 0.05, 0.5),
+    dts 960): the default mixed plan's log-likelihood within 1e-6 relative of
+    the FP64 plan's, its quadratic form refined to FP64 accuracy, and the
+    power-of-two variance identity of the mixed plan at this size."""
+    th = [1.0, 0.05, 0.5]
+    x, y = egs.generate_locations(100000, seed=0)
+    p = egs.Problem(ctx, 0, x, y, dts=960)
+    p.generate_observations(th, seed=0)
+    log("n=100000: FP64 evaluation")
+    l64, d64, q64 = p.loglik(th)
+    p.set_precision(egs.PRECISION_MIXED)
+    log("n=100000: mixed evaluations")
+    lm, dm, qm = p.loglik(th)
+    rel = abs(lm - l64) / abs(l64)
+    log("  fp64 %.10f mixed %.10f rel %.2e quad rel %.2e" % (l64, lm, rel, abs(qm - q64) / q64))
+    assert rel <= 1e-6
+    assert abs(qm - q64) <= 1e-10 * q64
+    _, dc, qc = p.loglik([th[0] * 2.0 ** -10, th[1], th[2]])
+    assert abs((dc - dm) - 100000 * np.log(2.0 ** -10)) <= 1e-10 * abs(100000 * np.log(2.0 ** -10))
+    assert abs(qc * 2.0 ** -10 - qm) <= 1e-12 * qm
+    p.close()
