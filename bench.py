@@ -67,6 +67,13 @@ def operands(args):
     return "bf16" if args.plain_bf16 else "fp16-split"
 
 
+def product_band(args, bivariate):
+    """The FP64-product band in effect: the option, else the library default."""
+    if args.fp64_product_band is not None:
+        return args.fp64_product_band
+    return 2 if bivariate else 1
+
+
 def tau_json(args):
     """--single-tau for the JSON line (a string for infinity, which JSON lacks)."""
     return "inf" if args.single_tau == float("inf") else args.single_tau
@@ -85,7 +92,7 @@ def measured_traffic(kernel, args):
     same = (t.get("kernel") == kernel and t.get("n") == args.n and t.get("dts") == args.dts
             and t.get("band_fp64") == args.band_fp64 and t.get("band_fp32") == args.band_fp32
             and t.get("single_band") == args.single_band and t.get("single_tau") == tau_json(args)
-            and t.get("fp64_product_band", -1) == args.fp64_product_band
+            and t.get("fp64_product_band", -1) == product_band(args, bivariate=False)
             and t.get("operands") == operands(args))
     return t["fetch_bytes_per_launch"] + t["write_bytes_per_launch"] if same else None
 
@@ -225,11 +232,13 @@ def bivariate_gcd(egs, args):
     fp64 = timed(egs, ctx, prob, theta, args.steps, args.warmup, "bivariate fp64")
     out["fp64"] = {"evals_per_s": fp64["evals_per_s"], "ms_per_step": fp64["ms_per_step"], "loglik": fp64["loglik"],
                    "tflops": (2 * n_loc) ** 3 / 3.0 / (fp64["ms_per_step"] * 1e-3) / 1e12}
+    out["fp64_product_band"] = product_band(args, bivariate=True)
     out["mixed"] = []
     for b64, b32 in ((1, 1), (2, 2), (4, 4), (8, 8)):
         prob.set_precision(egs.PRECISION_MIXED, b64, b32)
         prob.set_single_product(args.single_band, args.single_tau)
-        prob.set_fp64_product_band(args.fp64_product_band)
+        if args.fp64_product_band is not None:
+            prob.set_fp64_product_band(args.fp64_product_band)
         try:
             mixed = timed(egs, ctx, prob, theta, args.steps, args.warmup, "bivariate mixed (%d, %d)" % (b64, b32))
         except egs.NotPositiveDefinite as e:
@@ -270,7 +279,8 @@ def mle_compare(egs, args):
             fp64_prob.set_observations(prob.get_observations())
             prob.set_precision(mode, args.band_fp64, args.band_fp32)
             prob.set_single_product(args.single_band, args.single_tau)
-            prob.set_fp64_product_band(args.fp64_product_band)
+            if args.fp64_product_band is not None:
+                prob.set_fp64_product_band(args.fp64_product_band)
         count, fallback = [0], [0]
 
         def objective(v):
@@ -295,7 +305,8 @@ def mle_compare(egs, args):
     out["max_abs_theta_diff"] = max(d)
     out["fp64_loglik_at_mixed_theta_hat"] = fp64_prob.loglik(out["mixed"]["theta_hat"])[0]
     out["mixed_plan"] = {"band_fp64": args.band_fp64, "band_fp32": args.band_fp32, "single_band": args.single_band,
-                         "single_tau": tau_json(args), "fp64_product_band": args.fp64_product_band}
+                         "single_tau": tau_json(args),
+                         "fp64_product_band": product_band(args, bivariate=False)}
     fp64_prob.close()
     prob.close()
     ctx.close()
@@ -315,9 +326,9 @@ def main():
     ap.add_argument("--single-band", type=int, default=0,
                     help="blocks of tiles beyond this band may take one FP16 product instead of the split three "
                          "(negative: never)")
-    ap.add_argument("--fp64-product-band", type=int, default=1,
+    ap.add_argument("--fp64-product-band", type=int, default=None,
                     help="FP64-band tiles beyond this band take split FP16 products into FP64 storage "
-                         "(negative: none)")
+                         "(negative: none; default: the library's, 1 for univariate and 2 for bivariate kernels)")
     ap.add_argument("--single-tau", type=float, default=2.0 ** -16,
                     help="... when their panel rows' scales multiply to <= tau times the panel's largest squared")
     ap.add_argument("--plain-bf16", action="store_true", help="single BF16 product instead of the split one")
@@ -379,7 +390,8 @@ def main():
     mode = egs.PRECISION_MIXED_BF16 if args.plain_bf16 else egs.PRECISION_MIXED
     prob.set_precision(mode, args.band_fp64, args.band_fp32)
     prob.set_single_product(args.single_band, args.single_tau)
-    prob.set_fp64_product_band(args.fp64_product_band)
+    if args.fp64_product_band is not None:
+        prob.set_fp64_product_band(args.fp64_product_band)
     mixed = timed(egs, ctx, prob, theta, args.steps, args.warmup, "mixed")
     rel_err = abs(mixed["loglik"] - fp64["loglik"]) / abs(fp64["loglik"])
 
@@ -421,7 +433,7 @@ def main():
                                        % (args.single_band, args.single_tau))),
                    "n": args.n, "dts": args.dts, "band_fp64": args.band_fp64, "band_fp32": args.band_fp32,
                    "single_band": args.single_band, "single_tau": tau_json(args),
-                   "fp64_product_band": args.fp64_product_band,
+                   "fp64_product_band": product_band(args, bivariate=False),
                    "operands": operands(args)},
         "roofline": {
             "bound": "mfma",

@@ -93,9 +93,10 @@ struct egs_problem_s {
     // 2^-24 K pmax^2, the split product's for the panel's largest block.
     int band_single = 0;
     // FP64 tiles with tile distance > band_prod64 (within band64) take split
-    // FP16 products into their FP64 storage instead of FP64 products.  1: the
-    // diagonal tiles and their neighbours take FP64 products (0, only the
-    // diagonal, put 2.5e-6 into the bivariate 25k likelihood)
+    // FP16 products into their FP64 storage instead of FP64 products.  Set at
+    // creation: 1 for univariate kernels (the diagonal tiles and their
+    // neighbours), 2 for the bivariate kernel, whose smoother second field is
+    // more sensitive (band 1 measured 1.2e-6 relative there under plan (2, 2))
     int band_prod64 = 1;
     double tau_single = 0x1p-16;
     // mixed plans: per-class update lists (descending column) and, for class c,
@@ -963,6 +964,7 @@ int egs_problem_create(egs_context_t ctx, int kernel, int64_t n_loc, const doubl
     auto *p = new egs_problem_s();
     p->ctx = ctx;
     p->kernel = kernel;
+    p->band_prod64 = kernel == EGS_KERNEL_BIVARIATE_MATERN_PARSIMONIOUS ? 2 : 1;
     p->p = kernel == EGS_KERNEL_BIVARIATE_MATERN_PARSIMONIOUS ? 2 : 1;
     p->metric = metric;
     p->dim = (z && kernel == EGS_KERNEL_UNIVARIATE_MATERN_STATIONARY) ? 3 : 2;
