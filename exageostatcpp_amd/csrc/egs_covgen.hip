@@ -25,39 +25,41 @@ struct CovTilesArgs {
     const MaternParams *prm_dev;   // optional: read the parameters here instead of prm
 };
 
+// One workgroup per 64-row strip of a tile (all ts columns): the strip's row
+// locations and the tile's column locations (3 ts doubles of dynamic LDS) are
+// staged once, then each wave writes 64 consecutive rows (512 bytes in FP64)
+// of one column at a time.
+
 __global__ void __launch_bounds__(kThreads) covgen_tiles_kernel(CovTilesArgs a) {
     __shared__ double s_rx[kBlk], s_ry[kBlk], s_rz[kBlk];
-    __shared__ double s_cx[kBlk], s_cy[kBlk], s_cz[kBlk];
+    extern __shared__ double s_cols[];
     // parameters from device memory when given (a captured graph replays with new theta)
     const MaternParams prm = a.prm_dev ? *a.prm_dev : a.prm;
     const int ts = a.map.ts;
-    const int nsb = ts / kBlk;
+    double *s_cx = s_cols, *s_cy = s_cols + ts, *s_cz = s_cols + 2 * ts;
     const int2 ij = a.pairs[blockIdx.y];
-    const int sbr = blockIdx.x % nsb;
-    const int sbc = blockIdx.x / nsb;
+    const int sbr = blockIdx.x;
     const int64_t r0 = (int64_t) ij.x * ts + sbr * kBlk;
-    const int64_t c0 = (int64_t) ij.y * ts + sbc * kBlk;
+    const int64_t c0 = (int64_t) ij.y * ts;
     const int t = threadIdx.x;
     const bool three_d = a.three_d != 0;
     const int shift = prm.kernel == 1 ? 1 : 0;
-    const int64_t nloc = a.n >> shift;
 
-    // Stage the locations of this block's rows and columns.
+    // Stage the locations of this strip's rows and of the tile's columns.
     if (t < kBlk) {
         const int64_t li = (r0 + t) >> shift;
         const bool ok = (r0 + t) < a.n;
         s_rx[t] = ok ? a.loc.x[li] : 0.0;
         s_ry[t] = ok ? a.loc.y[li] : 0.0;
         s_rz[t] = (ok && three_d) ? a.loc.z[li] : 0.0;
-    } else if (t < 2 * kBlk) {
-        const int u = t - kBlk;
+    }
+    for (int u = t; u < ts; u += kThreads) {
         const int64_t lj = (c0 + u) >> shift;
         const bool ok = (c0 + u) < a.n;
         s_cx[u] = ok ? a.loc.x[lj] : 0.0;
         s_cy[u] = ok ? a.loc.y[lj] : 0.0;
         s_cz[u] = (ok && three_d) ? a.loc.z[lj] : 0.0;
     }
-    (void) nloc;
     __syncthreads();
 
     const int r = t % kBlk;
@@ -68,7 +70,7 @@ __global__ void __launch_bounds__(kThreads) covgen_tiles_kernel(CovTilesArgs a) 
     const uint8_t store = a.map.store[ij.x * a.map.nt + ij.y];
     char *tile = a.map.base + a.map.off[ij.x * a.map.nt + ij.y];
     const int64_t sig_base = ((int64_t) ij.x * (ij.x + 1) / 2 + ij.y) * ts * ts;
-    for (int cc = t / kBlk; cc < kBlk; cc += kThreads / kBlk) {
+    for (int cc = t / kBlk; cc < ts; cc += kThreads / kBlk) {
         const int64_t gc = c0 + cc;
         double v;
         if (row_pad || gc >= a.n) {
@@ -79,7 +81,7 @@ __global__ void __launch_bounds__(kThreads) covgen_tiles_kernel(CovTilesArgs a) 
             const double d = dist / prm.range;
             v = (d == 0.0) ? prm.sig[pair] : matern_scaled(d, prm.con[pair], prm.sig[pair], prm.nu[pair]);
         }
-        const int64_t idx = (int64_t) (sbr * kBlk + r) + (int64_t) (sbc * kBlk + cc) * ts;
+        const int64_t idx = (int64_t) (sbr * kBlk + r) + (int64_t) cc * ts;
         if (store == kStoreF64) reinterpret_cast<double *>(tile)[idx] = v;
         else reinterpret_cast<float *>(tile)[idx] = (float) v;
         if (a.sig64) a.sig64[sig_base + idx] = v;
@@ -121,8 +123,8 @@ void launch_covgen_tiles(const TileMap &map, const int2 *pairs, int npairs, cons
     a.sig64 = sig64;
     a.prm_dev = prm_dev;
     const int nsb = map.ts / kBlk;
-    dim3 grid(nsb * nsb, npairs);
-    hipLaunchKernelGGL(covgen_tiles_kernel, grid, dim3(kThreads), 0, s, a);
+    dim3 grid(nsb, npairs);
+    hipLaunchKernelGGL(covgen_tiles_kernel, grid, dim3(kThreads), 3 * sizeof(double) * (size_t) map.ts, s, a);
 }
 
 void launch_covgen_rect(double *out, int rows, int cols, int64_t row_off, int64_t col_off, const MaternParams &prm,
