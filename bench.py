@@ -8,9 +8,12 @@ stream, seed 0), dense tile size 960.  Each step regenerates Sigma(theta) in
 HBM, factors it, and reduces log|Sigma| and z' Sigma^-1 z; inputs are resident
 in HBM when the timed region starts.
 
-The headline value is the mixed-precision plan (tiles within band 1 of the
-diagonal FP64, the rest FP32 storage updated with row-scaled split-FP16
-products accumulated in FP32, z' Sigma^-1 z refined to an FP64 residual <= 1e-12).  The same data is first
+The headline value is the library's default mixed-precision plan: tiles within
+band 4 of the diagonal stored in FP64 (FP64 products on the diagonal tiles,
+row-scaled split-FP16 products on the others), the rest stored in FP32 and
+updated per 192 x 192 block with the split-FP16 product or, where the panel
+rows are small, one FP16 product, accumulated in FP32; z' Sigma^-1 z refined
+to an FP64 residual <= 1e-12.  The same data is first
 timed with the FP64 plan; the line reports both rates, the speedup and the
 relative log-likelihood error of the mixed plan against FP64.
 
@@ -71,7 +74,7 @@ def product_band(args, bivariate):
     """The FP64-product band in effect: the option, else the library default."""
     if args.fp64_product_band is not None:
         return args.fp64_product_band
-    return 2 if bivariate else 1
+    return 2 if bivariate else 0
 
 
 def tau_json(args):

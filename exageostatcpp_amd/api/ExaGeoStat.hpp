@@ -71,7 +71,8 @@ public:
     // Defaults 4 and 4: no FP32 band, which measured both faster and at least as
     // accurate as an FP32 band at n = 100k.  FP64 storage up to band 4 keeps
     // the 40k fit's estimate within 1e-4 of the FP64 one (band 1: 1.7e-3);
-    // only the diagonal tiles and their neighbours take FP64 products.
+    // only the diagonal tiles take FP64 products (two bands of tiles for the
+    // bivariate kernel).
     void SetDenseBandDP(int aBand) { mBandDP = aBand; }
     int GetDenseBandDP() const { return mBandDP; }
     void SetBandDenseSP(int aBand) { mBandSP = aBand; }

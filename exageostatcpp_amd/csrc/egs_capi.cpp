@@ -94,10 +94,10 @@ struct egs_problem_s {
     int band_single = 0;
     // FP64 tiles with tile distance > band_prod64 (within band64) take split
     // FP16 products into their FP64 storage instead of FP64 products.  Set at
-    // creation: 1 for univariate kernels (the diagonal tiles and their
-    // neighbours), 2 for the bivariate kernel, whose smoother second field is
-    // more sensitive (band 1 measured 1.2e-6 relative there under plan (2, 2))
-    int band_prod64 = 1;
+    // creation: 0 for univariate kernels (the diagonal tiles only), 2 for the
+    // bivariate kernel, whose smoother second field is more sensitive (band 1
+    // measured 1.2e-6 relative there under plan (2, 2), band 0 2.5e-6)
+    int band_prod64 = 0;
     double tau_single = 0x1p-16;
     // mixed plans: per-class update lists (descending column) and, for class c,
     // cls_count[c][j] = number of its tiles in columns >= j
@@ -560,10 +560,14 @@ double update_flops_cls(const egs_problem_s *p, int c, int j0, int j1) {
     return f;
 }
 
+// FP64 updates of at most this many tiles on the critical stream use 64 x 64
+// workgroups: 225 per tile, each done in a ninth of the time of a 192 x 192 one.
+constexpr int kShortUpdate = 8;
+
 // Updates of step k under a mixed plan for tile columns [j0, nt) restricted to
 // the column range encoded by the class prefix counts: entries [lo_c, hi_c) of
-// each class list.
-void launch_mixed_update(egs_problem_s *p, int k, int jcol_lo, int jcol_hi, hipStream_t st) {
+// each class list.  critical: the launches are on the critical path.
+void launch_mixed_update(egs_problem_s *p, int k, int jcol_lo, int jcol_hi, hipStream_t st, bool critical = false) {
     const egs::TileMap m = p->map();
     static const int prof_id[kClasses] = {EGS_PROFILE_UPDATE, EGS_PROFILE_UPDATE_F32, EGS_PROFILE_UPDATE_16,
                                           EGS_PROFILE_UPDATE_16_SELECT, EGS_PROFILE_UPDATE_16};
@@ -575,7 +579,9 @@ void launch_mixed_update(egs_problem_s *p, int k, int jcol_lo, int jcol_hi, hipS
         const int pi = prof_begin(p, prof_id[c], update_flops_cls(p, c, jcol_lo, jcol_hi), st);
         const int2 *pr = p->d_pairs_cls[c] + beg;
         const int slot = k % kPanelSlots;
-        if (c == 0) egs::launch_update192_panel(m, p->d_p64[slot], pr, end - beg, k, st);
+        if (c == 0 && critical && end - beg <= kShortUpdate)
+            egs::launch_update64(m, p->d_p64[slot], pr, end - beg, k, st);
+        else if (c == 0) egs::launch_update192_panel(m, p->d_p64[slot], pr, end - beg, k, st);
         else if (c == 1) egs::launch_update192_f32(m, p->d_p32[slot], pr, end - beg, k, st);
         else {
             const egs::Operands op =
@@ -646,10 +652,11 @@ int run_potrf(egs_problem_s *p) {
         }
         if (k > 0) EGS_HIP(hipStreamWaitEvent(s, p->ev_bulk[k - 1], 0));
         if (is_mixed(p)) {
-            launch_mixed_update(p, k, k + 1, k + 2, s);
+            launch_mixed_update(p, k, k + 1, k + 2, s, true);
         } else {
             const int pi = prof_begin(p, EGS_PROFILE_UPDATE, update_flops(p, k + 1, k + 2), s);
-            egs::launch_update(m, p->d_pairs + n_beyond, n_next, k, s);
+            if (n_next <= kShortUpdate) egs::launch_update64(m, nullptr, p->d_pairs + n_beyond, n_next, k, s);
+            else egs::launch_update(m, p->d_pairs + n_beyond, n_next, k, s);
             prof_end(p, pi, s);
         }
         factor_panel(p, k + 1);
@@ -964,7 +971,7 @@ int egs_problem_create(egs_context_t ctx, int kernel, int64_t n_loc, const doubl
     auto *p = new egs_problem_s();
     p->ctx = ctx;
     p->kernel = kernel;
-    p->band_prod64 = kernel == EGS_KERNEL_BIVARIATE_MATERN_PARSIMONIOUS ? 2 : 1;
+    p->band_prod64 = kernel == EGS_KERNEL_BIVARIATE_MATERN_PARSIMONIOUS ? 2 : 0;
     p->p = kernel == EGS_KERNEL_BIVARIATE_MATERN_PARSIMONIOUS ? 2 : 1;
     p->metric = metric;
     p->dim = (z && kernel == EGS_KERNEL_UNIVARIATE_MATERN_STATIONARY) ? 3 : 2;

@@ -130,8 +130,11 @@ __device__ __forceinline__ void acc_foreach(const Acc64 &acc, F f) {
 template <typename TX>
 __global__ void __launch_bounds__(kThreads) trsm_rows_kernel(TileMap map, int k, int64_t g0, int jb0, int jb1,
                                                              const double *winv_k) {
+    // 66.5 KB of LDS: T is formed after each GEMM through sm (it reuses sm.a),
+    // so that a workgroup fits beside one workgroup of the trailing update
     __shared__ GemmSmem sm;
-    __shared__ double sT[kBlk][kLdsPad];
+    static_assert(sizeof(GemmSmem::a) >= sizeof(double) * kBlk * kLdsPad, "T must fit in the staging buffer");
+    double (*sT)[kLdsPad] = reinterpret_cast<double (*)[kLdsPad]>(&sm.a[0][0][0]);
     const int ts = map.ts;
     const int nsb = ts / kBlk;
     const int64_t g = g0 + blockIdx.x;
@@ -174,7 +177,9 @@ __global__ void __launch_bounds__(kThreads) trsm_rows_kernel(TileMap map, int k,
 // ------------------------------------------------------------------------
 // Trailing update, FP64.
 // ------------------------------------------------------------------------
-__global__ void __launch_bounds__(kThreads) update_kernel(TileMap map, const int2 *pairs, int k) {
+// p64 (may be null): read panel k from its contiguous FP64 copy (tile i at
+// p64 + (i - k - 1) ts^2) instead of from the tiles.
+__global__ void __launch_bounds__(kThreads) update_kernel(TileMap map, const double *p64, const int2 *pairs, int k) {
     __shared__ GemmSmem sm;
     const int ts = map.ts;
     const int nsb = ts / kBlk;
@@ -183,8 +188,9 @@ __global__ void __launch_bounds__(kThreads) update_kernel(TileMap map, const int
     const int sbn = blockIdx.x / nsb;
     if (ij.x == ij.y && sbm < sbn) return;
     double *C = tile_f64(map, ij.x, ij.y) + sbm * kBlk + (int64_t) sbn * kBlk * ts;
-    const double *A = tile_f64(map, ij.x, k) + sbm * kBlk;
-    const double *B = tile_f64(map, ij.y, k) + sbn * kBlk;
+    const int64_t tt = (int64_t) ts * ts;
+    const double *A = (p64 ? p64 + (ij.x - k - 1) * tt : tile_f64(map, ij.x, k)) + sbm * kBlk;
+    const double *B = (p64 ? p64 + (ij.y - k - 1) * tt : tile_f64(map, ij.y, k)) + sbn * kBlk;
     Acc64 acc;
     acc_zero(acc);
     wg_gemm_nt(acc, A, ts, B, ts, ts, sm);
@@ -211,8 +217,13 @@ void launch_update(const TileMap &map, const int2 *pairs, int npairs, int k, hip
         launch_update192(map, pairs, npairs, k, s);
         return;
     }
+    launch_update64(map, nullptr, pairs, npairs, k, s);
+}
+
+void launch_update64(const TileMap &map, const double *p64, const int2 *pairs, int npairs, int k, hipStream_t s) {
+    if (npairs <= 0) return;
     const int nsb = map.ts / kBlk;
-    hipLaunchKernelGGL(update_kernel, dim3(nsb * nsb, npairs), dim3(kThreads), 0, s, map, pairs, k);
+    hipLaunchKernelGGL(update_kernel, dim3(nsb * nsb, npairs), dim3(kThreads), 0, s, map, p64, pairs, k);
 }
 
 // ------------------------------------------------------------------------
@@ -285,128 +296,79 @@ void launch_tinv(const TileMap &map, const double *winv, double *tinv, hipStream
 //                       from the scratch copy into the tile.
 // ------------------------------------------------------------------------
 namespace {
-__device__ __forceinline__ double pick4(const double (&d)[4], int q) {
-    return q == 0 ? d[0] : (q == 1 ? d[1] : (q == 2 ? d[2] : d[3]));
+// value of v in lane l (l uniform across the wave)
+__device__ __forceinline__ double readlane_f64(double v, int l) {
+    const int lo = __builtin_amdgcn_readlane(__double2loint(v), l);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), l);
+    return __hiloint2double(hi, lo);
 }
 }  // namespace
 
+// Block (jb, jb) of diagonal tile k: L = chol(A) in place and W = L^{-1} into
+// winv_k, both lower (zeros above the diagonal).  The block lives in registers:
+// thread (w, i) = (threadIdx.x / 64, lane) holds row i of columns c = w + 4 q,
+// q < 16, in y[q]: column c of A until step c, then column c of X, which
+// becomes L^{-1}.  Column step j (64 steps, one barrier each) is the
+// right-looking Cholesky step A_ic -= L_ij L_cj (c > j) fused with the
+// column-oriented inversion X_jc /= L_jj, X_ic -= L_ij X_jc (i > j, c <= j).
+// X is kept with its rows unscaled (row j of X times L_jj); the rows are
+// scaled once at the end.  Both updates are then y_ic -= r_i g_c with
+// r_i = A_ij / A_jj (i > j, else 0) and g_c the uniform operand: A_cj from LDS
+// (column j, published by its owner wave at the end of step j - 1, double
+// buffered) for c > j, the unscaled X_jc read from lane j for c <= j.  At step
+// j the owner wave of column j stores L's column j and restarts its register
+// as column j of X.
 __global__ void __launch_bounds__(kThreads) potrf64_kernel(TileMap map, int k, int jb, double *winv_k, int *info,
                                                            int64_t n) {
-    __shared__ double sA[kBlk][kLdsPad];
-    __shared__ double sW[kBlk][kLdsPad];
-    __shared__ double sTmp[3 * 16 * 16 + 48 * 16];
+    constexpr int kCols = kBlk / 4;   // columns per thread
+    __shared__ double s_col[2][kBlk];
+    __shared__ double s_rs[kBlk];     // 1 / L_jj
     const int ts = map.ts;
     const int t = threadIdx.x;
-    const int lane = t & 63, w = t >> 6;
+    const int i = t & 63;
+    const int w = __builtin_amdgcn_readfirstlane(t >> 6);   // wave index, held in a scalar register
     double *Ajj = tile_f64(map, k, k) + (int64_t) jb * kBlk + (int64_t) jb * kBlk * ts;
-    for (int e = t; e < kBlk * kBlk; e += kThreads) {
-        const int r = e & 63, c = e >> 6;
-        sA[r][c] = Ajj[r + (int64_t) c * ts];
-        sW[r][c] = 0.0;
-    }
+    double y[kCols];
+#pragma unroll
+    for (int q = 0; q < kCols; ++q) y[q] = Ajj[i + (int64_t) (w + 4 * q) * ts];
+    if (w == 0) s_col[0][i] = y[0];
     __syncthreads();
-    for (int b = 0; b < 4; ++b) {
-        const int o = 16 * b;
-        // (1) 16 x 16 diagonal block in the registers of wave 0:
-        //     lane l holds row i = l % 16, columns 4 (l / 16) .. 4 (l / 16) + 3
-        if (w == 0) {
-            const int i = lane & 15, jq = lane >> 4;
-            double d[4];
-#pragma unroll
-            for (int q = 0; q < 4; ++q) d[q] = sA[o + i][o + 4 * jq + q];
-#pragma unroll
-            for (int j = 0; j < 16; ++j) {
-                const int jg = j >> 2, js = j & 3;
-                const double piv = __shfl(pick4(d, js), j + 16 * jg, 64);
-                if (lane == 0 && !(piv > 0.0)) {
-                    const int64_t col = (int64_t) k * ts + (int64_t) jb * kBlk + o + j;
-                    if (col < n) atomicCAS(info, 0, (int) (col + 1));
-                }
-                const double sq = sqrt(piv);
-                const double inv = 1.0 / sq;
-                if (jq == jg) {
-                    if (i == j) d[js] = sq;
-                    else if (i > j) d[js] *= inv;
-                }
-                const double lij = __shfl(pick4(d, js), i + 16 * jg, 64);
-#pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    const int c = 4 * jq + q;
-                    const double lcj = __shfl(pick4(d, js), c + 16 * jg, 64);
-                    if (c > j && i >= c) d[q] -= lij * lcj;
-                }
-            }
-#pragma unroll
-            for (int q = 0; q < 4; ++q) sA[o + i][o + 4 * jq + q] = d[q];
+    for (int j = 0; j < kBlk; ++j) {
+        const int buf = j & 1;
+        const double piv = s_col[buf][j];
+        if (t == 0 && !(piv > 0.0)) {
+            const int64_t col = (int64_t) k * ts + (int64_t) jb * kBlk + j;
+            if (col < n) atomicCAS(info, 0, (int) (col + 1));
         }
-        __syncthreads();
-        // (2) inverse of the 16 x 16 factor: thread c < 16 solves column c,
-        //     keeping the column in registers
-        if (t < 16) {
-            const int c = t;
-            double wcol[16];
+        const double sq = sqrt(piv);
+        const double rs = 1.0 / sq;
+        if (t == 0) s_rs[j] = rs;
+        const double lo = i > j ? s_col[buf][i] * rs : 0.0;   // L_ij below the diagonal
+        const double r = lo * rs;
+        // column j of the unscaled X after its own step
+        const double xj0 = i == j ? 1.0 : -r;
+        // the LDS operands of every column at once: one wait for all of them
+        double gl[kCols];
 #pragma unroll
-            for (int r = 0; r < 16; ++r) {
-                double acc = (r == c) ? 1.0 : 0.0;
+        for (int q = 0; q < kCols; ++q) gl[q] = s_col[buf][w + 4 * q];
 #pragma unroll
-                for (int l = 0; l < r; ++l)
-                    if (l >= c) acc -= sA[o + r][o + l] * wcol[l];
-                wcol[r] = (r < c) ? 0.0 : acc / sA[o + r][o + r];
-            }
-#pragma unroll
-            for (int r = 0; r < 16; ++r) sW[o + r][o + c] = wcol[r];
-        }
-        __syncthreads();
-        const int m = 48 - o;   // rows below the 16 x 16 block
-        if (m > 0) {
-            // (3) L[r][o + c] = sum_{l <= c} A[r][o + l] Dinv[c][l] for the rows below
-            for (int e = t; e < m * 16; e += kThreads) {
-                const int r = o + 16 + (e >> 4), c = e & 15;
-                double acc = 0.0;
-                for (int l = 0; l <= c; ++l) acc += sA[r][o + l] * sW[o + c][o + l];
-                sTmp[e] = acc;
-            }
-            __syncthreads();
-            for (int e = t; e < m * 16; e += kThreads) sA[o + 16 + (e >> 4)][o + (e & 15)] = sTmp[e];
-            __syncthreads();
-            // (4) trailing lower part: A[r][c] -= sum_l L[r][o + l] L[c][o + l]
-            for (int e = t; e < m * m; e += kThreads) {
-                const int r = o + 16 + e % m, c = o + 16 + e / m;
-                if (c > r) continue;
-                double acc = 0.0;
-#pragma unroll
-                for (int l = 0; l < 16; ++l) acc += sA[r][o + l] * sA[c][o + l];
-                sA[r][c] -= acc;
-            }
-            __syncthreads();
-        }
-    }
-    // off-diagonal 16 x 16 blocks of W = L^{-1}, by distance d = rB - cB:
-    // W_rc = -Dinv_r sum_{l = c}^{r - 1} L_rl W_lc
-    for (int d = 1; d < 4; ++d) {
-        const int nb = 4 - d;
-        for (int e = t; e < nb * 256; e += kThreads) {
-            const int cB = e >> 8, rB = cB + d;
-            const int rr = e & 15, cc = (e >> 4) & 15;
-            double acc = 0.0;
-            for (int l = 16 * cB; l < 16 * rB; ++l) acc += sA[16 * rB + rr][l] * sW[l][16 * cB + cc];
-            sTmp[e] = acc;
-        }
-        __syncthreads();
-        for (int e = t; e < nb * 256; e += kThreads) {
-            const int cB = e >> 8, rB = cB + d;
-            const int rr = e & 15, cc = (e >> 4) & 15;
-            double acc = 0.0;
-            for (int l = 0; l <= rr; ++l) acc += sW[16 * rB + rr][16 * rB + l] * sTmp[(cB << 8) + (cc << 4) + l];
-            sW[16 * rB + rr][16 * cB + cc] = -acc;
+        for (int q = 0; q < kCols; ++q) {
+            const int c = w + 4 * q;   // uniform across the wave
+            double g = gl[q];
+            if (c <= j) g = readlane_f64(y[q], j);
+            const double upd = fma(-r, g, y[q]);
+            if (c == j) Ajj[i + (int64_t) c * ts] = i > j ? lo : (i == j ? sq : 0.0);
+            y[q] = c == j ? xj0 : upd;
+            if (c == j + 1) s_col[buf ^ 1][i] = upd;
         }
         __syncthreads();
     }
     double *W = winv_k + (int64_t) jb * kBlk * kBlk;
-    for (int e = t; e < kBlk * kBlk; e += kThreads) {
-        const int r = e & 63, c = e >> 6;
-        Ajj[r + (int64_t) c * ts] = (r >= c) ? sA[r][c] : 0.0;
-        W[r + c * kBlk] = (r >= c) ? sW[r][c] : 0.0;
+    const double rsi = s_rs[i];
+#pragma unroll
+    for (int q = 0; q < kCols; ++q) {
+        const int c = w + 4 * q;
+        W[i + c * kBlk] = i >= c ? y[q] * rsi : 0.0;
     }
 }
 
@@ -418,9 +380,14 @@ __device__ __forceinline__ void times_wt(Acc64 &acc, const double *X, int64_t ld
 
 __global__ void __launch_bounds__(kThreads) diag_update_kernel(TileMap map, int k, int jb, const double *winv_k,
                                                                double *scratch) {
+    // 66.5 KB of LDS: L_r and L_c stay in registers until both GEMMs are done,
+    // then go through sm (sm.a and sm.b), so that a workgroup fits beside one
+    // workgroup of the trailing update
     __shared__ GemmSmem sm;
-    __shared__ double sLr[kBlk][kLdsPad];
-    __shared__ double sLc[kBlk][kLdsPad];
+    static_assert(sizeof(GemmSmem::a) >= sizeof(double) * kBlk * kLdsPad, "L_r must fit in the staging buffer");
+    static_assert(sizeof(GemmSmem::b) >= sizeof(double) * kBlk * kLdsPad, "L_c must fit in the staging buffer");
+    double (*sLr)[kLdsPad] = reinterpret_cast<double (*)[kLdsPad]>(&sm.a[0][0][0]);
+    double (*sLc)[kLdsPad] = reinterpret_cast<double (*)[kLdsPad]>(&sm.b[0][0][0]);
     const int ts = map.ts;
     const int nsb = ts / kBlk;
     // blockIdx.x enumerates the pairs jb < c <= r < nsb column by column
@@ -435,13 +402,12 @@ __global__ void __launch_bounds__(kThreads) diag_update_kernel(TileMap map, int 
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int mq = wv & 1, nq = wv >> 1;
     const int lr = lane & 15, lk = lane >> 4;
-    Acc64 acc;
-    times_wt(acc, T + r * kBlk + (int64_t) jb * kBlk * ts, ts, W, sm);
-    acc_foreach(acc, [&](int rr, int cc, double v) { sLr[rr][cc] = v; });
-    if (c != r) {
-        times_wt(acc, T + c * kBlk + (int64_t) jb * kBlk * ts, ts, W, sm);
-        acc_foreach(acc, [&](int rr, int cc, double v) { sLc[rr][cc] = v; });
-    }
+    Acc64 acc_r, acc_c;
+    times_wt(acc_r, T + r * kBlk + (int64_t) jb * kBlk * ts, ts, W, sm);
+    if (c != r) times_wt(acc_c, T + c * kBlk + (int64_t) jb * kBlk * ts, ts, W, sm);
+    // the GEMMs end with a barrier: sm is free
+    acc_foreach(acc_r, [&](int rr, int cc, double v) { sLr[rr][cc] = v; });
+    if (c != r) acc_foreach(acc_c, [&](int rr, int cc, double v) { sLc[rr][cc] = v; });
     __syncthreads();
     const double (*Lc)[kLdsPad] = (c != r) ? sLc : sLr;
     // A_rc -= L_r L_c^T

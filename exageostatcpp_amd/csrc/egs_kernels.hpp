@@ -32,6 +32,10 @@ void launch_trsm_rows(const TileMap &map, int k, int64_t g0, int count, int jb0,
 // Trailing update C_ij -= L_ik L_jk^T for the first npairs lower tiles of the
 // update order (all (i, j) with i >= j > k).
 void launch_update(const TileMap &map, const int2 *pairs, int npairs, int k, hipStream_t s);
+// The 64 x 64-block update whatever ts (more, shorter workgroups than the
+// 192 x 192 form: lower latency for a few tiles); p64 as launch_update192_panel
+// when given, else panel k is read from the tiles.
+void launch_update64(const TileMap &map, const double *p64, const int2 *pairs, int npairs, int k, hipStream_t s);
 // 192 x 192-block variant of the update (egs_update.hip), used when ts % 192 == 0.
 bool update192_supported(int ts);
 void launch_update192(const TileMap &map, const int2 *pairs, int npairs, int k, hipStream_t s);

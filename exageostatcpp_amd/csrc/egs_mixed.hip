@@ -56,19 +56,30 @@ __global__ void __launch_bounds__(kThreads) panel_rowscale_kernel(TileMap map, i
     const int t = threadIdx.x;
     const int r = t & 63, g = t >> 6;
     const bool f32 = map.store[i * map.nt + k] == kStoreF32;
+    // 48 independent loads in flight per thread (3 rows x 16 columns): a
+    // column-at-a-time loop waits one memory latency per element
+    constexpr int kRows = kBM / 64, kCols = 16;
+    float m[kRows];
 #pragma unroll
-    for (int rs = 0; rs < kBM / 64; ++rs) {
-        const int64_t row = (int64_t) b * kBM + rs * 64 + r;
-        float m = 0.0f;
-        if (f32) {
-            const float *A = tile_f32(map, i, k) + row;
-            for (int c = g; c < ts; c += 4) m = fmaxf(m, fabsf(A[(int64_t) c * ts]));
-        } else {
-            const double *A = tile_f64(map, i, k) + row;
-            for (int c = g; c < ts; c += 4) m = fmaxf(m, fabsf((float) A[(int64_t) c * ts]));
+    for (int rs = 0; rs < kRows; ++rs) m[rs] = 0.0f;
+    auto scan = [&](const auto *A) {
+        for (int c0 = g; c0 < ts; c0 += 4 * kCols) {   // ts is a multiple of 64
+            float v[kRows][kCols];
+#pragma unroll
+            for (int rs = 0; rs < kRows; ++rs)
+#pragma unroll
+                for (int u = 0; u < kCols; ++u)
+                    v[rs][u] = (float) A[(int64_t) b * kBM + rs * 64 + r + (int64_t) (c0 + 4 * u) * ts];
+#pragma unroll
+            for (int rs = 0; rs < kRows; ++rs)
+#pragma unroll
+                for (int u = 0; u < kCols; ++u) m[rs] = fmaxf(m[rs], fabsf(v[rs][u]));
         }
-        sm[g][rs * 64 + r] = m;
-    }
+    };
+    if (f32) scan(tile_f32(map, i, k));
+    else scan(tile_f64(map, i, k));
+#pragma unroll
+    for (int rs = 0; rs < kRows; ++rs) sm[g][rs * 64 + r] = m[rs];
     __syncthreads();
     float mag = 0.0f;
     if (t < kBM) {
