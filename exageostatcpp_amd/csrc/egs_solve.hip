@@ -585,16 +585,13 @@ __global__ void __launch_bounds__(kThreads) fwd_diag_inv_kernel(const double *__
 // partial sums per lane over the columns of its wave.
 template <typename TA>
 __device__ __forceinline__ double panel_row_dot(const TA *__restrict__ A, const double *sy, int ts, int w) {
-    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
-    int c = w;
-    for (; c + 12 < ts; c += 16) {
-        a0 += (double) A[(int64_t) c * ts] * sy[c];
-        a1 += (double) A[(int64_t) (c + 4) * ts] * sy[c + 4];
-        a2 += (double) A[(int64_t) (c + 8) * ts] * sy[c + 8];
-        a3 += (double) A[(int64_t) (c + 12) * ts] * sy[c + 12];
+    // 16 independent loads per lane and step (ts is a multiple of 64)
+    double a[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int c = w; c < ts; c += 64) {
+#pragma unroll
+        for (int u = 0; u < 16; ++u) a[u & 3] += (double) A[(int64_t) (c + 4 * u) * ts] * sy[c + 4 * u];
     }
-    for (; c < ts; c += 4) a0 += (double) A[(int64_t) c * ts] * sy[c];
-    return (a0 + a1) + (a2 + a3);
+    return (a[0] + a[1]) + (a[2] + a[3]);
 }
 
 __global__ void __launch_bounds__(kThreads) fwd_panel_kernel(TileMap map, int k, const double *__restrict__ yk,
@@ -657,32 +654,29 @@ __global__ void __launch_bounds__(kThreads) bwd_row_kernel(TileMap map, int k, c
     const int lane = t & 63, w = t >> 6;
     for (int e = t; e < ts; e += kThreads) sy[e] = yk[e];
     __syncthreads();
-    const bool f32 = map.store[k * map.nt + j] == kStoreF32;
-    const double *A = tile_f64(map, k, j);
-    const float *Af = tile_f32(map, k, j);
-    for (int cc = 0; cc < 16; ++cc) {
-        const int c = cb * kBlk + w * 16 + cc;
-        double s0 = 0.0, s1 = 0.0;
-        int r0 = lane;
-        if (f32) {
-            const float *col = Af + (int64_t) c * ts;
-            for (; r0 + 64 < ts; r0 += 128) {
-                s0 += (double) col[r0] * sy[r0];
-                s1 += (double) col[r0 + 64] * sy[r0 + 64];
-            }
-            for (; r0 < ts; r0 += 64) s0 += (double) col[r0] * sy[r0];
-        } else {
-            const double *col = A + (int64_t) c * ts;
-            for (; r0 + 64 < ts; r0 += 128) {
-                s0 += col[r0] * sy[r0];
-                s1 += col[r0 + 64] * sy[r0 + 64];
-            }
-            for (; r0 < ts; r0 += 64) s0 += col[r0] * sy[r0];
+    // the wave's 16 columns advance together over the rows: 16 independent
+    // loads per lane and row step (one column at a time waits one memory
+    // latency per 64 rows)
+    const int c0 = cb * kBlk + w * 16;
+    double acc[16];
+#pragma unroll
+    for (int cc = 0; cc < 16; ++cc) acc[cc] = 0.0;
+    auto sweep = [&](const auto *A) {
+        const auto *col = A + (int64_t) c0 * ts;
+        for (int r = lane; r < ts; r += 64) {
+            const double yr = sy[r];
+#pragma unroll
+            for (int cc = 0; cc < 16; ++cc) acc[cc] += (double) col[r + (int64_t) cc * ts] * yr;
         }
-        double s = s0 + s1;
+    };
+    if (map.store[k * map.nt + j] == kStoreF32) sweep(tile_f32(map, k, j));
+    else sweep(tile_f64(map, k, j));
+#pragma unroll
+    for (int cc = 0; cc < 16; ++cc) {
+        double s = acc[cc];
 #pragma unroll
         for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
-        if (lane == 0) x[(int64_t) j * ts + c] -= s;
+        if (lane == 0) x[(int64_t) j * ts + c0 + cc] -= s;
     }
 }
 
