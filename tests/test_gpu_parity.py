@@ -104,6 +104,28 @@ def test_not_positive_definite_reports(ctx):
     assert np.isfinite(llh)
 
 
+@pytest.mark.parametrize("mixed", [False, True])
+def test_not_positive_definite_names_the_oracles_leading_minor(ctx, mixed):
+    """The failing column reported by the diagonal-block kernel is the one
+    dpotrf reports: sigma^2 < 0 fails at the first pivot, a bivariate cross
+    correlation beta > 1 (nu1 = nu2) at the second (the first location's 2 x 2
+    block has determinant 1 - beta^2 < 0)."""
+    x, y = orc.generate_locations(300, seed=5)
+    for kernel, th in ((0, [-1.0, 0.1, 0.5]), (1, [1.0, 1.0, 0.1, 0.5, 0.5, 1.5])):
+        A = orc.covariance_matrix(kernel, (x, y), (x, y), th, dts=192, lower=False)
+        with pytest.raises(RuntimeError) as info:
+            orc.potrf(A, 192)
+        minor = int(str(info.value).split("minor ")[1].rstrip(")"))
+        log("  kernel %d: oracle leading minor %d" % (kernel, minor))
+        p = egs.Problem(ctx, kernel, x, y, dts=192)
+        p.set_observations(np.ones(len(x) * (2 if kernel == 1 else 1)))
+        if mixed:
+            p.set_precision(egs.PRECISION_MIXED, 1, 3)
+        with pytest.raises(egs.NotPositiveDefinite, match=r"leading minor %d\)" % minor):
+            p.loglik(th)
+        p.close()
+
+
 # ------------------------------------------------------------------ likelihood
 @pytest.mark.parametrize("n,dts,theta", [(1600, 320, [1.0, 0.1, 0.5]), (900, 64, [1.0, 0.1, 1.0]),
                                          (1500, 960, [0.8, 0.03, 1.3]), (333, 128, [1.2, 0.2, 2.5])])
