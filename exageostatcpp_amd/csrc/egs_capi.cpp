@@ -144,6 +144,7 @@ struct egs_problem_s {
     std::vector<double> h_partials;
 
     bool has_obs = false;
+    bool has_cov = false;    // the tiles hold Sigma(cov_theta), not yet factored
     bool factored = false;
     double factored_theta[6] = {0, 0, 0, 0, 0, 0};
     double cov_theta[6] = {0, 0, 0, 0, 0, 0};
@@ -478,6 +479,7 @@ int layout_tiles(egs_problem_s *p) {
     EGS_HIP(hipMemcpy(p->d_off, p->h_off.data(), sizeof(int64_t) * p->h_off.size(), hipMemcpyHostToDevice));
     EGS_HIP(hipMemcpy(p->d_store, p->h_store.data(), p->h_store.size(), hipMemcpyHostToDevice));
     p->factored = false;
+    p->has_cov = false;
     return EGS_SUCCESS;
 }
 
@@ -492,6 +494,7 @@ int run_covariance(egs_problem_s *p, const double *theta) {
     prof_end(p, pi);
     EGS_HIP(hipGetLastError());
     p->factored = false;
+    p->has_cov = true;
     std::memcpy(p->cov_theta, theta, sizeof(double) * p->ntheta());
     return EGS_SUCCESS;
 }
@@ -611,6 +614,7 @@ int run_potrf(egs_problem_s *p) {
     hipStream_t s = p->ctx->stream, b = no_lookahead ? p->ctx->stream : p->ctx->bulk;
     const egs::TileMap m = p->map();
     const int nt = p->nt;
+    p->has_cov = false;   // the factorization overwrites the tiles
     for (int k = (int) p->ev_panel.size(); k < nt; ++k) {
         hipEvent_t e1 = nullptr, e2 = nullptr;
         EGS_HIP(hipEventCreateWithFlags(&e1, hipEventDisableTiming));
@@ -1125,6 +1129,7 @@ int egs_covariance(egs_problem_t p, const double *theta) {
 int egs_potrf(egs_problem_t p) {
     int st = check_problem(p);
     if (st) return st;
+    if (!p->has_cov) return fail(EGS_ERR_STATE, "no covariance matrix to factor: call egs_covariance first");
     st = run_potrf(p);
     if (st) return st;
     EGS_HIP(hipStreamSynchronize(p->ctx->stream));
@@ -1139,6 +1144,7 @@ int egs_solve_logdet_quad(egs_problem_t p, double *logdet, double *quad) {
     int st = check_problem(p);
     if (st) return st;
     if (!p->has_obs) return fail(EGS_ERR_STATE, "the problem has no observations");
+    if (!p->factored) return fail(EGS_ERR_STATE, "no factored matrix: call egs_potrf first");
     hipStream_t s = p->ctx->stream;
     egs::launch_copy_pad(p->d_work, p->d_obs, p->n, p->npad, s);
     if (is_mixed(p)) egs::launch_diag_logdet(p->map(), p->n, p->d_partials, s);   // quad from the refinement

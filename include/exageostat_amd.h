@@ -148,7 +148,10 @@ int egs_loglik(egs_problem_t problem, const double *theta, double *loglik, doubl
  *   RuntimeFunctions<T>::CovarianceMatrix (src/runtime/starpu/StarPuFunctions.cpp:24-45),
  *   ExaGeoStatPotrfTile (ChameleonDense.cpp:25-37),
  *   ExaGeoStatMeasureDetTileAsync + ExaGeoStatTrsmTile + ExaGeoStatDoubleDotProduct
- *   (StarPuFunctions.cpp:119-135, 189-204; ChameleonImplementation.cpp:337-347). */
+ *   (StarPuFunctions.cpp:119-135, 189-204; ChameleonImplementation.cpp:337-347).
+ * egs_potrf factors the matrix of the last egs_covariance call and fails with
+ * EGS_ERR_STATE when there is none (never generated, or already factored);
+ * egs_solve_logdet_quad needs a successful egs_potrf (or evaluation). */
 int egs_covariance(egs_problem_t problem, const double *theta);
 int egs_potrf(egs_problem_t problem);
 int egs_solve_logdet_quad(egs_problem_t problem, double *logdet, double *quad);

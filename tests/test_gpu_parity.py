@@ -92,6 +92,31 @@ def test_potrf_matches_oracle(ctx, n, dts):
     assert np.max(np.abs(Lg @ Lg.T - A)) < 1e-12
 
 
+def test_phases_must_run_in_order(ctx):
+    """egs_potrf factors the matrix of the last egs_covariance call once;
+    egs_solve_logdet_quad needs a factor.  Out of order, both refuse instead of
+    working on stale tiles, and the in-order sequence still matches the
+    one-call evaluation."""
+    x, y = _rng_locs(300, 3)
+    th = [1.0, 0.1, 0.5]
+    p = egs.Problem(ctx, 0, x, y, dts=64)
+    p.set_observations(np.linspace(-1.0, 1.0, 300))
+    with pytest.raises(egs.EGSError, match="call egs_covariance first"):
+        p.potrf()
+    with pytest.raises(egs.EGSError, match="call egs_potrf first"):
+        p.solve_logdet_quad()
+    p.covariance(th)
+    with pytest.raises(egs.EGSError, match="call egs_potrf first"):
+        p.solve_logdet_quad()
+    p.potrf()
+    with pytest.raises(egs.EGSError, match="call egs_covariance first"):
+        p.potrf()
+    ld, q = p.solve_logdet_quad()
+    _, ld1, q1 = p.loglik(th)
+    assert ld == ld1 and q == q1
+    p.close()
+
+
 def test_not_positive_definite_reports(ctx):
     log("not-SPD error path")
     x, y = _rng_locs(100, 7)
