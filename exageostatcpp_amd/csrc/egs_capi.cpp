@@ -1234,6 +1234,7 @@ int egs_loglik(egs_problem_t p, const double *theta, double *loglik, double *log
         EGS_HIP(hipEventRecord(p->ev[0], s));
         st = graph_launch(p, theta);
         if (st) return st;
+        p->has_cov = false;   // the replay factored the tiles
         EGS_HIP(hipEventRecord(p->ev[3], s));
         EGS_HIP(hipStreamSynchronize(s));
         std::memcpy(p->cov_theta, theta, sizeof(double) * p->ntheta());

@@ -106,3 +106,22 @@ def test_not_spd_reported_through_the_graph(ctx):
         p.loglik([-1.0, 0.1, 0.5])
     assert p.loglik([1.0, 0.1, 0.5]) == good
     p.close()
+
+
+def test_graph_replay_leaves_no_covariance_to_factor(ctx):
+    """After a replayed evaluation the tiles hold the factor: egs_potrf refuses
+    them (it factors only the matrix of the last egs_covariance call), even
+    when egs_covariance ran before the replay."""
+    x, y = orc.generate_locations(500, seed=6)
+    th = [1.0, 0.1, 0.5]
+    p = egs.Problem(ctx, 0, x, y, dts=128)
+    p.set_observations(np.linspace(-1.0, 1.0, 500))
+    p.set_graph(True)
+    p.loglik(th)               # captures
+    p.covariance(th)
+    p.loglik(th)               # replays over the generated covariance
+    with pytest.raises(egs.EGSError, match="call egs_covariance first"):
+        p.potrf()
+    p.covariance(th)
+    p.potrf()
+    p.close()
