@@ -306,8 +306,8 @@ __device__ __forceinline__ double readlane_f64(double v, int l) {
 
 // Block (jb, jb) of diagonal tile k: L = chol(A) in place and W = L^{-1} into
 // winv_k, both lower (zeros above the diagonal).  The block lives in registers:
-// thread (w, i) = (threadIdx.x / 64, lane) holds row i of columns c = w + 4 q,
-// q < 16, in y[q]: column c of A until step c, then column c of X, which
+// thread (w, i) = (threadIdx.x / 64, lane) holds row i of columns c = w + 8 q,
+// q < 8, in y[q]: column c of A until step c, then column c of X, which
 // becomes L^{-1}.  Column step j (64 steps, one barrier each) is the
 // right-looking Cholesky step A_ic -= L_ij L_cj (c > j) fused with the
 // column-oriented inversion X_jc /= L_jj, X_ic -= L_ij X_jc (i > j, c <= j).
@@ -318,9 +318,12 @@ __device__ __forceinline__ double readlane_f64(double v, int l) {
 // buffered) for c > j, the unscaled X_jc read from lane j for c <= j.  At step
 // j the owner wave of column j stores L's column j and restarts its register
 // as column j of X.
-__global__ void __launch_bounds__(kThreads) potrf64_kernel(TileMap map, int k, int jb, double *winv_k, int *info,
-                                                           int64_t n) {
-    constexpr int kCols = kBlk / 4;   // columns per thread
+constexpr int kPotrfWaves = 8;   // two waves per SIMD: each step's FMAs interleave
+
+__global__ void __launch_bounds__(64 * kPotrfWaves) potrf64_kernel(TileMap map, int k, int jb, double *winv_k,
+                                                                   int *info, int64_t n) {
+    constexpr int kW = kPotrfWaves;
+    constexpr int kCols = kBlk / kW;   // columns per thread
     __shared__ double s_col[2][kBlk];
     __shared__ double s_rs[kBlk];     // 1 / L_jj
     const int ts = map.ts;
@@ -330,7 +333,7 @@ __global__ void __launch_bounds__(kThreads) potrf64_kernel(TileMap map, int k, i
     double *Ajj = tile_f64(map, k, k) + (int64_t) jb * kBlk + (int64_t) jb * kBlk * ts;
     double y[kCols];
 #pragma unroll
-    for (int q = 0; q < kCols; ++q) y[q] = Ajj[i + (int64_t) (w + 4 * q) * ts];
+    for (int q = 0; q < kCols; ++q) y[q] = Ajj[i + (int64_t) (w + kW * q) * ts];
     if (w == 0) s_col[0][i] = y[0];
     __syncthreads();
     for (int j = 0; j < kBlk; ++j) {
@@ -350,10 +353,10 @@ __global__ void __launch_bounds__(kThreads) potrf64_kernel(TileMap map, int k, i
         // the LDS operands of every column at once: one wait for all of them
         double gl[kCols];
 #pragma unroll
-        for (int q = 0; q < kCols; ++q) gl[q] = s_col[buf][w + 4 * q];
+        for (int q = 0; q < kCols; ++q) gl[q] = s_col[buf][w + kW * q];
 #pragma unroll
         for (int q = 0; q < kCols; ++q) {
-            const int c = w + 4 * q;   // uniform across the wave
+            const int c = w + kW * q;   // uniform across the wave
             double g = gl[q];
             if (c <= j) g = readlane_f64(y[q], j);
             const double upd = fma(-r, g, y[q]);
@@ -367,7 +370,7 @@ __global__ void __launch_bounds__(kThreads) potrf64_kernel(TileMap map, int k, i
     const double rsi = s_rs[i];
 #pragma unroll
     for (int q = 0; q < kCols; ++q) {
-        const int c = w + 4 * q;
+        const int c = w + kW * q;
         W[i + c * kBlk] = i >= c ? y[q] * rsi : 0.0;
     }
 }
@@ -461,7 +464,7 @@ void launch_factor_diag(const TileMap &map, int k, double *winv_k, double *scrat
                         hipStream_t s) {
     const int nsb = map.ts / kBlk;
     for (int jb = 0; jb < nsb; ++jb) {
-        hipLaunchKernelGGL(potrf64_kernel, dim3(1), dim3(kThreads), 0, s, map, k, jb, winv_k, info, n);
+        hipLaunchKernelGGL(potrf64_kernel, dim3(1), dim3(64 * kPotrfWaves), 0, s, map, k, jb, winv_k, info, n);
         const int rem = nsb - jb - 1;
         if (rem > 0)
             hipLaunchKernelGGL(diag_update_kernel, dim3((unsigned) (rem * (rem + 1) / 2)), dim3(kThreads), 0, s, map,
