@@ -343,8 +343,14 @@ __global__ void __launch_bounds__(64 * kPotrfWaves) potrf64_kernel(TileMap map, 
             const int64_t col = (int64_t) k * ts + (int64_t) jb * kBlk + j;
             if (col < n) atomicCAS(info, 0, (int) (col + 1));
         }
-        const double sq = sqrt(piv);
-        const double rs = 1.0 / sq;
+        // 1 / sqrt(piv): the hardware estimate and two Newton steps (each
+        // doubles the correct bits; the chain is 7 dependent operations instead
+        // of a correctly rounded square root followed by a division)
+        double rs = __builtin_amdgcn_rsq(piv);
+        const double hp = 0.5 * piv;
+        rs = fma(rs, fma(-hp * rs, rs, 0.5), rs);
+        rs = fma(rs, fma(-hp * rs, rs, 0.5), rs);
+        const double sq = piv * rs;
         if (t == 0) s_rs[j] = rs;
         const double lo = i > j ? s_col[buf][i] * rs : 0.0;   // L_ij below the diagonal
         const double r = lo * rs;
