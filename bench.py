@@ -20,9 +20,9 @@ relative log-likelihood error of the mixed plan against FP64.
 The roofline entry is the dominant kernel of the mixed run (the update class
 with the largest time): algorithmic flops per launch over its average launch
 time, timed with HIP events on the stream it runs on.  The CPU baseline times
-the oracle (a C restatement of the same algorithm, OpenMP over tiles) on a
-bounded sample and scales it to n by the n^3 (factorization) and n^2
-(covariance, solve) laws.
+FP64 evaluations on the host cores (the oracle's covariance tiles, host LAPACK
+dpotrf and dtrtrs) on a bounded sample and scales the median to n by the n^3
+(factorization) and n^2 (covariance, solve) laws.
 """
 import argparse
 import json
@@ -41,26 +41,79 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(n_target, n_sample, dts, theta):
+def host_cores():
+    """(threads this process may use, physical cores of the host from lscpu)."""
+    affinity = len(os.sched_getaffinity(0))
+    physical = None
+    try:
+        import subprocess
+
+        out = subprocess.run(["lscpu", "-p=CORE,SOCKET"], capture_output=True, text=True, timeout=10).stdout
+        physical = len({l for l in out.splitlines() if l and not l.startswith("#")})
+    except (OSError, ValueError):
+        pass
+    return affinity, physical
+
+
+def cpu_baseline(n_target, n_sample, dts, theta, reps=3):
+    """FP64 likelihood evaluations on the host cores: the oracle's tiled
+    Matern covariance (C, OpenMP over tiles), host LAPACK dpotrf (OpenBLAS,
+    multithreaded) for the factorization, then log|Sigma| from the diagonal and
+    one triangular solve with dtrtrs.  One warm-up and `reps` timed
+    evaluations at n_sample; the median is scaled to n_target by n^3 for the
+    factorization and n^2 for covariance and solve."""
+    import numpy as np
+    import scipy.linalg.blas as blas
+    import scipy.linalg.lapack as lapack
+    import threadpoolctl
+
     from oracle import orc
 
+    affinity, physical = host_cores()
+    threads = min(16, affinity)
     x, y = orc.generate_locations(n_sample, seed=0)
-    z = orc.generate_observations(0, (x, y), theta, dts, seed=0)
-    t0 = time.perf_counter()
-    r = orc.loglik(0, (x, y), theta, z, dts)
-    wall = time.perf_counter() - t0
+    z = None
+    runs = []
+    with threadpoolctl.threadpool_limits(threads):
+        for rep in range(reps + 1):
+            t0 = time.perf_counter()
+            C = orc.covariance_matrix(0, (x, y), (x, y), theta, dts, order="F")
+            t1 = time.perf_counter()
+            L, info = lapack.dpotrf(C, lower=1, clean=0, overwrite_a=1)
+            t2 = time.perf_counter()
+            if info != 0:
+                raise RuntimeError("host dpotrf failed (info %d)" % info)
+            if z is None:
+                # observations z = L e from the warm-up factor (LinearAlgebraMethods.cpp:333-449)
+                e = orc.dlarnv_normal([0, 0, 0, 1], n_sample)[0]
+                z = blas.dtrmv(L, e, lower=1)
+            t3 = time.perf_counter()
+            logdet = 2.0 * np.sum(np.log(np.diag(L)))
+            w, info = lapack.dtrtrs(L, z, lower=1)
+            quad = float(w @ w)
+            llh = -0.5 * quad - 0.5 * logdet - 0.5 * n_sample * np.log(2.0 * np.pi)
+            t4 = time.perf_counter()
+            del C, L
+            if rep > 0:
+                runs.append((t1 - t0, t2 - t1, t4 - t3, t4 - t0 - (t3 - t2)))
+            log("cpu baseline rep %d: covariance %.2f s, dpotrf %.2f s, solve %.3f s, llh %.6f"
+                % (rep, t1 - t0, t2 - t1, t4 - t3, llh))
+    med = sorted(runs, key=lambda r: r[3])[len(runs) // 2]
     s = n_target / n_sample
-    t_scaled = r["t_facto"] * s ** 3 + (r["t_gen"] + r["t_solve"]) * s ** 2
-    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    t_scaled = med[1] * s ** 3 + (med[0] + med[2]) * s ** 2
     return {
         "value": 1.0 / t_scaled,
         "unit": "llh evals/s",
         "cores": threads,
+        "host_physical_cores": physical,
         "kind": "port",
-        "sample": "oracle (C restatement: tiled Cholesky + Matern covariance, OpenMP over tiles) on one FP64 "
-                  "likelihood evaluation at n=%d, dts=%d, same theta; measured %.2f s (covariance %.2f s, "
-                  "factorization %.2f s, solve %.2f s) and scaled to n=%d by n^3 for the factorization and n^2 "
-                  "for covariance and solve" % (n_sample, dts, wall, r["t_gen"], r["t_facto"], r["t_solve"], n_target),
+        "measured_evals_per_s_at_sample": 1.0 / med[3],
+        "sample": "median of %d FP64 likelihood evaluations (after one warm-up) at n=%d, same theta: oracle "
+                  "covariance (C, OpenMP over %d-tiles) %.2f s, host LAPACK dpotrf (OpenBLAS, %d threads) %.2f s "
+                  "(%.0f GFLOP/s), log-det and dtrtrs %.3f s; scaled to n=%d by n^3 for the factorization and n^2 "
+                  "for covariance and solve (a restatement on the host cores, not Chameleon+StarPU)"
+                  % (reps, n_sample, dts, med[0], threads, med[1], n_sample ** 3 / 3.0 / med[1] / 1e9, med[2],
+                     n_target),
     }
 
 
@@ -83,21 +136,24 @@ def tau_json(args):
 
 
 def measured_traffic(kernel, args):
-    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter
-    summary (FETCH_SIZE + WRITE_SIZE, copy-calibrated), when it was taken on
-    this same workload; None otherwise."""
+    """(HBM bytes per launch, algorithmic bytes per launch) of `kernel` from
+    the committed rocprofv3 counter summary (FETCH_SIZE + WRITE_SIZE,
+    copy-calibrated), when it was taken on this same workload; (None, None)
+    otherwise."""
     path = os.path.join(ROOT, "profiles", "%s_n%dk_traffic.json" % (kernel, args.n // 1000))
     try:
         with open(path) as fh:
             t = json.load(fh)
     except (OSError, ValueError):
-        return None
+        return None, None
     same = (t.get("kernel") == kernel and t.get("n") == args.n and t.get("dts") == args.dts
             and t.get("band_fp64") == args.band_fp64 and t.get("band_fp32") == args.band_fp32
             and t.get("single_band") == args.single_band and t.get("single_tau") == tau_json(args)
             and t.get("fp64_product_band", -1) == product_band(args, bivariate=False)
             and t.get("operands") == operands(args))
-    return t["fetch_bytes_per_launch"] + t["write_bytes_per_launch"] if same else None
+    if not same:
+        return None, None
+    return t["fetch_bytes_per_launch"] + t["write_bytes_per_launch"], t.get("algorithmic_bytes_per_launch")
 
 
 def timed(egs, ctx, prob, theta, steps, warmup, label):
@@ -335,8 +391,9 @@ def main():
     ap.add_argument("--single-tau", type=float, default=2.0 ** -16,
                     help="... when their panel rows' scales multiply to <= tau times the panel's largest squared")
     ap.add_argument("--plain-bf16", action="store_true", help="single BF16 product instead of the split one")
-    ap.add_argument("--cpu-sample", type=int, default=18240,
-                    help="n of the oracle sample (about 10-20 s on 16 host cores)")
+    ap.add_argument("--cpu-sample", type=int, default=20000,
+                    help="n of the host sample (each evaluation a few seconds on 16 host cores)")
+    ap.add_argument("--cpu-reps", type=int, default=3, help="timed host evaluations (median reported)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--mle", action="store_true",
                     help="fit theta with the FP64 and the mixed plan on the same data and report both estimates "
@@ -409,6 +466,7 @@ def main():
     if dom == "update_16_select" and pc["split"] + pc["single"] > 0:
         # MFMAs per product: 3 for split blocks, 1 for single-product ones
         mfma_per_flop = (3.0 * pc["split"] + pc["single"]) / (pc["split"] + pc["single"])
+    traffic, alg_bytes = measured_traffic(dom, args)
     dom64 = fp64["kernels"]["update_fp64"]
     ach64 = dom64["flops"] / (dom64["ms"] * 1e-3) / 1e12 if dom64["ms"] > 0 else 0.0
 
@@ -444,14 +502,19 @@ def main():
             "achieved": achieved,
             "peak": PEAK[prec],
             "unit": "TFLOP/s",
-            "frac": achieved * mfma_per_flop / PEAK[prec],
-            "traffic": measured_traffic(dom, args),
+            "frac": achieved / PEAK[prec],
+            "traffic": traffic,
             "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated)",
+            "algorithmic_bytes": alg_bytes,
+            "traffic_over_algorithmic": (traffic / alg_bytes) if traffic and alg_bytes else None,
             "launches": d["launches"],
             "avg_launch_ms": d["ms"] / max(1, d["launches"]),
             "mfma_per_product": mfma_per_flop,
-            "note": "achieved counts algorithmic flops; frac counts the MFMA work (3 MFMAs per split product, "
-                    "1 per single one, weighted by the block counts)",
+            "frac_mfma_issued": achieved * mfma_per_flop / PEAK[prec],
+            "note": "achieved and frac count algorithmic flops (2 ts^3 per tile update) over the launch time; "
+                    "frac_mfma_issued weights them by the MFMAs each product takes (3 per split product, 1 per "
+                    "single one, from the block counts); algorithmic_bytes is each updated tile read and written "
+                    "once",
         },
         "fp64": {
             "evals_per_s": fp64["evals_per_s"], "ms_per_step": fp64["ms_per_step"], "steps": fp64_steps,
@@ -477,7 +540,7 @@ def main():
     if not args.no_serve:
         out["serve"] = serve_phase(egs, args)
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(args.n, args.cpu_sample, args.dts, theta)
+        out["cpu_baseline"] = cpu_baseline(args.n, args.cpu_sample, args.dts, theta, args.cpu_reps)
     print(json.dumps(out))
 
 

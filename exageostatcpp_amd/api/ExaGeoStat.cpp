@@ -357,6 +357,7 @@ egs_problem_t ExaGeoStatData<T>::GetProblem(egs_context_t aContext, int aKernelI
     mKernelId = aKernelId;
     mMetric = aMetric;
     mDts = aDts;
+    mPlan[0] = EGS_PRECISION_FP64;   // a new problem starts under the FP64 plan
     return mProblem;
 }
 
@@ -365,6 +366,18 @@ void ExaGeoStatData<T>::ReleaseProblem() {
     if (mProblem) egs_problem_destroy(mProblem);
     mProblem = nullptr;
     mKernelId = mMetric = mDts = -1;
+    mPlan[0] = mPlan[1] = mPlan[2] = -1;
+}
+
+template<typename T>
+void ExaGeoStatData<T>::SetProblemPlan(int aMode, int aBandDP, int aBandSP) {
+    if (!mProblem) throw std::runtime_error("SetProblemPlan: no device problem");
+    const bool fp64 = aMode == EGS_PRECISION_FP64;
+    if (mPlan[0] == aMode && (fp64 || (mPlan[1] == aBandDP && mPlan[2] == aBandSP))) return;
+    exageostat::CheckStatus(egs_problem_set_precision(mProblem, aMode, aBandDP, aBandSP), "egs_problem_set_precision");
+    mPlan[0] = aMode;
+    mPlan[1] = aBandDP;
+    mPlan[2] = aBandSP;
 }
 
 template class ExaGeoStatData<float>;
@@ -372,7 +385,6 @@ template class ExaGeoStatData<double>;
 
 // ---------------------------------------------------------------- API
 namespace exageostat {
-namespace api {
 
 namespace {
 
@@ -384,10 +396,15 @@ int PrecisionMode(configurations::Configurations &aConf) {
     }
 }
 
-void ConfigurePrecision(egs_problem_t aProblem, configurations::Configurations &aConf) {
-    const int mode = PrecisionMode(aConf);
-    CheckStatus(egs_problem_set_precision(aProblem, mode, aConf.GetDenseBandDP(), aConf.GetBandDenseSP()),
-                "egs_problem_set_precision");
+template<typename T>
+dataunits::VectorDescriptor<T> &AsVector(void *apDescriptor, const char *aWhere) {
+    if (!apDescriptor) throw std::runtime_error(std::string(aWhere) + ": null vector descriptor");
+    return *static_cast<dataunits::VectorDescriptor<T> *>(apDescriptor);
+}
+
+egs_problem_t AsMatrix(void *apDescriptor, const char *aWhere) {
+    if (!apDescriptor) throw std::runtime_error(std::string(aWhere) + ": null matrix descriptor");
+    return static_cast<egs_problem_t>(apDescriptor);
 }
 
 // Fisher-Yates shuffle with rand() as PredictionHelpers<T>::Shuffle
@@ -407,28 +424,232 @@ void ShuffleTogether(std::vector<std::vector<double>> &aParts, dataunits::Locati
 
 }  // namespace
 
+namespace linearAlgebra {
+namespace dense {
+
+template<typename T>
+T HipDense<T>::ExaGeoStatMLETile(std::unique_ptr<ExaGeoStatData<T>> &aData,
+                                 configurations::Configurations &aConfigurations, const double *apTheta,
+                                 T *apMeasurementsMatrix, const kernels::Kernel<T> &aKernel) {
+    if (!aData) throw std::runtime_error("ExaGeoStatMLETile: no data");
+    if (!apTheta) throw std::runtime_error("ExaGeoStatMLETile: theta is null");
+    const size_t nz = static_cast<size_t>(aData->GetLocations()->GetSize()) * aKernel.GetVariablesNumber();
+    egs_problem_t p = aData->GetProblem(ExaGeoStatHardware::GetContext(), aKernel.GetDeviceKernelId(),
+                                        aConfigurations.GetDistanceMetric(), aConfigurations.GetDenseTileSize());
+    // Z restored from the measurements before every evaluation
+    // (ChameleonImplementation.cpp:113-125); the device copy is never overwritten
+    const std::vector<double> z = apMeasurementsMatrix ? ToDouble(apMeasurementsMatrix, nz) : aData->GetMeasurements();
+    if (z.size() != nz) throw std::runtime_error("ExaGeoStatMLETile: no measurements");
+    CheckStatus(egs_problem_set_observations(p, z.data(), (int64_t) nz), "egs_problem_set_observations");
+    aData->SetProblemPlan(PrecisionMode(aConfigurations), aConfigurations.GetDenseBandDP(),
+                          aConfigurations.GetBandDenseSP());
+    double llh = 0.0;
+    CheckStatus(egs_loglik(p, apTheta, &llh, nullptr, nullptr), "ExaGeoStatMLETile");
+    double t_cov = 0.0, t_fact = 0.0, t_solve = 0.0;
+    egs_last_timings(p, &t_cov, &t_fact, &t_solve);
+    const int np = aKernel.GetParametersNumbers();
+    std::vector<double> theta(apTheta, apTheta + np);
+    aData->SetMleIterations(aData->GetMleIterations() + 1);
+    auto *res = results::Results::GetInstance();
+    res->SetMLEIterations(aData->GetMleIterations());
+    res->SetMaximumTheta(theta);
+    res->SetLogLikValue(llh);
+    // the per-evaluation lines of ChameleonImplementation.cpp:273-310
+    if (Verbose()) {
+        std::printf("\t%d- Model Parameters (", aData->GetMleIterations());
+        for (int i = 0; i < np; ++i) std::printf("%.8f%s", theta[i], i + 1 < np ? ", " : "");
+        std::printf(")----> LogLi: %.18f\n", llh);
+    }
+    if (configurations::Configurations::GetVerbosity() == common::DETAILED_MODE) {
+        const double n = static_cast<double>(nz);
+        std::printf("---- Facto Time: %g\n---- dtrsm Time: %g\n---- Matrix Generation Time: %g\n"
+                    "---- Total Time: %g\n---- Gflop/s: %g\n",
+                    t_fact * 1e-3, t_solve * 1e-3, t_cov * 1e-3, (t_fact + t_solve) * 1e-3,
+                    (n * n * n / 3.0 + 2.0 * n * n) / 1e9 / std::max((t_fact + t_solve) * 1e-3, 1e-12));
+    }
+    return static_cast<T>(llh);
+}
+
+template<typename T>
+void HipDense<T>::ExaGeoStatLapackCopyTile(const common::UpperLower &, void *apA, void *apB) {
+    // the Z -> Z_COPY copy of ChameleonImplementation.cpp:329-335 on vector descriptors
+    auto &a = AsVector<T>(apA, "ExaGeoStatLapackCopyTile");
+    auto &b = AsVector<T>(apB, "ExaGeoStatLapackCopyTile");
+    if (a.size != b.size) throw std::runtime_error("ExaGeoStatLapackCopyTile: descriptor sizes differ");
+    std::copy(a.data, a.data + a.size, b.data);
+}
+
+template<typename T>
+void HipDense<T>::ExaGeoStatSequenceWait(void *apSequence) {
+    if (!apSequence) throw std::runtime_error("ExaGeoStatSequenceWait: null sequence");
+    CheckStatus(egs_synchronize(static_cast<egs_context_t>(apSequence)), "ExaGeoStatSequenceWait");
+}
+
+template<typename T>
+void HipDense<T>::ExaGeoStatCreateSequence(void *apSequence) {
+    // the reference passes the address of its sequence pointer
+    if (!apSequence) throw std::runtime_error("ExaGeoStatCreateSequence: null sequence");
+    *static_cast<egs_context_t *>(apSequence) = ExaGeoStatHardware::GetContext();
+}
+
+template<typename T>
+void HipDense<T>::ExaGeoStatPotrfTile(const common::UpperLower &aUpperLower, void *apA, int, void *, void *,
+                                      const int &, const int &) {
+    if (aUpperLower != common::EXAGEOSTAT_LOWER)
+        throw std::domain_error("ExaGeoStatPotrfTile: the tiles hold the lower triangle");
+    CheckStatus(egs_potrf(AsMatrix(apA, "ExaGeoStatPotrfTile")), "ExaGeoStatPotrfTile");
+}
+
+template<typename T>
+void HipDense<T>::ExaGeoStatTrsmTile(const common::Side &aSide, const common::UpperLower &aUpperLower,
+                                     const common::Trans &aTrans, const common::Diag &aDiag, const T &aAlpha,
+                                     void *apA, void *, void *, void *apZ, const int &) {
+    if (aSide != common::EXAGEOSTAT_LEFT || aUpperLower != common::EXAGEOSTAT_LOWER ||
+        aDiag != common::EXAGEOSTAT_NON_UNIT || aTrans == common::EXAGEOSTAT_CONJ_TRANS)
+        throw std::domain_error("ExaGeoStatTrsmTile: the factor is lower, non-unit, applied from the left");
+    auto &z = AsVector<T>(apZ, "ExaGeoStatTrsmTile");
+    std::vector<double> v = ToDouble(z.data, static_cast<size_t>(z.size));
+    CheckStatus(egs_trsm_vector(AsMatrix(apA, "ExaGeoStatTrsmTile"), aTrans == common::EXAGEOSTAT_TRANS ? 1 : 0,
+                                static_cast<double>(aAlpha), v.data(), z.size),
+                "ExaGeoStatTrsmTile");
+    for (int64_t i = 0; i < z.size; ++i) z.data[i] = static_cast<T>(v[i]);
+}
+
+template<typename T>
+void HipDense<T>::GenerateObservationsVector(configurations::Configurations &aConfigurations,
+                                             std::unique_ptr<ExaGeoStatData<T>> &aData,
+                                             dataunits::Locations<T> *apLocation1, dataunits::Locations<T> *,
+                                             dataunits::Locations<T> *, const int &aDistanceMetric,
+                                             const kernels::Kernel<T> &aKernel) {
+    // LinearAlgebraMethods.cpp:333-449: z = L(initial theta) e, e = dlarnv(3, {seed, seed, seed, 1})
+    if (!aData) throw std::runtime_error("GenerateObservationsVector: no data");
+    if (apLocation1 && apLocation1 != aData->GetLocations())
+        throw std::domain_error("GenerateObservationsVector: observations are generated at the data's locations");
+    std::vector<double> theta = aConfigurations.GetInitialTheta();
+    if (static_cast<int>(theta.size()) < aKernel.GetParametersNumbers())
+        throw std::domain_error("initial theta needs " + std::to_string(aKernel.GetParametersNumbers()) + " values");
+    egs_problem_t p = aData->GetProblem(ExaGeoStatHardware::GetContext(), aKernel.GetDeviceKernelId(),
+                                        aDistanceMetric, aConfigurations.GetDenseTileSize());
+    aData->SetProblemPlan(EGS_PRECISION_FP64, 0, 0);   // data is generated with the FP64 factor
+    std::vector<double> &obs = aData->GetMeasurements();
+    obs.assign(static_cast<size_t>(aData->GetLocations()->GetSize()) * aKernel.GetVariablesNumber(), 0.0);
+    CheckStatus(egs_generate_observations(p, theta.data(), aConfigurations.GetSeed(), obs.data()),
+                "GenerateObservationsVector");
+}
+
+template<typename T>
+void HipDense<T>::ExaGeoStatPosvTile(const common::UpperLower &aUpperLower, void *apA, void *apB) {
+    // dposv (LinearAlgebraMethods.cpp:1421-1428): factor Sigma, then L L^T x = b
+    ExaGeoStatPotrfTile(aUpperLower, apA, 0, nullptr, nullptr, 0, 0);
+    const T one = 1;
+    ExaGeoStatTrsmTile(common::EXAGEOSTAT_LEFT, aUpperLower, common::EXAGEOSTAT_NO_TRANS, common::EXAGEOSTAT_NON_UNIT,
+                       one, apA, nullptr, nullptr, apB, 0);
+    ExaGeoStatTrsmTile(common::EXAGEOSTAT_LEFT, aUpperLower, common::EXAGEOSTAT_TRANS, common::EXAGEOSTAT_NON_UNIT,
+                       one, apA, nullptr, nullptr, apB, 0);
+}
+
+template<typename T>
+T *HipDense<T>::ExaGeoStatMLEPredictTile(std::unique_ptr<ExaGeoStatData<T>> &, T *apTheta, const int &aZMissNumber,
+                                         const int &aZObsNumber, T *apZObs, T *apZActual, T *apZMiss,
+                                         configurations::Configurations &aConfiguration,
+                                         dataunits::Locations<T> &aMissLocations,
+                                         dataunits::Locations<T> &aObsLocations, const kernels::Kernel<T> &aKernel) {
+    // LinearAlgebraMethods.cpp:451-605: C22 = Sigma(obs, obs) factored on the
+    // GPU, zmiss = C12 C22^{-1} zobs with C12 regenerated, the three errors
+    if (!apTheta || !apZObs || !apZMiss || aZMissNumber <= 0 || aZObsNumber <= 0)
+        throw std::runtime_error("ExaGeoStatMLEPredictTile: invalid arguments");
+    const int p = aKernel.GetVariablesNumber();
+    const bool has_z = aObsLocations.GetDimension() != common::Dimension2D;
+    const double t0 = Seconds();
+    auto ox = ToDouble(aObsLocations.GetLocationX(), aZObsNumber), oy = ToDouble(aObsLocations.GetLocationY(), aZObsNumber);
+    auto oz = ToDouble(aObsLocations.GetLocationZ(), has_z ? aZObsNumber : 0);
+    auto mx = ToDouble(aMissLocations.GetLocationX(), aZMissNumber), my = ToDouble(aMissLocations.GetLocationY(), aZMissNumber);
+    auto mz = ToDouble(aMissLocations.GetLocationZ(), has_z ? aZMissNumber : 0);
+    egs_problem_t op = nullptr;
+    CheckStatus(egs_problem_create(ExaGeoStatHardware::GetContext(), aKernel.GetDeviceKernelId(), aZObsNumber,
+                                   ox.data(), oy.data(), oz.empty() ? nullptr : oz.data(),
+                                   aConfiguration.GetDistanceMetric(), aConfiguration.GetDenseTileSize(), &op),
+                "egs_problem_create");
+    std::unique_ptr<egs_problem_s, int (*)(egs_problem_t)> guard(op, egs_problem_destroy);
+    auto zobs = ToDouble(apZObs, static_cast<size_t>(aZObsNumber) * p);
+    CheckStatus(egs_problem_set_observations(op, zobs.data(), (int64_t) zobs.size()), "egs_problem_set_observations");
+    if (PrecisionMode(aConfiguration) != EGS_PRECISION_FP64)
+        CheckStatus(egs_problem_set_precision(op, PrecisionMode(aConfiguration), aConfiguration.GetDenseBandDP(),
+                                              aConfiguration.GetBandDenseSP()),
+                    "egs_problem_set_precision");
+    std::vector<double> theta(apTheta, apTheta + aKernel.GetParametersNumbers());
+    std::vector<double> zmiss(static_cast<size_t>(aZMissNumber) * p);
+    auto zact = ToDouble(apZActual, apZActual ? static_cast<size_t>(aZMissNumber) * p : 0);
+    double m3[3] = {-1.0, 0.0, 0.0};
+    CheckStatus(egs_predict_mspe(op, theta.data(), aZMissNumber, mx.data(), my.data(), mz.empty() ? nullptr : mz.data(),
+                                 zmiss.data(), zact.empty() ? nullptr : zact.data(), m3),
+                "ExaGeoStatMLEPredictTile");
+    for (size_t i = 0; i < zmiss.size(); ++i) apZMiss[i] = static_cast<T>(zmiss[i]);
+    const double dt = Seconds() - t0;
+    auto *res = results::Results::GetInstance();
+    res->SetMSPEExecutionTime(dt);
+    const double no = static_cast<double>(aZObsNumber) * p;
+    res->SetMSPEFlops((no * no * no / 3.0 + 2.0 * no * aZMissNumber * p) / 1e9 / std::max(dt, 1e-12));
+    if (apZActual) res->SetMSPEError(m3[0]);
+    T *all = new T[3];
+    for (int i = 0; i < 3; ++i) all[i] = static_cast<T>(m3[i]);
+    return all;
+}
+
+template class HipDense<float>;
+template class HipDense<double>;
+
+}  // namespace dense
+
+template<typename T>
+std::unique_ptr<LinearAlgebraMethods<T>> LinearAlgebraFactory<T>::CreateLinearAlgebraSolver(
+    common::Computation aComputation) {
+    if (aComputation == common::EXACT_DENSE) return std::make_unique<dense::HipDense<T>>();
+    if (aComputation == common::TILE_LOW_RANK)
+        throw std::runtime_error("Tile low rank generation isn't supported on the MI355X path (no HiCMA)");
+    throw std::domain_error("only EXACT_DENSE computation runs on the MI355X path");
+}
+
+template class LinearAlgebraFactory<float>;
+template class LinearAlgebraFactory<double>;
+
+}  // namespace linearAlgebra
+
+namespace api {
+
 template<typename T>
 void ExaGeoStat<T>::ExaGeoStatLoadData(configurations::Configurations &aConfigurations,
                                        std::unique_ptr<ExaGeoStatData<T>> &aData) {
+    if (aConfigurations.GetIsSynthetic()) {
+        ExaGeoStatGenerateData(aConfigurations, aData);
+        return;
+    }
     std::srand(aConfigurations.GetSeed());
     auto kernel = CreateKernel<T>(aConfigurations);
     const Dimension dim = aConfigurations.GetDimension();
     if (dim == common::DimensionST) throw std::domain_error("space-time data is not on the MI355X path");
-    if (!aConfigurations.GetIsSynthetic()) {
-        // real data from a CSV file (CSVLoader<T>::LoadData)
-        std::vector<T> z, xs, ys, zs;
-        dataLoader::csv::CSVLoader<T>::GetInstance()->ReadData(aConfigurations, z, xs, ys, zs,
-                                                               kernel->GetVariablesNumber());
-        const int nloc = static_cast<int>(xs.size());
-        aData = std::make_unique<ExaGeoStatData<T>>(nloc, dim);
-        aData->GetLocations()->SetLocationX(*xs.data(), nloc);
-        aData->GetLocations()->SetLocationY(*ys.data(), nloc);
-        if (!zs.empty()) aData->GetLocations()->SetLocationZ(*zs.data(), nloc);
-        aData->GetMeasurements() = ToDouble(z);
-        results::Results::GetInstance()->SetIsSynthetic(false);
-        results::Results::GetInstance()->SetGeneratedLocationsNumber(nloc);
-        return;
-    }
+    // real data from a CSV file (CSVLoader<T>::LoadData)
+    std::vector<T> z, xs, ys, zs;
+    dataLoader::csv::CSVLoader<T>::GetInstance()->ReadData(aConfigurations, z, xs, ys, zs,
+                                                           kernel->GetVariablesNumber());
+    const int nloc = static_cast<int>(xs.size());
+    aData = std::make_unique<ExaGeoStatData<T>>(nloc, dim);
+    aData->GetLocations()->SetLocationX(*xs.data(), nloc);
+    aData->GetLocations()->SetLocationY(*ys.data(), nloc);
+    if (!zs.empty()) aData->GetLocations()->SetLocationZ(*zs.data(), nloc);
+    aData->GetMeasurements() = ToDouble(z);
+    results::Results::GetInstance()->SetIsSynthetic(false);
+    results::Results::GetInstance()->SetGeneratedLocationsNumber(nloc);
+}
+
+template<typename T>
+void ExaGeoStat<T>::ExaGeoStatGenerateData(configurations::Configurations &aConfigurations,
+                                           std::unique_ptr<ExaGeoStatData<T>> &aData) {
+    std::srand(aConfigurations.GetSeed());
+    auto kernel = CreateKernel<T>(aConfigurations);
+    const Dimension dim = aConfigurations.GetDimension();
+    if (dim == common::DimensionST) throw std::domain_error("space-time data is not on the MI355X path");
+    auto solver = linearAlgebra::LinearAlgebraFactory<T>::CreateLinearAlgebraSolver(aConfigurations.GetComputation());
     const int n = aConfigurations.GetProblemSize();
     const double t0 = Seconds();
     std::vector<double> x(n), y(n), z(dim == common::Dimension3D ? n : 0);
@@ -442,15 +663,9 @@ void ExaGeoStat<T>::ExaGeoStatLoadData(configurations::Configurations &aConfigur
         loc->GetLocationY()[i] = static_cast<T>(y[i]);
         if (!z.empty()) loc->GetLocationZ()[i] = static_cast<T>(z[i]);
     }
-    std::vector<double> theta = aConfigurations.GetInitialTheta();
-    if (static_cast<int>(theta.size()) < kernel->GetParametersNumbers())
-        throw std::domain_error("initial theta needs " + std::to_string(kernel->GetParametersNumbers()) + " values");
-    egs_problem_t p = aData->GetProblem(ExaGeoStatHardware::GetContext(), kernel->GetDeviceKernelId(),
-                                        aConfigurations.GetDistanceMetric(), aConfigurations.GetDenseTileSize());
+    solver->GenerateObservationsVector(aConfigurations, aData, loc, loc, nullptr, aConfigurations.GetDistanceMetric(),
+                                       *kernel);
     std::vector<double> &obs = aData->GetMeasurements();
-    obs.assign(static_cast<size_t>(n) * kernel->GetVariablesNumber(), 0.0);
-    CheckStatus(egs_generate_observations(p, theta.data(), aConfigurations.GetSeed(), obs.data()),
-                "GenerateObservationsVector");
     if (aConfigurations.GetLogger()) {
         std::vector<T> zt(obs.begin(), obs.end());
         std::string path = aConfigurations.GetLoggerPath();
@@ -471,16 +686,13 @@ T ExaGeoStat<T>::ExaGeoStatDataModeling(configurations::Configurations &aConfigu
                                         std::unique_ptr<ExaGeoStatData<T>> &aData, T *apMeasurementsMatrix) {
     if (!aData) throw std::runtime_error("ExaGeoStatDataModeling: no data");
     auto kernel = CreateKernel<T>(aConfigurations);
+    auto solver = linearAlgebra::LinearAlgebraFactory<T>::CreateLinearAlgebraSolver(aConfigurations.GetComputation());
     aConfigurations.InitializeAllTheta();
     const int np = kernel->GetParametersNumbers();
     const size_t nz = static_cast<size_t>(aData->GetLocations()->GetSize()) * kernel->GetVariablesNumber();
     std::vector<double> &obs = aData->GetMeasurements();
     if (apMeasurementsMatrix) obs = ToDouble(apMeasurementsMatrix, nz);
     if (obs.size() != nz) throw std::runtime_error("ExaGeoStatDataModeling: no measurements");
-    egs_problem_t p = aData->GetProblem(ExaGeoStatHardware::GetContext(), kernel->GetDeviceKernelId(),
-                                        aConfigurations.GetDistanceMetric(), aConfigurations.GetDenseTileSize());
-    CheckStatus(egs_problem_set_observations(p, obs.data(), (int64_t) nz), "egs_problem_set_observations");
-    ConfigurePrecision(p, aConfigurations);
 
     std::vector<double> lb = aConfigurations.GetLowerBounds(), ub = aConfigurations.GetUpperBounds();
     std::vector<double> x = aConfigurations.GetStartingTheta();
@@ -491,49 +703,42 @@ T ExaGeoStat<T>::ExaGeoStatDataModeling(configurations::Configurations &aConfigu
         x[i] = std::min(ub[i], std::max(lb[i], x[i]));
     }
 
-    // One evaluation of ExaGeoStatMLETile (ChameleonImplementation.cpp:35-327):
-    // llh, its time and flops, and the reference's per-evaluation summary keys.
+    // StarpuRuntimeSolver<T>::DataModelingAPI (StarpuRuntimeSolver.cpp:67-79):
+    // one ExaGeoStatMLETile per optimizer point, with the modeling totals
     struct State {
-        egs_problem_t p;
+        linearAlgebra::LinearAlgebraMethods<T> *solver;
+        std::unique_ptr<ExaGeoStatData<T>> *data;
+        configurations::Configurations *conf;
+        const kernels::Kernel<T> *kernel;
         std::vector<double> theta;
         const std::vector<int> *free_idx;
-        ExaGeoStatData<T> *data;
         double n, total_time = 0.0, total_flops = 0.0, best = -INFINITY;
         std::vector<double> best_theta;
-        int status = EGS_SUCCESS;
+        bool failed = false;
         std::string error;
-        int count = 0;
-    } st{p, x, &free_idx, aData.get(), static_cast<double>(nz)};
+    } st{solver.get(), &aData, &aConfigurations, kernel.get(), x, &free_idx, static_cast<double>(nz)};
     auto objective = [](const double *v, void *user) -> double {
         auto *s = static_cast<State *>(user);
         for (size_t j = 0; j < s->free_idx->size(); ++j) s->theta[(*s->free_idx)[j]] = v[j];
         double llh = 0.0;
         const double t0 = Seconds();
-        const int code = egs_loglik(s->p, s->theta.data(), &llh, nullptr, nullptr);
-        const double dt = Seconds() - t0;
-        if (code != EGS_SUCCESS) {
-            if (s->status == EGS_SUCCESS) {
-                s->status = code;
-                s->error = egs_last_error();
+        try {
+            // the observations are the data's measurements (already set)
+            llh = static_cast<double>(s->solver->ExaGeoStatMLETile(*s->data, *s->conf, s->theta.data(), nullptr,
+                                                                    *s->kernel));
+        } catch (const std::exception &e) {
+            // the optimizer is C code: record the failure and stop it with NaN
+            if (!s->failed) {
+                s->failed = true;
+                s->error = e.what();
             }
             return NAN;
         }
-        ++s->count;
-        s->total_time += dt;
+        s->total_time += Seconds() - t0;
         s->total_flops += (s->n * s->n * s->n / 3.0 + 2.0 * s->n * s->n) / 1e9;
-        s->data->SetMleIterations(s->data->GetMleIterations() + 1);
         auto *res = results::Results::GetInstance();
-        res->SetMLEIterations(s->data->GetMleIterations());
-        res->SetMaximumTheta(s->theta);
-        res->SetLogLikValue(llh);
         res->SetTotalModelingExecutionTime(s->total_time);
         res->SetTotalModelingFlops(s->total_flops);
-        if (Verbose()) {
-            std::printf("\t%d- Model Parameters (", s->data->GetMleIterations());
-            for (size_t i = 0; i < s->theta.size(); ++i)
-                std::printf("%.8f%s", s->theta[i], i + 1 < s->theta.size() ? ", " : "");
-            std::printf(")----> LogLi: %.18f\n", llh);
-        }
         if (llh > s->best) {
             s->best = llh;
             s->best_theta = s->theta;
@@ -555,7 +760,7 @@ T ExaGeoStat<T>::ExaGeoStatDataModeling(configurations::Configurations &aConfigu
         egs_bobyqa(nf, xf.data(), lf.data(), uf.data(), aConfigurations.GetTolerance(),
                    aConfigurations.GetMaxMleIterations(), 0.0, objective, &st, &fmin, &nev, &reason);
     }
-    if (st.status != EGS_SUCCESS) throw std::runtime_error("ExaGeoStatMLETile failed: " + st.error);
+    if (st.failed) throw std::runtime_error(st.error);
     aConfigurations.SetStartingTheta(st.best_theta);
     aConfigurations.SetEstimatedTheta(st.best_theta);
     if (Verbose()) {
@@ -653,37 +858,17 @@ void ExaGeoStat<T>::ExaGeoStatPrediction(configurations::Configurations &aConfig
         z_obs = ToDouble(apMeasurementsMatrix, static_cast<size_t>(n_obs) * p);
     }
 
-    // ExaGeoStatMLEPredictTile (LinearAlgebraMethods.cpp:451-605) on the GPU.
-    const double t0 = Seconds();
-    auto ox = ToDouble(obs_loc.GetLocationX(), n_obs), oy = ToDouble(obs_loc.GetLocationY(), n_obs);
-    auto oz = ToDouble(obs_loc.GetLocationZ(), dim == common::Dimension2D ? 0 : n_obs);
-    auto mx = ToDouble(miss.GetLocationX(), z_miss), my = ToDouble(miss.GetLocationY(), z_miss);
-    auto mz = ToDouble(miss.GetLocationZ(), dim == common::Dimension2D ? 0 : z_miss);
-    egs_problem_t op = nullptr;
-    CheckStatus(egs_problem_create(ExaGeoStatHardware::GetContext(), kernel->GetDeviceKernelId(), n_obs, ox.data(),
-                                   oy.data(), oz.empty() ? nullptr : oz.data(), aConfigurations.GetDistanceMetric(),
-                                   aConfigurations.GetDenseTileSize(), &op),
-                "egs_problem_create");
-    std::unique_ptr<egs_problem_s, int (*)(egs_problem_t)> guard(op, egs_problem_destroy);
-    CheckStatus(egs_problem_set_observations(op, z_obs.data(), (int64_t) z_obs.size()), "egs_problem_set_observations");
-    ConfigurePrecision(op, aConfigurations);
-    std::vector<double> zmiss(static_cast<size_t>(z_miss) * p);
-    double mspe = -1.0;
-    std::vector<double> theta(aConfigurations.GetEstimatedTheta().begin(),
-                              aConfigurations.GetEstimatedTheta().begin() + np);
-    CheckStatus(egs_predict(op, theta.data(), z_miss, mx.data(), my.data(), mz.empty() ? nullptr : mz.data(),
-                            zmiss.data(), z_actual.empty() ? nullptr : z_actual.data(), &mspe),
-                "ExaGeoStatMLEPredictTile");
-    const double dt = Seconds() - t0;
-    auto *res = results::Results::GetInstance();
-    res->SetMSPEExecutionTime(dt);
-    const double no = static_cast<double>(n_obs) * p;
-    res->SetMSPEFlops((no * no * no / 3.0 + 2.0 * no * z_miss * p) / 1e9 / std::max(dt, 1e-12));
-    if (!z_actual.empty()) {
-        res->SetMSPEError(mspe);
-        if (Verbose()) std::printf("\t\t- Prediction value: %f\n", mspe);
-    }
-    res->SetPredictedMissedValues(zmiss);
+    // Prediction.cpp:165-197: the solver's ExaGeoStatMLEPredictTile
+    auto solver = linearAlgebra::LinearAlgebraFactory<T>::CreateLinearAlgebraSolver(aConfigurations.GetComputation());
+    std::vector<T> zobs_t(z_obs.begin(), z_obs.end()), zact_t(z_actual.begin(), z_actual.end());
+    std::vector<T> zmiss_t(static_cast<size_t>(z_miss) * p);
+    std::vector<T> theta(aConfigurations.GetEstimatedTheta().begin(), aConfigurations.GetEstimatedTheta().begin() + np);
+    std::unique_ptr<T[]> errors(solver->ExaGeoStatMLEPredictTile(aData, theta.data(), z_miss, n_obs, zobs_t.data(),
+                                                                 zact_t.empty() ? nullptr : zact_t.data(),
+                                                                 zmiss_t.data(), aConfigurations, miss, obs_loc,
+                                                                 *kernel));
+    if (!z_actual.empty() && Verbose()) std::printf("\t\t- Prediction value: %f\n", static_cast<double>(errors[0]));
+    results::Results::GetInstance()->SetPredictedMissedValues(std::vector<double>(zmiss_t.begin(), zmiss_t.end()));
 }
 
 template class ExaGeoStat<float>;

@@ -33,6 +33,10 @@ enum Dimension { Dimension2D = 0, Dimension3D = 1, DimensionST = 2 };
 enum DistanceMetric { EUCLIDEAN_DISTANCE = 0, GREAT_CIRCLE_DISTANCE = 1 };
 enum Computation { EXACT_DENSE = 0, DIAGONAL_APPROX = 1, TILE_LOW_RANK = 2 };
 enum Precision { SINGLE = 0, DOUBLE = 1, MIXED = 2 };
+enum UpperLower : int { EXAGEOSTAT_UPPER = 121, EXAGEOSTAT_LOWER = 122, EXAGEOSTAT_UPPER_LOWER = 123 };
+enum Side { EXAGEOSTAT_LEFT = 141, EXAGEOSTAT_RIGHT = 142 };
+enum Trans { EXAGEOSTAT_NO_TRANS = 111, EXAGEOSTAT_TRANS = 112, EXAGEOSTAT_CONJ_TRANS = 113 };
+enum Diag { EXAGEOSTAT_NON_UNIT = 131, EXAGEOSTAT_UNIT = 132 };
 }  // namespace common
 
 // Throws std::runtime_error with the library's message when status != 0
@@ -215,6 +219,9 @@ public:
     // Device problem for (kernel, metric, dts); rebuilt when any of them changes.
     egs_problem_t GetProblem(egs_context_t aContext, int aKernelId, int aMetric, int aDts);
     void ReleaseProblem();
+    // Sets the problem's precision plan (mode, FP64 band, FP32 band) unless it
+    // is already in effect; a plan change re-lays out the tiles on the device.
+    void SetProblemPlan(int aMode, int aBandDP, int aBandSP);
 
 private:
     std::unique_ptr<exageostat::dataunits::Locations<T>> mLocations;
@@ -222,6 +229,7 @@ private:
     std::vector<double> mZ;
     egs_problem_t mProblem = nullptr;
     int mKernelId = -1, mMetric = -1, mDts = -1;
+    int mPlan[3] = {-1, -1, -1};
 };
 
 
@@ -396,6 +404,109 @@ private:
 
 }  // namespace exageostat
 
+namespace exageostat {
+namespace dataunits {
+
+// A vector descriptor of the dense solver: caller memory and its length (the
+// reference's Z descriptor aliases user memory, LinearAlgebraMethods.cpp:75-77).
+// Matrix descriptors are the device-resident problem (an egs_problem_t holding
+// the lower tiles of Sigma, then of L).
+template<typename T>
+struct VectorDescriptor {
+    T *data = nullptr;
+    int64_t size = 0;
+};
+
+}  // namespace dataunits
+
+namespace linearAlgebra {
+
+// The solver interface (LinearAlgebraMethods.hpp:36-330): the reference's pure
+// virtuals with their signatures (:140-142, :153, :161, :170, :185-186,
+// :203-205), plus the three base-class operations that the reference
+// implements with direct CHAMELEON_* calls (GenerateObservationsVector,
+// ExaGeoStatPosvTile, ExaGeoStatMLEPredictTile; LinearAlgebraMethods.cpp:333-605,
+// 1421-1428), made virtual here so that the concrete solver owns them.
+// Descriptors (void *) are egs_problem_t for matrices and
+// dataunits::VectorDescriptor<T> * for vectors; a sequence is an
+// egs_context_t.  Failures throw std::runtime_error.
+template<typename T>
+class LinearAlgebraMethods {
+public:
+    virtual ~LinearAlgebraMethods() = default;
+
+    virtual T ExaGeoStatMLETile(std::unique_ptr<ExaGeoStatData<T>> &aData,
+                                configurations::Configurations &aConfigurations, const double *apTheta,
+                                T *apMeasurementsMatrix, const kernels::Kernel<T> &aKernel) = 0;
+    virtual void ExaGeoStatLapackCopyTile(const common::UpperLower &aUpperLower, void *apA, void *apB) = 0;
+    virtual void ExaGeoStatSequenceWait(void *apSequence) = 0;
+    virtual void ExaGeoStatCreateSequence(void *apSequence) = 0;
+    virtual void ExaGeoStatPotrfTile(const common::UpperLower &aUpperLower, void *apA, int aBand, void *apCD,
+                                     void *apCrk, const int &aMaxRank, const int &aAcc) = 0;
+    virtual void ExaGeoStatTrsmTile(const common::Side &aSide, const common::UpperLower &aUpperLower,
+                                    const common::Trans &aTrans, const common::Diag &aDiag, const T &aAlpha,
+                                    void *apA, void *apCD, void *apCrk, void *apZ, const int &aMaxRank) = 0;
+
+    virtual void GenerateObservationsVector(configurations::Configurations &aConfigurations,
+                                            std::unique_ptr<ExaGeoStatData<T>> &aData,
+                                            dataunits::Locations<T> *apLocation1, dataunits::Locations<T> *apLocation2,
+                                            dataunits::Locations<T> *apLocation3, const int &aDistanceMetric,
+                                            const kernels::Kernel<T> &aKernel) = 0;
+    virtual void ExaGeoStatPosvTile(const common::UpperLower &aUpperLower, void *apA, void *apB) = 0;
+    // aZMissNumber / aZObsNumber count locations; the vectors hold P values per
+    // location.  Returns new T[3] = {mspe, mspe1, mspe2} (the caller deletes
+    // it), as LinearAlgebraMethods.cpp:596-600 does.
+    virtual T *ExaGeoStatMLEPredictTile(std::unique_ptr<ExaGeoStatData<T>> &aData, T *apTheta,
+                                        const int &aZMissNumber, const int &aZObsNumber, T *apZObs, T *apZActual,
+                                        T *apZMiss, configurations::Configurations &aConfiguration,
+                                        dataunits::Locations<T> &aMissLocations,
+                                        dataunits::Locations<T> &aObsLocations,
+                                        const kernels::Kernel<T> &aKernel) = 0;
+};
+
+namespace dense {
+
+// The EXACT_DENSE solver on the MI355X: the role of ChameleonDense<T> with
+// ChameleonImplementation<T> (ChameleonDense.cpp:25-37,
+// ChameleonImplementation.cpp:35-365), every step on the GPU through the C ABI.
+template<typename T>
+class HipDense : public LinearAlgebraMethods<T> {
+public:
+    T ExaGeoStatMLETile(std::unique_ptr<ExaGeoStatData<T>> &aData, configurations::Configurations &aConfigurations,
+                        const double *apTheta, T *apMeasurementsMatrix, const kernels::Kernel<T> &aKernel) override;
+    void ExaGeoStatLapackCopyTile(const common::UpperLower &aUpperLower, void *apA, void *apB) override;
+    void ExaGeoStatSequenceWait(void *apSequence) override;
+    void ExaGeoStatCreateSequence(void *apSequence) override;
+    void ExaGeoStatPotrfTile(const common::UpperLower &aUpperLower, void *apA, int aBand, void *apCD, void *apCrk,
+                             const int &aMaxRank, const int &aAcc) override;
+    void ExaGeoStatTrsmTile(const common::Side &aSide, const common::UpperLower &aUpperLower,
+                            const common::Trans &aTrans, const common::Diag &aDiag, const T &aAlpha, void *apA,
+                            void *apCD, void *apCrk, void *apZ, const int &aMaxRank) override;
+    void GenerateObservationsVector(configurations::Configurations &aConfigurations,
+                                    std::unique_ptr<ExaGeoStatData<T>> &aData, dataunits::Locations<T> *apLocation1,
+                                    dataunits::Locations<T> *apLocation2, dataunits::Locations<T> *apLocation3,
+                                    const int &aDistanceMetric, const kernels::Kernel<T> &aKernel) override;
+    void ExaGeoStatPosvTile(const common::UpperLower &aUpperLower, void *apA, void *apB) override;
+    T *ExaGeoStatMLEPredictTile(std::unique_ptr<ExaGeoStatData<T>> &aData, T *apTheta, const int &aZMissNumber,
+                                const int &aZObsNumber, T *apZObs, T *apZActual, T *apZMiss,
+                                configurations::Configurations &aConfiguration,
+                                dataunits::Locations<T> &aMissLocations, dataunits::Locations<T> &aObsLocations,
+                                const kernels::Kernel<T> &aKernel) override;
+};
+
+}  // namespace dense
+
+// LinearAlgebraFactory.cpp:31-54: EXACT_DENSE gives the MI355X dense solver;
+// the tile-low-rank and diagonal-super-tile solvers are not on this path.
+template<typename T>
+class LinearAlgebraFactory {
+public:
+    static std::unique_ptr<LinearAlgebraMethods<T>> CreateLinearAlgebraSolver(common::Computation aComputation);
+};
+
+}  // namespace linearAlgebra
+}  // namespace exageostat
+
 // Runtime bring-up (ExaGeoStatHardware.hpp): one GPU context shared by the
 // API calls while the object lives.  The reference's core count drives
 // StarPU workers; here the host side is single-threaded and the GPU count may
@@ -423,12 +534,20 @@ template<typename T>
 class ExaGeoStat {
 public:
     // ExaGeoStat.cpp:73-88: srand(seed), generate locations (Morton-ordered,
-    // 16-bit snapped), then z = L(initial theta) e with the dlarnv stream.
+    // 16-bit snapped), then z = L(initial theta) e with the dlarnv stream
+    // (the solver's GenerateObservationsVector); real data is read from the
+    // configured CSV file.
     static void ExaGeoStatLoadData(configurations::Configurations &aConfigurations,
                                    std::unique_ptr<ExaGeoStatData<T>> &aData);
+    // Synthetic data generation: ExaGeoStatLoadData's synthetic branch,
+    // whatever the configuration's data path says.
+    static void ExaGeoStatGenerateData(configurations::Configurations &aConfigurations,
+                                       std::unique_ptr<ExaGeoStatData<T>> &aData);
     // ExaGeoStat.cpp:91-107 + StarpuRuntimeSolver.cpp:28-79: BOBYQA on the
-    // log-likelihood from the lower bounds; returns the best log-likelihood
-    // and stores the best theta as the estimated theta.
+    // log-likelihood from the lower bounds, each evaluation the solver's
+    // ExaGeoStatMLETile (the solver from LinearAlgebraFactory for the
+    // configured computation); returns the best log-likelihood and stores the
+    // best theta as the estimated theta.
     static T ExaGeoStatDataModeling(configurations::Configurations &aConfigurations,
                                     std::unique_ptr<ExaGeoStatData<T>> &aData, T *apMeasurementsMatrix = nullptr);
     // ExaGeoStat.cpp:111-125 + Prediction.cpp:35-210: kriging of the missing

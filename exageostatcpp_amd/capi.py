@@ -4,9 +4,11 @@ The library must be built (``__graft_entry__.build()`` or
 ``python exageostatcpp_amd/build.py``); loading fails loudly otherwise.  There
 is no CPU fallback: every numerical result comes from the gfx950 kernels.
 """
+import atexit
 import ctypes
 import ctypes.util
 import os
+import weakref
 
 import numpy as np
 
@@ -78,6 +80,7 @@ EXPORTS = {
     "egs_potrf": ([_vp], _int),
     "egs_solve_logdet_quad": ([_vp, _dp, _dp], _int),
     "egs_get_matrix": ([_vp, _dp, _i64], _int),
+    "egs_trsm_vector": ([_vp, _int, ctypes.c_double, _dp, _i64], _int),
     "egs_last_timings": ([_vp, _dp, _dp, _dp], _int),
     "egs_last_refinement": ([_vp, ctypes.POINTER(_int), _dp], _int),
     "egs_last_product_counts": ([_vp, ctypes.POINTER(_i64), ctypes.POINTER(_i64)], _int),
@@ -86,6 +89,7 @@ EXPORTS = {
     "egs_generate_locations": ([_i64, _int, _dp, _dp, _dp], _int),
     "egs_generate_observations": ([_vp, _dp, _int, _dp], _int),
     "egs_predict": ([_vp, _dp, _i64, _dp, _dp, _dp, _dp, _dp, _dp], _int),
+    "egs_predict_mspe": ([_vp, _dp, _i64, _dp, _dp, _dp, _dp, _dp, _dp], _int),
     "egs_bobyqa": ([_int, _dp, _dp, _dp, ctypes.c_double, _int, ctypes.c_double, _vp, _vp, _dp,
                     ctypes.POINTER(_int), ctypes.POINTER(_int)], _int),
     "egs_data_modeling": ([_vp, _dp, _dp, _dp, ctypes.c_double, _int, _dp, _dp, ctypes.POINTER(_int)], _int),
@@ -100,6 +104,28 @@ EXPORTS = {
 }
 
 OBJECTIVE_FN = ctypes.CFUNCTYPE(ctypes.c_double, _dp, _vp)
+
+# Handles still open when the interpreter exits are released here, servers
+# first, then problems, then contexts, while the HIP runtime is still up.
+# Left to garbage collection they would be freed in no fixed order during
+# interpreter teardown (a problem after its context, device calls from
+# finalizers), which the exit path must never depend on.
+_open_servers = weakref.WeakSet()
+_open_problems = weakref.WeakSet()
+_open_contexts = weakref.WeakSet()
+_exiting = False
+
+
+@atexit.register
+def _release_open_handles():
+    global _exiting
+    for group in (_open_servers, _open_problems, _open_contexts):
+        for obj in list(group):
+            try:
+                obj.close()
+            except Exception:
+                pass
+    _exiting = True
 
 
 def lib():
@@ -140,18 +166,30 @@ class Context:
     """One GPU: ExaGeoStatHardware's role for this library."""
 
     def __init__(self, device=0):
+        self.handle = None
         h = _vp()
         _check(lib().egs_init(device, ctypes.byref(h)))
         self.handle = h
         self.device = device
+        _open_contexts.add(self)
 
     def synchronize(self):
         _check(lib().egs_synchronize(self.handle))
 
     def close(self):
+        """egs_finalize: releases the streams; problems still open on this
+        context lose their device memory (closing them later is safe)."""
         if self.handle:
-            lib().egs_finalize(self.handle)
-            self.handle = None
+            h, self.handle = self.handle, None
+            _open_contexts.discard(self)
+            lib().egs_finalize(h)
+
+    def __del__(self):
+        if not _exiting:
+            try:
+                self.close()
+            except Exception:
+                pass
 
     def __enter__(self):
         return self
@@ -187,21 +225,25 @@ class Problem:
         self.n_loc = len(self.x)
         self.n = self.n_loc * self.p
         self.dts = dts
+        self.handle = None
         h = _vp()
         _check(lib().egs_problem_create(ctx.handle, self.kernel, self.n_loc, px, py, pz, metric, dts,
                                         ctypes.byref(h)))
         self.handle = h
+        _open_problems.add(self)
 
     def close(self):
         if self.handle:
-            lib().egs_problem_destroy(self.handle)
-            self.handle = None
+            h, self.handle = self.handle, None
+            _open_problems.discard(self)
+            lib().egs_problem_destroy(h)
 
     def __del__(self):
-        try:
-            self.close()
-        except Exception:
-            pass
+        if not _exiting:
+            try:
+                self.close()
+            except Exception:
+                pass
 
     def device_bytes(self):
         b = _i64()
@@ -243,6 +285,12 @@ class Problem:
         ld, q = ctypes.c_double(), ctypes.c_double()
         _check(lib().egs_solve_logdet_quad(self.handle, ctypes.byref(ld), ctypes.byref(q)))
         return ld.value, q.value
+
+    def trsm(self, z, trans=False, alpha=1.0):
+        """alpha L^{-1} z (or alpha L^{-T} z) with the current FP64 factor."""
+        z = np.array(z, dtype=np.float64)
+        _check(lib().egs_trsm_vector(self.handle, 1 if trans else 0, float(alpha), z.ctypes.data_as(_dp), len(z)))
+        return z
 
     def get_matrix(self):
         """Lower tiles of the current device matrix as an (n, n) array (upper part zero)."""
@@ -303,6 +351,19 @@ class Problem:
                                  ctypes.byref(mspe)))
         return out, mspe.value
 
+    def predict_mspe(self, theta, xm, ym, zm=None, z_actual=None):
+        """Prediction with the reference's three errors: (zmiss, [mspe, mspe1, mspe2])."""
+        th, pth = _arr(theta)
+        xm, pxm = _arr(xm)
+        ym, pym = _arr(ym)
+        zm, pzm = _arr(zm)
+        za, pza = _arr(z_actual)
+        out = np.empty(len(xm) * self.p)
+        m3 = np.empty(3)
+        _check(lib().egs_predict_mspe(self.handle, pth, len(xm), pxm, pym, pzm, out.ctypes.data_as(_dp), pza,
+                                      m3.ctypes.data_as(_dp)))
+        return out, m3
+
     def data_modeling(self, lb, ub, start=None, tolerance=1e-9, max_iterations=1000):
         """Maximum-likelihood fit (StarpuRuntimeSolver<T>::ModelingOperations): BOBYQA over
         [lb, ub] from start (default lb).  Returns (theta_hat, best loglik, evaluations)."""
@@ -346,21 +407,25 @@ class Server:
     """Request loop over many regions on one GPU (egs_server_*)."""
 
     def __init__(self, device=0, hbm_budget_bytes=64 << 30, workers=4):
+        self.handle = None
         h = _vp()
         _check(lib().egs_server_create(device, int(hbm_budget_bytes), workers, ctypes.byref(h)))
         self.handle = h
         self._p = {}
+        _open_servers.add(self)
 
     def close(self):
         if self.handle:
-            lib().egs_server_destroy(self.handle)
-            self.handle = None
+            h, self.handle = self.handle, None
+            _open_servers.discard(self)
+            lib().egs_server_destroy(h)
 
     def __del__(self):
-        try:
-            self.close()
-        except Exception:
-            pass
+        if not _exiting:
+            try:
+                self.close()
+            except Exception:
+                pass
 
     def add_region(self, kernel, x, y, obs, z=None, metric=0, dts=320):
         x, px = _arr(x)

@@ -124,6 +124,8 @@ struct egs_problem_s {
     double *d_tinv = nullptr, *d_ybuf = nullptr;
     double *d_mv = nullptr;                  // matrix-vector chunk partials
     double *d_dot = nullptr;                 // dot-product block partials
+    double *d_pred = nullptr;                // prediction targets and outputs, pred_cap doubles
+    int64_t pred_cap = 0;
     int refine_iters = 0;
     double refine_resid = 0.0;
 
@@ -302,6 +304,9 @@ void free_problem_buffers(egs_problem_s *p) {
     hipFree(p->d_mv);
     hipFree(p->d_dot);
     p->d_cg = p->d_mv = p->d_dot = nullptr;
+    hipFree(p->d_pred);
+    p->d_pred = nullptr;
+    p->pred_cap = 0;
     free_sigma_copy(p);
     free_tile_inverses(p);
     for (auto &e : p->ev)
@@ -854,6 +859,75 @@ int read_partials(egs_problem_s *p, double &logdet, double &quad) {
     return EGS_SUCCESS;
 }
 
+// A covariance tile request of egs_covariance_tile: only the locations the
+// tile touches are staged on the device, rows / P of the first set from
+// row_off / P on and cols / P of the second from col_off / P on.
+struct TileRequest {
+    int rows = 0, cols = 0;
+    int64_t s1 = 0, c1 = 0, s2 = 0, c2 = 0;   // first location and count of each set
+    bool three_d = false;
+    egs::MaternParams prm;
+    // output tile, then x, y, z of each location slice
+    int64_t scratch_doubles() const { return (int64_t) rows * cols + 3 * (c1 + c2); }
+};
+
+int check_tile_request(egs_context_t ctx, int kernel, const double *A, int rows, int cols, int row_off, int col_off,
+                       const double *x1, const double *y1, const double *z1, int64_t n1, const double *x2,
+                       const double *y2, const double *z2, int64_t n2, const double *theta, int metric,
+                       TileRequest &t) {
+    if (!ctx || !A || !x1 || !y1 || !x2 || !y2) return fail(EGS_ERR_INVALID_ARG, "null argument");
+    if (rows < 0 || cols < 0 || row_off < 0 || col_off < 0) return fail(EGS_ERR_INVALID_ARG, "negative size");
+    if (metric != 0 && metric != 1) return fail(EGS_ERR_INVALID_ARG, "unknown distance metric %d", metric);
+    const int pvars = kernel == EGS_KERNEL_BIVARIATE_MATERN_PARSIMONIOUS ? 2 : 1;
+    if (kernel == EGS_KERNEL_BIVARIATE_MATERN_PARSIMONIOUS && ((row_off | col_off) & 1))
+        return fail(EGS_ERR_INVALID_ARG, "bivariate tiles need even row and column offsets");
+    if ((int64_t) row_off + rows > n1 * pvars || (int64_t) col_off + cols > n2 * pvars)
+        return fail(EGS_ERR_INVALID_ARG, "tile exceeds the location sets");
+    t.three_d = (kernel == EGS_KERNEL_UNIVARIATE_MATERN_STATIONARY) && z1 && z2;
+    if (t.three_d && metric == 1) return fail(EGS_ERR_INVALID_ARG, "Great Circle (GC) distance is only valid for 2D!");
+    int st = make_params(kernel, metric, theta, t.prm);
+    if (st) return st;
+    t.rows = rows;
+    t.cols = cols;
+    if (rows == 0 || cols == 0) return EGS_SUCCESS;
+    // row_off and col_off are multiples of P, so the slices start at a
+    // location and the tile's offsets inside them are 0
+    t.s1 = row_off / pvars;
+    t.c1 = ((int64_t) row_off + rows - 1) / pvars - t.s1 + 1;
+    t.s2 = col_off / pvars;
+    t.c2 = ((int64_t) col_off + cols - 1) / pvars - t.s2 + 1;
+    return EGS_SUCCESS;
+}
+
+// Stages the location slices in `buf` (scratch_doubles() long), evaluates the
+// tile into it and copies the tile to A; waits for the copy.
+int covariance_tile_on(egs_context_t ctx, const TileRequest &t, double *A, const double *x1, const double *y1,
+                       const double *z1, const double *x2, const double *y2, const double *z2, double *buf) {
+    hipStream_t s = ctx->stream;
+    double *d_out = buf;
+    double *dx1 = buf + (int64_t) t.rows * t.cols, *dy1 = dx1 + t.c1, *dz1 = dy1 + t.c1;
+    double *dx2 = dz1 + t.c1, *dy2 = dx2 + t.c2, *dz2 = dy2 + t.c2;
+    const size_t nb1 = sizeof(double) * (size_t) t.c1, nb2 = sizeof(double) * (size_t) t.c2;
+    hipError_t e = hipMemcpyAsync(dx1, x1 + t.s1, nb1, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(dy1, y1 + t.s1, nb1, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(dx2, x2 + t.s2, nb2, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(dy2, y2 + t.s2, nb2, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess && t.three_d) e = hipMemcpyAsync(dz1, z1 + t.s1, nb1, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess && t.three_d) e = hipMemcpyAsync(dz2, z2 + t.s2, nb2, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) {
+        egs::launch_covgen_rect(d_out, t.rows, t.cols, 0, 0, t.prm, dx1, dy1, t.three_d ? dz1 : nullptr, dx2, dy2,
+                                t.three_d ? dz2 : nullptr, s);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(A, d_out, sizeof(double) * (size_t) t.rows * t.cols, hipMemcpyDeviceToHost, s);
+    // wait even after a failed enqueue: the caller may free buf next
+    const hipError_t e2 = hipStreamSynchronize(s);
+    if (e == hipSuccess) e = e2;
+    if (e != hipSuccess) return fail(EGS_ERR_HIP, "covariance tile: %s", hipGetErrorString(e));
+    return EGS_SUCCESS;
+}
+
 uint64_t morton_spread(uint64_t v) {
     uint64_t out = 0;
     for (int b = 0; b < 16; ++b) out |= ((v >> b) & 1ull) << (4 * b);
@@ -921,42 +995,16 @@ int egs_synchronize(egs_context_t ctx) {
 int egs_covariance_tile(egs_context_t ctx, int kernel, double *A, int rows, int cols, int row_off, int col_off,
                         const double *x1, const double *y1, const double *z1, int64_t n1, const double *x2,
                         const double *y2, const double *z2, int64_t n2, const double *theta, int metric) {
-    if (!ctx || !A || !x1 || !y1 || !x2 || !y2) return fail(EGS_ERR_INVALID_ARG, "null argument");
-    if (rows < 0 || cols < 0 || row_off < 0 || col_off < 0) return fail(EGS_ERR_INVALID_ARG, "negative size");
-    if (metric != 0 && metric != 1) return fail(EGS_ERR_INVALID_ARG, "unknown distance metric %d", metric);
-    const int pvars = kernel == EGS_KERNEL_BIVARIATE_MATERN_PARSIMONIOUS ? 2 : 1;
-    if (kernel == EGS_KERNEL_BIVARIATE_MATERN_PARSIMONIOUS && ((row_off | col_off) & 1))
-        return fail(EGS_ERR_INVALID_ARG, "bivariate tiles need even row and column offsets");
-    if ((int64_t) row_off + rows > n1 * pvars || (int64_t) col_off + cols > n2 * pvars)
-        return fail(EGS_ERR_INVALID_ARG, "tile exceeds the location sets");
-    const bool three_d = (kernel == EGS_KERNEL_UNIVARIATE_MATERN_STATIONARY) && z1 && z2;
-    if (three_d && metric == 1) return fail(EGS_ERR_INVALID_ARG, "Great Circle (GC) distance is only valid for 2D!");
-    egs::MaternParams prm;
-    int st = make_params(kernel, metric, theta, prm);
-    if (st) return st;
-    if (rows == 0 || cols == 0) return EGS_SUCCESS;
+    TileRequest t;
+    int st = check_tile_request(ctx, kernel, A, rows, cols, row_off, col_off, x1, y1, z1, n1, x2, y2, z2, n2, theta,
+                                metric, t);
+    if (st || rows == 0 || cols == 0) return st;
     EGS_HIP(hipSetDevice(ctx->device));
     double *buf = nullptr;
-    const size_t nb1 = sizeof(double) * (size_t) n1, nb2 = sizeof(double) * (size_t) n2;
-    const size_t out_bytes = sizeof(double) * (size_t) rows * cols;
-    EGS_HIP(hipMalloc(&buf, out_bytes + 3 * nb1 + 3 * nb2));
-    double *d_out = buf;
-    double *dx1 = buf + (size_t) rows * cols, *dy1 = dx1 + n1, *dz1 = dy1 + n1;
-    double *dx2 = dz1 + n1, *dy2 = dx2 + n2, *dz2 = dy2 + n2;
-    hipStream_t s = ctx->stream;
-    hipMemcpyAsync(dx1, x1, nb1, hipMemcpyHostToDevice, s);
-    hipMemcpyAsync(dy1, y1, nb1, hipMemcpyHostToDevice, s);
-    hipMemcpyAsync(dx2, x2, nb2, hipMemcpyHostToDevice, s);
-    hipMemcpyAsync(dy2, y2, nb2, hipMemcpyHostToDevice, s);
-    if (three_d) {
-        hipMemcpyAsync(dz1, z1, nb1, hipMemcpyHostToDevice, s);
-        hipMemcpyAsync(dz2, z2, nb2, hipMemcpyHostToDevice, s);
-    }
-    egs::launch_covgen_rect(d_out, rows, cols, row_off, col_off, prm, dx1, dy1, three_d ? dz1 : nullptr, dx2, dy2,
-                            three_d ? dz2 : nullptr, s);
-    hipError_t e = hipMemcpyAsync(A, d_out, out_bytes, hipMemcpyDeviceToHost, s);
-    if (e == hipSuccess) e = hipStreamSynchronize(s);
-    hipFree(buf);
+    EGS_HIP(hipMalloc(&buf, sizeof(double) * (size_t) t.scratch_doubles()));
+    st = covariance_tile_on(ctx, t, A, x1, y1, z1, x2, y2, z2, buf);
+    const hipError_t e = hipFree(buf);
+    if (st) return st;
     if (e != hipSuccess) return fail(EGS_ERR_HIP, "covariance tile: %s", hipGetErrorString(e));
     return EGS_SUCCESS;
 }
@@ -1160,6 +1208,27 @@ int egs_solve_logdet_quad(egs_problem_t p, double *logdet, double *quad) {
     }
     if (logdet) *logdet = ld;
     if (quad) *quad = q;
+    return EGS_SUCCESS;
+}
+
+int egs_trsm_vector(egs_problem_t p, int trans, double alpha, double *z, int64_t n) {
+    int st = check_problem(p);
+    if (st) return st;
+    if (!z || n != p->n) return fail(EGS_ERR_INVALID_ARG, "the vector must have n_loc * P = %lld entries", (long long) p->n);
+    if (trans != 0 && trans != 1) return fail(EGS_ERR_INVALID_ARG, "trans must be 0 (L) or 1 (L^T)");
+    if (!p->factored) return fail(EGS_ERR_STATE, "no factored matrix: call egs_potrf first");
+    if (is_mixed(p))
+        return fail(EGS_ERR_UNSUPPORTED, "the triangular solve with the stored factor runs under the FP64 plan");
+    hipStream_t s = p->ctx->stream;
+    EGS_HIP(hipMemcpyAsync(p->d_work, z, sizeof(double) * (size_t) n, hipMemcpyHostToDevice, s));
+    egs::launch_copy_pad(p->d_work, p->d_work, p->n, p->npad, s);
+    if (trans == 0) run_forward(p);
+    else run_backward(p);
+    EGS_HIP(hipGetLastError());
+    EGS_HIP(hipMemcpyAsync(z, p->d_work, sizeof(double) * (size_t) n, hipMemcpyDeviceToHost, s));
+    EGS_HIP(hipStreamSynchronize(s));
+    if (alpha != 1.0)
+        for (int64_t i = 0; i < n; ++i) z[i] *= alpha;
     return EGS_SUCCESS;
 }
 
@@ -1446,15 +1515,33 @@ int egs_generate_observations(egs_problem_t p, const double *theta, int seed, do
     return EGS_SUCCESS;
 }
 
-int egs_predict(egs_problem_t p, const double *theta, int64_t n_miss, const double *xm, const double *ym,
-                const double *zm, double *zmiss, const double *z_actual, double *mspe) {
+int egs_predict_mspe(egs_problem_t p, const double *theta, int64_t n_miss, const double *xm, const double *ym,
+                     const double *zm, double *zmiss, const double *z_actual, double *mspe3) {
     int st = check_problem(p);
     if (st) return st;
     if (!theta || !xm || !ym || !zmiss || n_miss <= 0) return fail(EGS_ERR_INVALID_ARG, "invalid prediction request");
     if (!p->has_obs) return fail(EGS_ERR_STATE, "the problem has no observations");
     const bool three_d = p->dim == 3;
     if (three_d && !zm) return fail(EGS_ERR_INVALID_ARG, "3D problem needs z coordinates of the targets");
+    egs::MaternParams prm;
+    st = make_params(p->kernel, p->metric, theta, prm);
+    if (st) return st;
     hipStream_t s = p->ctx->stream;
+    // targets and predictions share one buffer that the problem keeps (grown
+    // on demand), so repeated predictions allocate nothing
+    const int64_t rows = n_miss * p->p;
+    const int64_t need = rows + 3 * n_miss;
+    if (need > p->pred_cap) {
+        EGS_HIP(hipStreamSynchronize(s));
+        hipFree(p->d_pred);
+        p->d_pred = nullptr;
+        p->pred_cap = 0;
+        if (hipMalloc(&p->d_pred, sizeof(double) * (size_t) need) != hipSuccess) {
+            p->d_pred = nullptr;
+            return fail(EGS_ERR_OUT_OF_MEMORY, "cannot allocate the prediction buffer");
+        }
+        p->pred_cap = need;
+    }
     const bool reuse = p->factored && std::memcmp(p->factored_theta, theta, sizeof(double) * p->ntheta()) == 0;
     if (!reuse) {
         st = run_covariance(p, theta);
@@ -1472,13 +1559,7 @@ int egs_predict(egs_problem_t p, const double *theta, int64_t n_miss, const doub
         run_forward(p);
         run_backward(p);
     }
-    egs::MaternParams prm;
-    st = make_params(p->kernel, p->metric, theta, prm);
-    if (st) return st;
-    const int64_t rows = n_miss * p->p;
-    double *buf = nullptr;
-    EGS_HIP(hipMalloc(&buf, sizeof(double) * (size_t) (rows + 3 * n_miss)));
-    double *d_out = buf, *dxm = buf + rows, *dym = dxm + n_miss, *dzm = dym + n_miss;
+    double *d_out = p->d_pred, *dxm = d_out + rows, *dym = dxm + n_miss, *dzm = dym + n_miss;
     hipError_t e = hipMemcpyAsync(dxm, xm, sizeof(double) * n_miss, hipMemcpyHostToDevice, s);
     if (e == hipSuccess) e = hipMemcpyAsync(dym, ym, sizeof(double) * n_miss, hipMemcpyHostToDevice, s);
     if (e == hipSuccess && three_d) e = hipMemcpyAsync(dzm, zm, sizeof(double) * n_miss, hipMemcpyHostToDevice, s);
@@ -1488,8 +1569,8 @@ int egs_predict(egs_problem_t p, const double *theta, int64_t n_miss, const doub
         e = hipGetLastError();
     }
     if (e == hipSuccess) e = hipMemcpyAsync(zmiss, d_out, sizeof(double) * (size_t) rows, hipMemcpyDeviceToHost, s);
-    if (e == hipSuccess) e = hipStreamSynchronize(s);
-    hipFree(buf);
+    const hipError_t e2 = hipStreamSynchronize(s);
+    if (e == hipSuccess) e = e2;
     if (e != hipSuccess) return fail(EGS_ERR_HIP, "prediction: %s", hipGetErrorString(e));
     if (!reuse) {
         st = read_info(p);
@@ -1497,16 +1578,35 @@ int egs_predict(egs_problem_t p, const double *theta, int64_t n_miss, const doub
         p->factored = true;
         std::memcpy(p->factored_theta, theta, sizeof(double) * p->ntheta());
     }
-    if (mspe) {
+    if (mspe3) {
         if (z_actual) {
-            double acc = 0.0;
-            for (int64_t i = 0; i < rows; ++i) acc += (z_actual[i] - zmiss[i]) * (z_actual[i] - zmiss[i]);
-            *mspe = acc / (double) rows;
+            // LinearAlgebraMethods.cpp:558-600 with the dmse and dmse-bivariate
+            // codelets: all values, then the first (even entries) and second
+            // (odd entries) variable of the bivariate kernel, 0 otherwise
+            double acc = 0.0, acc1 = 0.0, acc2 = 0.0;
+            for (int64_t i = 0; i < rows; ++i) {
+                const double d = (z_actual[i] - zmiss[i]) * (z_actual[i] - zmiss[i]);
+                acc += d;
+                if (i & 1) acc2 += d;
+                else acc1 += d;
+            }
+            mspe3[0] = acc / (double) rows;
+            mspe3[1] = p->p == 2 ? acc1 / (double) n_miss : 0.0;
+            mspe3[2] = p->p == 2 ? acc2 / (double) n_miss : 0.0;
         } else {
-            *mspe = -1.0;
+            mspe3[0] = -1.0;
+            mspe3[1] = mspe3[2] = 0.0;
         }
     }
     return EGS_SUCCESS;
+}
+
+int egs_predict(egs_problem_t p, const double *theta, int64_t n_miss, const double *xm, const double *ym,
+                const double *zm, double *zmiss, const double *z_actual, double *mspe) {
+    double m3[3] = {-1.0, 0.0, 0.0};
+    const int st = egs_predict_mspe(p, theta, n_miss, xm, ym, zm, zmiss, z_actual, m3);
+    if (st == EGS_SUCCESS && mspe) *mspe = m3[0];
+    return st;
 }
 
 int egs_bobyqa(int n, double *x, const double *lb, const double *ub, double ftol_abs, int maxeval, double rho_end,

@@ -156,6 +156,13 @@ int egs_covariance(egs_problem_t problem, const double *theta);
 int egs_potrf(egs_problem_t problem);
 int egs_solve_logdet_quad(egs_problem_t problem, double *logdet, double *quad);
 
+/* ChameleonImplementation<T>::ExaGeoStatTrsmTile with side Left, Lower,
+ * NonUnit (ChameleonImplementation.cpp:337-347): z <- alpha L^{-1} z
+ * (trans 0) or alpha L^{-T} z (trans 1) with the factor of the last egs_potrf
+ * or evaluation, for a host vector of n_loc * P entries (the reference's Z
+ * aliases caller memory, LinearAlgebraMethods.cpp:75-77).  FP64 plan only. */
+int egs_trsm_vector(egs_problem_t problem, int trans, double alpha, double *z, int64_t n);
+
 /* Copies the lower tiles of the current device matrix (Sigma after
  * egs_covariance, L after egs_potrf) into a column-major n x n host array
  * (ExaGeoStatDesc2Lap, LinearAlgebraMethods.cpp:1373-1380).  The strictly
@@ -207,6 +214,15 @@ int egs_generate_observations(egs_problem_t problem, const double *theta, int se
  * *mspe receives mean((z_actual - zmiss)^2) (dmse-codelet.cpp:40-73), else -1. */
 int egs_predict(egs_problem_t problem, const double *theta, int64_t n_miss, const double *xm, const double *ym,
                 const double *zm, double *zmiss, const double *z_actual, double *mspe);
+/* The same prediction returning the three errors ExaGeoStatMLEPredictTile
+ * returns (LinearAlgebraMethods.cpp:575-600): mspe3[0] over all n_miss * P
+ * values, and for the bivariate kernel mspe3[1], mspe3[2] over the first and
+ * second variable (the even and odd entries of the interleaved vector, as the
+ * dmse-bivariate codelet splits them, dmse-bivariate-codelet.cpp:77-100), each
+ * divided by n_miss; 0 and 0 for univariate kernels.  Without z_actual:
+ * {-1, 0, 0}. */
+int egs_predict_mspe(egs_problem_t problem, const double *theta, int64_t n_miss, const double *xm, const double *ym,
+                     const double *zm, double *zmiss, const double *z_actual, double *mspe3);
 
 /* ---- maximum-likelihood driver ----------------------------------------------
  * Bound-constrained derivative-free minimisation (Powell's BOBYQA), the

@@ -589,6 +589,34 @@ int orc_krige(int kernel, int64_t n_obs, int64_t n_miss, int dts,
     return st;
 }
 
+void orc_mspe(int kernel, int64_t nm, int dts, const double *z_actual, const double *zmiss, double *out) {
+    /* per-tile partials added in tile order (dmse-codelet.cpp:62-75;
+     * dmse-bivariate-codelet.cpp:77-100: within a tile, even i -> error 1, odd i -> error 2) */
+    double e = 0.0, e1 = 0.0, e2 = 0.0;
+    for (int64_t t0 = 0; t0 < nm; t0 += dts) {
+        int64_t rows = nm - t0 < dts ? nm - t0 : dts;
+        double l = 0.0, l1 = 0.0, l2 = 0.0;
+        for (int64_t i = 0; i < rows; i++) {
+            double d = pow(z_actual[t0 + i] - zmiss[t0 + i], 2);
+            if (i % 2 == 0) l1 += d;
+            else l2 += d;
+            l += d;
+        }
+        e += l;
+        e1 += l1;
+        e2 += l2;
+    }
+    /* LinearAlgebraMethods.cpp:575-577 */
+    out[0] = e / (double) nm;
+    if (orc_variables(kernel) == 2) {
+        out[1] = e1 / (double) (nm / 2);
+        out[2] = e2 / (double) (nm / 2);
+    } else {
+        out[1] = 0.0;
+        out[2] = 0.0;
+    }
+}
+
 void orc_shuffle(orc_rand_t *s, double *z, double *x, double *y, int n) {
     /* PredictionHelpers.cpp:141-168 */
     if (n <= 1) return;

@@ -114,6 +114,13 @@ int orc_krige(int kernel, int64_t n_obs, int64_t n_miss, int dts,
               const double *theta, int metric, const double *zobs,
               const double *z_actual, double *zmiss, double *mspe);
 
+/* Prediction errors over tiles of dts values, as the dmse / dmse-bivariate
+ * codelets accumulate them (dmse-codelet.cpp:62-75, dmse-bivariate-codelet.cpp:77-100)
+ * and LinearAlgebraMethods.cpp:575-600 normalizes them: out[0] = sum of all
+ * squared errors / nm; for the bivariate kernel out[1], out[2] = sums over the
+ * even (first variable) and odd (second variable) entries / (nm / 2), else 0. */
+void orc_mspe(int kernel, int64_t nm, int dts, const double *z_actual, const double *zmiss, double *out);
+
 /* Fisher-Yates shuffle of PredictionHelpers.cpp:141-168 (one array). */
 void orc_shuffle(orc_rand_t *s, double *z, double *x, double *y, int n);
 

@@ -53,6 +53,7 @@ def lib():
                                  _dp, _dp, _dp, _dp, _dp, _dp]
         L.orc_generate_observations.argtypes = [ctypes.c_int, _i64, ctypes.c_int, _dp, _dp, _dp, _dp,
                                                 ctypes.c_int, ctypes.c_int, _dp, _dp]
+        L.orc_mspe.argtypes = [ctypes.c_int, _i64, ctypes.c_int, _dp, _dp, _dp]
         L.orc_krige.argtypes = [ctypes.c_int, _i64, _i64, ctypes.c_int, _dp, _dp, _dp, _dp, _dp, ctypes.c_int,
                                 _dp, _dp, _dp, _dp]
         _lib = L
@@ -147,9 +148,10 @@ def covariance_tile(kernel, rows, cols, row_off, col_off, loc1, loc2, theta, met
     return A.T.copy()
 
 
-def covariance_matrix(kernel, loc1, loc2, theta, dts, metric=0, lower=True):
+def covariance_matrix(kernel, loc1, loc2, theta, dts, metric=0, lower=True, order="C"):
     """Sigma built tile-by-tile as DCMGCodelet::InsertTask (dcmg-codelet.cpp:47-74); returns a
-    numpy (m, n) array (lower tiles only when lower and square)."""
+    numpy (m, n) array (lower tiles only when lower and square), C-contiguous, or with
+    order="F" the column-major result itself (no copy)."""
     p = 2 if _kid(kernel) == 1 else 1
     m = len(loc1[0]) * p
     n = len(loc2[0]) * p
@@ -163,7 +165,7 @@ def covariance_matrix(kernel, loc1, loc2, theta, dts, metric=0, lower=True):
                                      _p(x2), _p(y2), _p(z2), _p(th), metric)
     if st:
         raise RuntimeError("oracle covariance failed (%d)" % st)
-    return C.T.copy()
+    return C.T if order == "F" else C.T.copy()
 
 
 def potrf(A, nb=256):
@@ -225,6 +227,15 @@ def krige(kernel, obs_loc, miss_loc, theta, zobs, dts, z_actual=None, metric=0):
     if st:
         raise RuntimeError("CHAMELEON_dposv_Tile Failed!")
     return zmiss, mspe.value
+
+
+def mspe(kernel, z_actual, zmiss, dts):
+    """[mspe, mspe1, mspe2] as ExaGeoStatMLEPredictTile returns them
+    (LinearAlgebraMethods.cpp:575-600; dmse / dmse-bivariate codelets)."""
+    za, zm = _f64(z_actual), _f64(zmiss)
+    out = np.zeros(3)
+    lib().orc_mspe(_kid(kernel), len(za), int(dts), _p(za), _p(zm), _p(out))
+    return out
 
 
 def shuffle(rng, z, x, y):

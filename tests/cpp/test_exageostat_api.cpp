@@ -7,6 +7,7 @@
 //   test_exageostat_api mle t0 t1 t2 llh
 //        n = 1600 fit from the lower bounds; theta-hat and llh compared with
 //        the optimum given on the command line (computed from the oracle)
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -21,6 +22,7 @@
 using namespace exageostat::common;
 using namespace exageostat::dataunits;
 using namespace exageostat::configurations;
+using exageostat::dataunits::VectorDescriptor;
 
 static int g_failures = 0;
 
@@ -153,6 +155,35 @@ static void cpu_cases() {
         }
         REQUIRE(threw);
     });
+    run("solver factory: EXACT_DENSE only (LinearAlgebraFactory.cpp:31-54)", [] {
+        using exageostat::linearAlgebra::LinearAlgebraFactory;
+        auto solver = LinearAlgebraFactory<double>::CreateLinearAlgebraSolver(EXACT_DENSE);
+        REQUIRE(solver != nullptr);
+        REQUIRE(dynamic_cast<exageostat::linearAlgebra::dense::HipDense<double> *>(solver.get()) != nullptr);
+        for (Computation c : {TILE_LOW_RANK, DIAGONAL_APPROX}) {
+            bool threw = false;
+            try {
+                LinearAlgebraFactory<double>::CreateLinearAlgebraSolver(c);
+            } catch (const std::exception &) {
+                threw = true;
+            }
+            REQUIRE(threw);
+        }
+    });
+    run("solver: Z copy between vector descriptors (ChameleonImplementation.cpp:329-335)", [] {
+        auto solver = exageostat::linearAlgebra::LinearAlgebraFactory<double>::CreateLinearAlgebraSolver(EXACT_DENSE);
+        std::vector<double> a(kZ16, kZ16 + 16), b(16, 0.0), c(3, 0.0);
+        VectorDescriptor<double> da{a.data(), 16}, db{b.data(), 16}, dc{c.data(), 3};
+        solver->ExaGeoStatLapackCopyTile(EXAGEOSTAT_UPPER_LOWER, &da, &db);
+        REQUIRE(b == a);
+        bool threw = false;
+        try {
+            solver->ExaGeoStatLapackCopyTile(EXAGEOSTAT_UPPER_LOWER, &da, &dc);
+        } catch (const std::runtime_error &) {
+            threw = true;
+        }
+        REQUIRE(threw);
+    });
     run("prediction without an estimated theta throws", [] {
         Configurations c;
         c.SetIsMSPE(true);
@@ -265,6 +296,146 @@ static void gpu_cases() {
         auto zm = exageostat::results::Results::GetInstance()->GetPredictedMissedValues();
         REQUIRE(zm.size() == 2);
         REQUIRE(std::fabs(zm[0] - (-1.04437)) < 1e-5 && std::fabs(zm[1] - (-1.63133)) < 1e-5);
+    });
+    run("solver: ExaGeoStatMLETile at BOBYQA's first three points gives -24.026000 (TestExaGeoStatApi.cpp:93)", [] {
+        Configurations c;
+        c.SetKernelName("UnivariateMaternStationary");
+        c.SetDenseTileSize(8);
+        auto hardware = ExaGeoStatHardware(EXACT_DENSE, 1, 0);
+        auto solver = exageostat::linearAlgebra::LinearAlgebraFactory<double>::CreateLinearAlgebraSolver(EXACT_DENSE);
+        using exageostat::kernels::Kernel;
+        std::unique_ptr<Kernel<double>> k(
+            exageostat::plugins::PluginRegistry<Kernel<double>>::Create("UnivariateMaternStationary", 1));
+        auto data = std::make_unique<ExaGeoStatData<double>>(16, Dimension2D);
+        data->GetLocations()->SetLocationX(*kX16, 16);
+        data->GetLocations()->SetLocationY(*kY16, 16);
+        double z[16];
+        std::memcpy(z, kZ16, sizeof(z));
+        // lb, lb + (ub - lb)/4 e_1, lb + (ub - lb)/4 e_2 with lb = 0.1, ub = 5
+        const double pts[3][3] = {{0.1, 0.1, 0.1}, {1.325, 0.1, 0.1}, {0.1, 1.325, 0.1}};
+        double best = -INFINITY;
+        for (auto &th : pts) best = std::max(best, solver->ExaGeoStatMLETile(data, c, th, z, *k));
+        REQUIRE(std::fabs(best - (-24.026000)) < 1e-6);
+        REQUIRE(data->GetMleIterations() == 3);
+        // the observations were only read
+        for (int i = 0; i < 16; ++i) REQUIRE(z[i] == kZ16[i]);
+    });
+    run("solver: GenerateObservationsVector reproduces z (TestExaGeoStatApi.cpp:52-54)", [] {
+        Configurations c;
+        c.SetKernelName("UnivariateMaternStationary");
+        c.SetDenseTileSize(9);
+        c.SetSeed(0);
+        c.SetInitialTheta({1, 0.1, 0.5});
+        auto hardware = ExaGeoStatHardware(EXACT_DENSE, 1, 0);
+        auto solver = exageostat::linearAlgebra::LinearAlgebraFactory<double>::CreateLinearAlgebraSolver(EXACT_DENSE);
+        using exageostat::kernels::Kernel;
+        std::unique_ptr<Kernel<double>> k(
+            exageostat::plugins::PluginRegistry<Kernel<double>>::Create("UnivariateMaternStationary", 1));
+        auto data = std::make_unique<ExaGeoStatData<double>>(16, Dimension2D);
+        data->GetLocations()->SetLocationX(*kX16, 16);
+        data->GetLocations()->SetLocationY(*kY16, 16);
+        solver->GenerateObservationsVector(c, data, data->GetLocations(), data->GetLocations(), nullptr, 0, *k);
+        for (int i = 0; i < 16; ++i) REQUIRE(std::fabs(data->GetMeasurements()[i] - kZ16[i]) < 1e-6);
+    });
+    run("solver: potrf, trsm and posv on the device descriptors", [] {
+        Configurations c;
+        c.SetKernelName("UnivariateMaternStationary");
+        c.SetDenseTileSize(8);
+        auto hardware = ExaGeoStatHardware(EXACT_DENSE, 1, 0);
+        auto solver = exageostat::linearAlgebra::LinearAlgebraFactory<double>::CreateLinearAlgebraSolver(EXACT_DENSE);
+        void *seq = nullptr;
+        solver->ExaGeoStatCreateSequence(&seq);
+        REQUIRE(seq == ExaGeoStatHardware::GetContext());
+        auto data = std::make_unique<ExaGeoStatData<double>>(16, Dimension2D);
+        data->GetLocations()->SetLocationX(*kX16, 16);
+        data->GetLocations()->SetLocationY(*kY16, 16);
+        egs_problem_t p = data->GetProblem(ExaGeoStatHardware::GetContext(), EGS_KERNEL_UNIVARIATE_MATERN_STATIONARY,
+                                           0, 8);
+        const double theta[3] = {1.0, 0.1, 0.5};
+        // Sigma on the host through the plugin (nu = 1/2: exp(-d / range))
+        std::vector<double> S(256);
+        for (int i = 0; i < 16; ++i)
+            for (int j = 0; j < 16; ++j) {
+                const double dx = kX16[i] - kX16[j], dy = kY16[i] - kY16[j];
+                S[i + 16 * j] = std::exp(-std::sqrt(dx * dx + dy * dy) / 0.1);
+            }
+        // potrf + trsm: ||L^-1 z||^2 equals z' Sigma^-1 z of an evaluation
+        exageostat::CheckStatus(egs_covariance(p, theta), "egs_covariance");
+        solver->ExaGeoStatPotrfTile(EXAGEOSTAT_LOWER, p, 0, nullptr, nullptr, 0, 0);
+        std::vector<double> w(kZ16, kZ16 + 16);
+        VectorDescriptor<double> dw{w.data(), 16};
+        solver->ExaGeoStatTrsmTile(EXAGEOSTAT_LEFT, EXAGEOSTAT_LOWER, EXAGEOSTAT_NO_TRANS, EXAGEOSTAT_NON_UNIT, 1.0, p,
+                                   nullptr, nullptr, &dw, 0);
+        double q = 0.0;
+        for (double v : w) q += v * v;
+        // posv: Sigma x = z
+        exageostat::CheckStatus(egs_covariance(p, theta), "egs_covariance");
+        std::vector<double> x(kZ16, kZ16 + 16);
+        VectorDescriptor<double> dx{x.data(), 16};
+        solver->ExaGeoStatPosvTile(EXAGEOSTAT_LOWER, p, &dx);
+        solver->ExaGeoStatSequenceWait(seq);
+        double zx = 0.0;
+        for (int i = 0; i < 16; ++i) {
+            double r = 0.0;
+            for (int j = 0; j < 16; ++j) r += S[i + 16 * j] * x[j];
+            REQUIRE(std::fabs(r - kZ16[i]) < 1e-10);
+            zx += kZ16[i] * x[i];
+        }
+        REQUIRE(std::fabs(q - zx) < 1e-10 * std::fabs(zx));
+        // an upper factor is not what the tiles hold
+        bool threw = false;
+        try {
+            solver->ExaGeoStatPotrfTile(EXAGEOSTAT_UPPER, p, 0, nullptr, nullptr, 0, 0);
+        } catch (const std::domain_error &) {
+            threw = true;
+        }
+        REQUIRE(threw);
+    });
+    run("solver: prediction returns {mspe, mspe1, mspe2} (LinearAlgebraMethods.cpp:596-600)", [] {
+        Configurations c;
+        c.SetKernelName("BivariateMaternParsimonious");
+        c.SetDenseTileSize(8);
+        auto hardware = ExaGeoStatHardware(EXACT_DENSE, 1, 0);
+        auto solver = exageostat::linearAlgebra::LinearAlgebraFactory<double>::CreateLinearAlgebraSolver(EXACT_DENSE);
+        using exageostat::kernels::Kernel;
+        std::unique_ptr<Kernel<double>> k(
+            exageostat::plugins::PluginRegistry<Kernel<double>>::Create("BivariateMaternParsimonious", 1));
+        Locations<double> obs(12, Dimension2D), miss(4, Dimension2D);
+        obs.SetLocationX(*kX16, 12);
+        obs.SetLocationY(*kY16, 12);
+        miss.SetLocationX(kX16[12], 4);
+        miss.SetLocationY(kY16[12], 4);
+        std::vector<double> zobs(24), zact(8), zmiss(8);
+        for (int i = 0; i < 24; ++i) zobs[i] = kZ16[i % 16] * (i % 2 ? 0.5 : 1.0);
+        for (int i = 0; i < 8; ++i) zact[i] = 0.1 * i - 0.3;
+        double theta[6] = {1.0, 1.0, 0.1, 0.5, 1.0, 0.5};
+        std::unique_ptr<ExaGeoStatData<double>> none;
+        std::unique_ptr<double[]> e(solver->ExaGeoStatMLEPredictTile(none, theta, 4, 12, zobs.data(), zact.data(),
+                                                                      zmiss.data(), c, miss, obs, *k));
+        double s = 0.0, s1 = 0.0, s2 = 0.0;
+        for (int i = 0; i < 8; ++i) {
+            const double d = (zact[i] - zmiss[i]) * (zact[i] - zmiss[i]);
+            s += d;
+            (i % 2 ? s2 : s1) += d;
+        }
+        REQUIRE(std::fabs(e[0] - s / 8) < 1e-14 && std::fabs(e[1] - s1 / 4) < 1e-14 && std::fabs(e[2] - s2 / 4) < 1e-14);
+        REQUIRE(std::fabs(e[0] - 0.5 * (e[1] + e[2])) < 1e-14);
+    });
+    run("ExaGeoStatGenerateData is the synthetic branch of ExaGeoStatLoadData", [] {
+        Configurations c;
+        c.SetSeed(0);
+        c.SetProblemSize(16);
+        c.SetKernelName("UnivariateMaternStationary");
+        c.SetDenseTileSize(9);
+        c.SetInitialTheta({1, 0.1, 0.5});
+        auto hardware = ExaGeoStatHardware(EXACT_DENSE, 1, 0);
+        std::unique_ptr<ExaGeoStatData<double>> a, b;
+        exageostat::api::ExaGeoStat<double>::ExaGeoStatGenerateData(c, a);
+        exageostat::api::ExaGeoStat<double>::ExaGeoStatLoadData(c, b);
+        for (int i = 0; i < 16; ++i) {
+            REQUIRE(a->GetMeasurements()[i] == b->GetMeasurements()[i]);
+            REQUIRE(std::fabs(a->GetMeasurements()[i] - kZ16[i]) < 1e-6);
+        }
     });
     run("kernel plugin evaluates a tile on the GPU", [] {
         auto hardware = ExaGeoStatHardware(EXACT_DENSE, 1, 0);

@@ -317,3 +317,62 @@ def test_kriging_matches_oracle(ctx, n_obs, dts, theta):
     zm, _ = p.predict(theta, xm, ym)
     zo, _ = orc.krige(0, (x, y), (xm, ym), theta, z, dts)
     assert np.max(np.abs(zm - zo)) < 1e-8
+
+
+@pytest.mark.parametrize("mixed", [False, True])
+def test_bivariate_kriging_matches_oracle(ctx, mixed):
+    """Bivariate kriging with the reference's three errors: predictions within
+    1e-8 of the oracle, [mspe, mspe1, mspe2] within 1e-8 relative (FP64 plan;
+    the mixed plan's refined solve within 1e-7 on the predictions)."""
+    log("bivariate kriging mixed=%s" % mixed)
+    theta = [1.0, 1.0, 0.1, 0.5, 1.0, 0.5]
+    x, y = orc.generate_locations(700, seed=21)
+    z = orc.generate_observations(1, (x, y), theta, 192, seed=2)
+    rng = np.random.default_rng(5)
+    xm, ym = rng.random(40), rng.random(40)
+    za = rng.standard_normal(80)
+    p = egs.Problem(ctx, 1, x, y, dts=192)
+    p.set_observations(z)
+    if mixed:
+        p.set_precision(egs.PRECISION_MIXED, 1, 1)
+    zm, m3 = p.predict_mspe(theta, xm, ym, z_actual=za)
+    zo, mo = orc.krige(1, (x, y), (xm, ym), theta, z, 192, z_actual=za)
+    mo3 = orc.mspe(1, za, zo, 192)
+    assert abs(mo3[0] - mo) <= 1e-14 * mo
+    tol = 1e-7 if mixed else 1e-8
+    assert np.max(np.abs(zm - zo)) < tol
+    assert np.allclose(m3, mo3, rtol=tol, atol=0)
+    # the pooled error through egs_predict is the first of the three
+    _, mspe = p.predict(theta, xm, ym, z_actual=za)
+    assert mspe == m3[0]
+    p.close()
+
+
+def test_univariate_predict_mspe_leaves_per_variable_errors_zero(ctx):
+    theta = [1.0, 0.1, 1.0]
+    x, y = orc.generate_locations(400, seed=4)
+    z = orc.generate_observations(0, (x, y), theta, 128, seed=3)
+    rng = np.random.default_rng(8)
+    xm, ym, za = rng.random(30), rng.random(30), rng.standard_normal(30)
+    p = egs.Problem(ctx, 0, x, y, dts=128)
+    p.set_observations(z)
+    zm, m3 = p.predict_mspe(theta, xm, ym, z_actual=za)
+    zo, mo = orc.krige(0, (x, y), (xm, ym), theta, z, 128, z_actual=za)
+    assert np.max(np.abs(zm - zo)) < 1e-8
+    assert abs(m3[0] - mo) <= 1e-8 * mo and m3[1] == 0.0 and m3[2] == 0.0
+    _, m_none = p.predict_mspe(theta, xm, ym)
+    assert list(m_none) == [-1.0, 0.0, 0.0]
+    p.close()
+
+
+@pytest.mark.parametrize("kernel,theta", [(0, [1.0, 0.1, 0.5]), (1, [1.0, 1.0, 0.1, 0.5, 1.0, 0.5])])
+def test_covariance_tile_interior_offsets_match_oracle(ctx, kernel, theta):
+    """A tile away from the origin of both location sets (the plugin stages only
+    the locations it touches): equal to the oracle's tile at 1e-13 relative,
+    including a last tile whose row count is odd for the bivariate kernel."""
+    x, y = orc.generate_locations(900, seed=6)
+    xs, ys = x[::-1].copy(), y[::-1].copy()
+    for rows, cols, r0, c0 in ((96, 64, 320, 640), (17, 33, 1000 if kernel else 500, 12), (5, 3, 0, 0)):
+        got = ctx.covariance_tile(kernel, rows, cols, r0, c0, (x, y), (xs, ys), theta)
+        ref = orc.covariance_tile(kernel, rows, cols, r0, c0, (x, y), (xs, ys), theta)
+        assert np.max(np.abs(got - ref)) <= 1e-13 * np.max(np.abs(ref))

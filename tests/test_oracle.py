@@ -343,3 +343,32 @@ def test_kriging_vs_numpy():
     C22 = orc.covariance_matrix(0, (xo, yo), (xo, yo), th, dts=120, lower=False)
     C12 = orc.covariance_matrix(0, (xm, ym), (xo, yo), th, dts=120, lower=False)
     assert np.allclose(zm, C12 @ np.linalg.solve(C22, zo), atol=1e-10)
+
+
+def test_mspe_per_variable_hand_computed():
+    """[mspe, mspe1, mspe2] of LinearAlgebraMethods.cpp:575-600: errors
+    1..6 in tiles of 4 give squared sums 91 (all), 1+9+25 (even entries, the
+    first variable) and 4+16+36 (odd entries), over 6, 3 and 3 values."""
+    out = orc.mspe(1, [1, 2, 3, 4, 5, 6], np.zeros(6), 4)
+    assert np.allclose(out, [91 / 6, 35 / 3, 56 / 3], rtol=0, atol=1e-15)
+    # univariate: the per-variable errors stay 0 as the reference leaves them
+    assert np.allclose(orc.mspe(0, [1, 2, 3], np.zeros(3), 2), [14 / 3, 0, 0], rtol=0, atol=1e-15)
+
+
+def test_mspe_matches_krige_and_splits_evenly():
+    """The pooled error equals orc_krige's and the mean of the two per-variable
+    errors (an equal number of values of each variable); the tile size does not
+    change the result beyond summation order."""
+    rng = np.random.default_rng(11)
+    xo, yo = rng.random(60), rng.random(60)
+    xm, ym = rng.random(9), rng.random(9)
+    th = [1.0, 1.0, 0.1, 0.5, 1.0, 0.5]
+    zo = rng.standard_normal(120)
+    za = rng.standard_normal(18)
+    zm, mspe = orc.krige(1, (xo, yo), (xm, ym), th, zo, 20, z_actual=za)
+    out = orc.mspe(1, za, zm, 20)
+    assert abs(out[0] - mspe) <= 1e-14 * mspe
+    assert abs(out[0] - 0.5 * (out[1] + out[2])) <= 1e-14 * out[0]
+    assert np.allclose(out, orc.mspe(1, za, zm, 6), rtol=1e-14, atol=0)
+    d = (za - zm) ** 2
+    assert np.allclose(out, [d.mean(), d[0::2].mean(), d[1::2].mean()], rtol=1e-14, atol=0)
