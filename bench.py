@@ -207,58 +207,110 @@ def kernel_table(ks, steps):
     return out
 
 
-def serve_phase(egs, args):
-    """Serving at a fixed request rate: regions with n log-uniform in
-    [500, 20000], Poisson arrivals, 80% likelihood evaluations at random theta
-    and 20% kriging of 100 new points at the theta of the region's previous
-    evaluation (a cached factor); latency is submission to completion."""
+def serve_workload(egs, args):
+    """The serving mix: args.serve_requests requests; 80% likelihood evaluations,
+    each for a new region with n log-uniform in [500, 20000] (jittered-grid
+    locations, theta uniform in [0.5, 2] x [0.02, 0.3] x [0.3, 2.5], z simulated
+    at theta with the FP64 factor on the GPU), 20% kriging of 100 new uniform
+    locations against one of the 16 regions most recently submitted (at its
+    theta, so the factor cached on the device is reused); Poisson arrival times
+    at args.serve_rate per second."""
     import numpy as np
 
     rng = np.random.default_rng(7)
-    sizes = np.exp(rng.uniform(np.log(500), np.log(20000), args.serve_regions)).astype(int)
-    ctx = egs.Context(0)
-    srv = egs.capi.Server(0, 64 << 30, args.serve_workers)
+    reqs, regions = [], []
+    for i in range(args.serve_requests):
+        if regions and rng.random() < 0.2:
+            r = int(rng.integers(max(0, len(regions) - 16), len(regions)))
+            reqs.append(("predict", r, rng.random(100), rng.random(100)))
+        else:
+            n = int(np.exp(rng.uniform(np.log(500), np.log(20000))))
+            th = [rng.uniform(0.5, 2.0), rng.uniform(0.02, 0.3), rng.uniform(0.3, 2.5)]
+            regions.append([n, th, 10000 + len(regions), None, None, None])
+            reqs.append(("loglik", len(regions) - 1, None, None))
+    arrivals = np.cumsum(rng.exponential(1.0 / args.serve_rate, len(reqs)))
     t0 = time.perf_counter()
-    ids, last = [], {}
-    for r, n in enumerate(sizes):
-        x, y = egs.generate_locations(int(n), seed=1000 + r)
-        th = [rng.uniform(0.5, 2.0), rng.uniform(0.02, 0.3), rng.uniform(0.3, 2.5)]
-        p = egs.Problem(ctx, "UnivariateMaternStationary", x, y, dts=320)
-        z = p.generate_observations(th, seed=r)
-        p.close()
-        ids.append(srv.add_region(0, x, y, z, dts=320))
-        last[r] = th
-    ctx.close()
-    log("serve: %d regions generated in %.1f s" % (len(sizes), time.perf_counter() - t0))
-    # warm-up: one evaluation per region (makes every region resident)
-    for t in [srv.submit_loglik(ids[r], last[r]) for r in range(len(sizes))]:
-        srv.wait(t)
+    egs.capi.memory_cache(64 << 30)
+    with egs.Context(0) as ctx:
+        for reg in regions:
+            n, th, seed = reg[0], reg[1], reg[2]
+            x, y = egs.generate_locations(n, seed=seed)
+            p = egs.Problem(ctx, "UnivariateMaternStationary", x, y, dts=320)
+            reg[3], reg[4], reg[5] = x, y, p.generate_observations(th, seed=seed)
+            p.close()
+    egs.capi.memory_cache(0)
+    log("serve: %d regions (n %d..%d) generated in %.1f s" % (len(regions), min(r[0] for r in regions),
+                                                              max(r[0] for r in regions), time.perf_counter() - t0))
+    return reqs, regions, arrivals
+
+
+def serve_run(egs, args, reqs, regions, arrivals, mode):
+    """Runs the serving mix through one server under `mode`; returns the
+    summary and the per-request results."""
+    import numpy as np
+
+    mixed = mode != egs.PRECISION_FP64
+    dts = 384 if mixed else 320   # mixed plans need a multiple of 192
+    srv = egs.capi.Server(0, args.serve_budget_gb << 30, args.serve_workers)
+    ids = [srv.add_region(0, r[3], r[4], r[5], dts=dts, mode=mode, band_fp64=args.band_fp64,
+                          band_fp32=args.band_fp32) for r in regions]
+    # warm-up on two extra small regions (kernel loading), then fresh statistics
+    rng = np.random.default_rng(11)
+    for k in range(2):
+        x, y = egs.generate_locations(600, seed=90 + k)
+        w = srv.add_region(0, x, y, rng.standard_normal(600), dts=dts, mode=mode)
+        srv.wait(srv.submit_loglik(w, [1.0, 0.1, 0.5]))
     srv.reset_stats()
-    arrivals = np.cumsum(rng.exponential(1.0 / args.serve_rate, args.serve_requests))
     tickets = []
     start = time.perf_counter()
-    for i, ta in enumerate(arrivals):
+    for (kind, r, xm, ym), ta in zip(reqs, arrivals):
         dt = start + ta - time.perf_counter()
         if dt > 0:
             time.sleep(dt)
-        r = int(rng.integers(len(sizes)))
-        if rng.random() < 0.2:
-            tickets.append(srv.submit_predict(ids[r], last[r], rng.random(100), rng.random(100)))
+        th = regions[r][1]
+        if kind == "predict":
+            tickets.append(srv.submit_predict(ids[r], th, xm, ym))
         else:
-            th = [rng.uniform(0.5, 2.0), rng.uniform(0.02, 0.3), rng.uniform(0.3, 2.5)]
-            last[r] = th
             tickets.append(srv.submit_loglik(ids[r], th))
-    for t in tickets:
-        srv.wait(t)
+    out = []
+    for i, t in enumerate(tickets):
+        v, zm, _ = srv.wait(t)
+        out.append(zm if reqs[i][0] == "predict" else v)
     wall = time.perf_counter() - start
     st = srv.stats()
+    bs = srv.batch_stats()
     srv.close()
-    log("serve: %d requests at %.0f/s: p50 %.1f ms, p99 %.1f ms" % (len(tickets), args.serve_rate, st["p50_ms"],
-                                                                      st["p99_ms"]))
-    return {"requests": len(tickets), "rate_per_s": args.serve_rate, "regions": len(sizes),
-            "n_range": [int(sizes.min()), int(sizes.max())], "workers": args.serve_workers,
-            "mix": "80% llh at random theta, 20% kriging of 100 points at the region's last theta",
-            "p50_ms": st["p50_ms"], "p99_ms": st["p99_ms"], "completed_per_s": len(tickets) / wall}
+    log("serve %s: %d requests at %.0f/s: p50 %.1f ms, p99 %.1f ms, %d batches holding %d requests, "
+        "%d of %d predictions on a cached factor" % ("mixed" if mixed else "fp64", len(tickets), args.serve_rate,
+                                                    st["p50_ms"], st["p99_ms"], bs["batches"], bs["batched_requests"],
+                                                    bs["predict_hits"], bs["predicts"]))
+    return {"plan": ("mixed (%d, %d), dts %d" % (args.band_fp64, args.band_fp32, dts)) if mixed else "fp64, dts 320",
+            "p50_ms": st["p50_ms"], "p99_ms": st["p99_ms"], "completed_per_s": len(tickets) / wall,
+            "batches": bs["batches"], "batched_requests": bs["batched_requests"],
+            "predictions_on_cached_factor": [bs["predict_hits"], bs["predicts"]]}, out
+
+
+def serve_phase(egs, args):
+    """Serving at a fixed request rate under the FP64 and the mixed plan (see
+    serve_workload); latency is submission to completion.  The per-request
+    results of the two plans are compared (relative log-likelihood error and
+    largest prediction difference of mixed against FP64)."""
+    import numpy as np
+
+    reqs, regions, arrivals = serve_workload(egs, args)
+    fp64, r64 = serve_run(egs, args, reqs, regions, arrivals, egs.PRECISION_FP64)
+    mixed, rmx = serve_run(egs, args, reqs, regions, arrivals, egs.PRECISION_MIXED)
+    llh_err = max(abs(a - b) / abs(b) for (k, *_), a, b in zip(reqs, rmx, r64) if k == "loglik")
+    pred_err = max((float(np.max(np.abs(a - b))) for (k, *_), a, b in zip(reqs, rmx, r64) if k == "predict"),
+                   default=0.0)
+    sizes = [r[0] for r in regions]
+    return {"requests": len(reqs), "rate_per_s": args.serve_rate, "regions": len(regions),
+            "n_range": [min(sizes), max(sizes)], "workers": args.serve_workers,
+            "hbm_budget_gb": args.serve_budget_gb,
+            "mix": "80% likelihood of a new region each (n log-uniform in [500, 20000], theta uniform, z simulated "
+                   "at theta), 20% kriging of 100 new points against one of the 16 latest regions at its theta",
+            "p50_ms": fp64["p50_ms"], "p99_ms": fp64["p99_ms"], "fp64": fp64, "mixed": mixed,
+            "mixed_vs_fp64": {"max_rel_llh_err": llh_err, "max_abs_prediction_diff": pred_err}}
 
 
 def bivariate_gcd(egs, args):
@@ -315,13 +367,56 @@ def bivariate_gcd(egs, args):
     print(json.dumps(out))
 
 
+def c2_workflow(egs, args):
+    """The smooth-field workflow: n = 40,000 locations on a jittered 200 x 200
+    grid (srand(0)) and 1,000 uniform held-out points with observations
+    generated jointly (z = L(theta) e over all 41,000 points, theta = (1, 0.1,
+    1.0), FP64 on the GPU); maximum likelihood on the 40,000 observations with
+    the FP64 plan, dts 960, from the lower bounds (0.01, 0.01, 0.01) within
+    [0.01, 5]^3 and at most --c2-iterations evaluations; then kriging of the
+    held-out points at the estimate, with the MSPE against their generated
+    values (ExaGeoStatDataModeling then ExaGeoStatPrediction)."""
+    import numpy as np
+
+    n, held, theta = 40000, 1000, [1.0, 0.1, 1.0]
+    x, y = egs.generate_locations(n, seed=0)
+    rng = np.random.default_rng(n + held)
+    xm, ym = rng.random(held), rng.random(held)
+    t0 = time.perf_counter()
+    with egs.Context(0) as ctx:
+        pj = egs.Problem(ctx, "UnivariateMaternStationary", np.concatenate([x, xm]), np.concatenate([y, ym]),
+                         dts=args.dts)
+        z = pj.generate_observations(theta, seed=0)
+        pj.close()
+        zo, zh = z[:n], z[n:]
+        t1 = time.perf_counter()
+        p = egs.Problem(ctx, "UnivariateMaternStationary", x, y, dts=args.dts)
+        p.set_observations(zo)
+        th, llh, nev = p.data_modeling([0.01] * 3, [5.0] * 3, tolerance=args.mle_ftol,
+                                       max_iterations=args.c2_iterations)
+        t2 = time.perf_counter()
+        zm, m3 = p.predict_mspe(list(th), xm, ym, z_actual=zh)
+        t3 = time.perf_counter()
+        p.close()
+    log("c2: theta-hat %s llh %.10f after %d evaluations in %.1f s; MSPE %.6g over %d held-out points"
+        % (list(th), llh, nev, t2 - t1, m3[0], held))
+    out = {"workload": "smooth-field workflow: n=%d jittered 200 x 200 grid + %d uniform held-out points generated "
+                       "jointly, theta_true=%s, FP64, dts=%d, MLE from the lower bounds in [0.01, 5]^3 with at most %d "
+                       "evaluations (ftol %g), then kriging of the held-out points" % (
+                           n, held, theta, args.dts, args.c2_iterations, args.mle_ftol),
+           "theta_hat": list(th), "loglik": llh, "evaluations": nev, "mle_seconds": t2 - t1,
+           "evals_per_s": nev / (t2 - t1), "data_seconds": t1 - t0, "mspe": m3[0], "prediction_seconds": t3 - t2,
+           "held_out_variance": float(np.var(zh))}
+    print(json.dumps(out))
+
+
 def mle_compare(egs, args):
     """Maximum-likelihood fits of one data set with the FP64 and the mixed plan.
     Both start at the generating theta inside the box [theta/2, 2 theta] (range
     [0.4, 4] x theta) so that both converge within the evaluation budget; a point
     where the mixed factor is not positive definite is evaluated with the FP64
     plan instead (counted in the output)."""
-    n = args.n if args.n != 100000 else 40000
+    n = args.mle_n
     theta = args.theta or [1.0, 0.05, 0.5]
     lb = [0.5 * theta[0], 0.4 * theta[1], 0.5 * theta[2]]
     ub = [2.0 * theta[0], 4.0 * theta[1], 2.0 * theta[2]]
@@ -405,13 +500,19 @@ def main():
                          "stop on the radius alone")
     ap.add_argument("--bivariate", action="store_true",
                     help="the bivariate great-circle workload (25,000 locations on the sphere) instead of the default")
+    ap.add_argument("--c2", action="store_true",
+                    help="the smooth-field workflow: MLE at n=40000 (nu = 1) then kriging of 1000 jointly generated "
+                         "held-out points")
+    ap.add_argument("--c2-iterations", type=int, default=100)
+    ap.add_argument("--mle-n", type=int, default=40000, help="n of the --mle fits")
     ap.add_argument("--theta", type=float, nargs=3, default=None)
     ap.add_argument("--maxeval", type=int, default=500)
     ap.add_argument("--no-serve", action="store_true", help="skip the serving phase")
-    ap.add_argument("--serve-requests", type=int, default=400)
-    ap.add_argument("--serve-rate", type=float, default=20.0, help="request arrivals per second (Poisson)")
-    ap.add_argument("--serve-regions", type=int, default=16)
+    ap.add_argument("--serve-requests", type=int, default=2000)
+    ap.add_argument("--serve-rate", type=float, default=50.0, help="request arrivals per second (Poisson)")
     ap.add_argument("--serve-workers", type=int, default=4)
+    ap.add_argument("--serve-budget-gb", type=int, default=160, help="HBM budget of the server")
+    ap.add_argument("--serve-only", action="store_true", help="only the serving phase")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a 1 GiB device-to-device copy (calibrates FETCH_SIZE/WRITE_SIZE counters)")
     args = ap.parse_args()
@@ -432,6 +533,11 @@ def main():
 
     import exageostatcpp_amd as egs
 
+    if args.serve_only:
+        print(json.dumps(serve_phase(egs, args)))
+        return
+    if args.c2:
+        return c2_workflow(egs, args)
     if args.mle:
         return mle_compare(egs, args)
     if args.bivariate:
@@ -447,6 +553,15 @@ def main():
 
     fp64_steps = args.fp64_steps if args.fp64_steps is not None else args.steps
     fp64 = timed(egs, ctx, prob, theta, fp64_steps, args.warmup, "fp64")
+    # kriging of 1000 uniform points with each plan's factor at theta (reused
+    # from the last evaluation): the prediction error of mixed against FP64
+    import numpy as np
+
+    krng = np.random.default_rng(1000)
+    kx, ky = krng.random(1000), krng.random(1000)
+    t_k = time.perf_counter()
+    zk64 = prob.predict(theta, kx, ky)[0]
+    t_k64 = time.perf_counter() - t_k
     mode = egs.PRECISION_MIXED_BF16 if args.plain_bf16 else egs.PRECISION_MIXED
     prob.set_precision(mode, args.band_fp64, args.band_fp32)
     prob.set_single_product(args.single_band, args.single_tau)
@@ -454,6 +569,14 @@ def main():
         prob.set_fp64_product_band(args.fp64_product_band)
     mixed = timed(egs, ctx, prob, theta, args.steps, args.warmup, "mixed")
     rel_err = abs(mixed["loglik"] - fp64["loglik"]) / abs(fp64["loglik"])
+    t_k = time.perf_counter()
+    zkmx = prob.predict(theta, kx, ky)[0]
+    t_kmx = time.perf_counter() - t_k
+    kriging = {"points": 1000, "max_abs_diff_vs_fp64": float(np.max(np.abs(zkmx - zk64))),
+               "rel_l2_diff_vs_fp64": float(np.linalg.norm(zkmx - zk64) / np.linalg.norm(zk64)),
+               "fp64_ms": 1000.0 * t_k64, "mixed_ms": 1000.0 * t_kmx}
+    log("kriging of 1000 points, mixed vs FP64: max abs diff %.3g, rel L2 %.3g"
+        % (kriging["max_abs_diff_vs_fp64"], kriging["rel_l2_diff_vs_fp64"]))
 
     # dominant kernel of the mixed run
     peaks = {"update_fp64": ("fp64", 1.0), "update_fp32": ("fp32", 1.0),
@@ -529,6 +652,7 @@ def main():
             "logdet_err_vs_fp64": mixed["logdet"] - fp64["logdet"],
             "quad_rel_err_vs_fp64": (mixed["quad"] - fp64["quad"]) / fp64["quad"],
             "refinement": mixed["refinement"],
+            "kriging_at_theta": kriging,
             "block_updates_beyond_single_band": mixed["product_counts"],
             "speedup_vs_fp64": mixed["evals_per_s"] / fp64["evals_per_s"],
             "phases_ms_per_step": mixed["phases_ms_per_step"],

@@ -20,7 +20,7 @@ API_LIB = os.path.join(HERE, "libexageostat_cpp.so")
 CXX = os.environ.get("CXX", "g++")
 
 SOURCES = ["egs_covgen.hip", "egs_chol.hip", "egs_update.hip", "egs_mixed.hip", "egs_solve.hip", "egs_capi.cpp",
-           "egs_bobyqa.cpp", "egs_serve.cpp"]
+           "egs_bobyqa.cpp", "egs_serve.cpp", "egs_devmem.cpp"]
 ARCH = os.environ.get("EGS_OFFLOAD_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 

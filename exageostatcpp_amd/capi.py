@@ -68,7 +68,10 @@ EXPORTS = {
                              _int], _int),
     "egs_problem_create": ([_vp, _int, _i64, _dp, _dp, _dp, _int, _int, ctypes.POINTER(_vp)], _int),
     "egs_problem_destroy": ([_vp], _int),
+    "egs_problem_set_context": ([_vp, _vp], _int),
     "egs_problem_device_bytes": ([_vp, ctypes.POINTER(_i64)], _int),
+    "egs_problem_estimate_bytes": ([_int, _i64, _int, _int, _int, _int, _int, ctypes.POINTER(_i64)], _int),
+    "egs_memory_cache": ([_i64, ctypes.POINTER(_i64), ctypes.POINTER(_i64)], _int),
     "egs_problem_set_precision": ([_vp, _int, _int, _int], _int),
     "egs_problem_set_single_product": ([_vp, _int, ctypes.c_double], _int),
     "egs_problem_set_fp64_product_band": ([_vp, _int], _int),
@@ -76,6 +79,7 @@ EXPORTS = {
     "egs_problem_set_observations": ([_vp, _dp, _i64], _int),
     "egs_problem_get_observations": ([_vp, _dp, _i64], _int),
     "egs_loglik": ([_vp, _dp, _dp, _dp, _dp], _int),
+    "egs_loglik_batch": ([ctypes.POINTER(_vp), _dp, _int, _dp, _dp, _dp, ctypes.POINTER(_int)], _int),
     "egs_covariance": ([_vp, _dp], _int),
     "egs_potrf": ([_vp], _int),
     "egs_solve_logdet_quad": ([_vp, _dp, _dp], _int),
@@ -83,6 +87,8 @@ EXPORTS = {
     "egs_trsm_vector": ([_vp, _int, ctypes.c_double, _dp, _i64], _int),
     "egs_last_timings": ([_vp, _dp, _dp, _dp], _int),
     "egs_last_refinement": ([_vp, ctypes.POINTER(_int), _dp], _int),
+    "egs_problem_set_fallback": ([_vp, _int], _int),
+    "egs_last_fallback": ([_vp, ctypes.POINTER(_int), ctypes.POINTER(_i64)], _int),
     "egs_last_product_counts": ([_vp, ctypes.POINTER(_i64), ctypes.POINTER(_i64)], _int),
     "egs_problem_profile": ([_vp, _int], _int),
     "egs_problem_kernel_stats": ([_vp, _int, ctypes.POINTER(_i64), _dp, _dp], _int),
@@ -96,11 +102,15 @@ EXPORTS = {
     "egs_server_create": ([_int, _i64, _int, ctypes.POINTER(_vp)], _int),
     "egs_server_destroy": ([_vp], _int),
     "egs_server_add_region": ([_vp, _int, _i64, _dp, _dp, _dp, _dp, _int, _int, ctypes.POINTER(_int)], _int),
+    "egs_server_add_region_plan": ([_vp, _int, _i64, _dp, _dp, _dp, _dp, _int, _int, _int, _int, _int,
+                                    ctypes.POINTER(_int)], _int),
     "egs_server_submit_loglik": ([_vp, _int, _dp, ctypes.POINTER(_i64)], _int),
     "egs_server_submit_predict": ([_vp, _int, _dp, _i64, _dp, _dp, ctypes.POINTER(_i64)], _int),
     "egs_server_wait": ([_vp, _i64, _dp, _dp, _i64, _dp], _int),
     "egs_server_stats": ([_vp, ctypes.POINTER(_i64), _dp, _dp, _dp, ctypes.POINTER(_i64)], _int),
     "egs_server_reset_stats": ([_vp], _int),
+    "egs_server_batch_stats": ([_vp, ctypes.POINTER(_i64), ctypes.POINTER(_i64), ctypes.POINTER(_i64),
+                                ctypes.POINTER(_i64)], _int),
 }
 
 OBJECTIVE_FN = ctypes.CFUNCTYPE(ctypes.c_double, _dp, _vp)
@@ -308,6 +318,17 @@ class Problem:
         _check(lib().egs_last_refinement(self.handle, ctypes.byref(it), ctypes.byref(res)))
         return {"iterations": it.value, "residual": res.value}
 
+    def set_fallback(self, max_iterations):
+        """Mixed plans: repeat an evaluation under FP64 when the refinement needs
+        more than max_iterations iterations or the mixed factor is not SPD (0: never)."""
+        _check(lib().egs_problem_set_fallback(self.handle, int(max_iterations)))
+
+    def fallback(self):
+        """(the last evaluation fell back to FP64, evaluations that did)."""
+        f, t = _int(), _i64()
+        _check(lib().egs_last_fallback(self.handle, ctypes.byref(f), ctypes.byref(t)))
+        return bool(f.value), t.value
+
     def set_graph(self, enable=True):
         """Replay FP64-plan evaluations from a captured hipGraph."""
         _check(lib().egs_problem_set_graph(self.handle, 1 if enable else 0))
@@ -427,14 +448,15 @@ class Server:
             except Exception:
                 pass
 
-    def add_region(self, kernel, x, y, obs, z=None, metric=0, dts=320):
+    def add_region(self, kernel, x, y, obs, z=None, metric=0, dts=320, mode=PRECISION_FP64, band_fp64=4,
+                   band_fp32=4):
         x, px = _arr(x)
         y, py = _arr(y)
         z, pz = _arr(z)
         obs, po = _arr(obs)
         rid = _int()
-        _check(lib().egs_server_add_region(self.handle, _kid(kernel), len(x), px, py, pz, po, metric, dts,
-                                           ctypes.byref(rid)))
+        _check(lib().egs_server_add_region_plan(self.handle, _kid(kernel), len(x), px, py, pz, po, metric, dts, mode,
+                                                band_fp64, band_fp32, ctypes.byref(rid)))
         self._p[rid.value] = 2 if _kid(kernel) == KERNEL_BIVARIATE else 1
         return rid.value
 
@@ -470,6 +492,47 @@ class Server:
 
     def reset_stats(self):
         _check(lib().egs_server_reset_stats(self.handle))
+
+    def batch_stats(self):
+        b, br, p, ph = _i64(), _i64(), _i64(), _i64()
+        _check(lib().egs_server_batch_stats(self.handle, ctypes.byref(b), ctypes.byref(br), ctypes.byref(p),
+                                            ctypes.byref(ph)))
+        return {"batches": b.value, "batched_requests": br.value, "predicts": p.value, "predict_hits": ph.value}
+
+
+def loglik_batch(problems, thetas):
+    """egs_loglik_batch: [(loglik, logdet, quad) or the exception of that problem] per problem."""
+    n = len(problems)
+    hs = (_vp * n)(*[p.handle for p in problems])
+    th = np.zeros(6 * n)
+    for b, t in enumerate(thetas):
+        th[6 * b:6 * b + len(t)] = t
+    llh, ld, q = np.zeros(n), np.zeros(n), np.zeros(n)
+    st = (_int * n)()
+    _check(lib().egs_loglik_batch(hs, th.ctypes.data_as(_dp), n, llh.ctypes.data_as(_dp), ld.ctypes.data_as(_dp),
+                                  q.ctypes.data_as(_dp), st))
+    out = []
+    for b in range(n):
+        if st[b] == SUCCESS:
+            out.append((llh[b], ld[b], q[b]))
+        else:
+            out.append((NotPositiveDefinite if st[b] == ERR_NOT_SPD else EGSError)(st[b], "problem %d failed" % b))
+    return out
+
+
+def estimate_bytes(kernel, n_loc, dts, mode=PRECISION_FP64, band_fp64=4, band_fp32=4, dim=2):
+    """Device bytes a problem of this shape and plan holds once evaluated."""
+    b = _i64()
+    _check(lib().egs_problem_estimate_bytes(_kid(kernel), int(n_loc), dim, dts, mode, band_fp64, band_fp32,
+                                            ctypes.byref(b)))
+    return b.value
+
+
+def memory_cache(limit_bytes=-1):
+    """Sets the device block cache limit (negative: unchanged); returns (cached, live) bytes."""
+    c, l = _i64(), _i64()
+    _check(lib().egs_memory_cache(int(limit_bytes), ctypes.byref(c), ctypes.byref(l)))
+    return c.value, l.value
 
 
 def srand(seed):

@@ -10,6 +10,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdarg>
+#include <mutex>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -19,6 +20,7 @@
 #include "../../include/exageostat_amd.h"
 #include "egs_bobyqa.hpp"
 #include "egs_common.hpp"
+#include "egs_devmem.hpp"
 #include "egs_kernels.hpp"
 #include "egs_matern.hpp"
 
@@ -58,6 +60,9 @@ struct egs_context_s {
     hipStream_t stream = nullptr;     // critical path: next panel, solves, copies
     hipStream_t bulk = nullptr;       // trailing updates that the next panel does not wait for
     std::vector<egs_problem_s *> problems;   // live problems, released by egs_finalize
+    std::mutex mu;                           // guards problems (problems move between contexts)
+    egs::BatchJob *d_jobs = nullptr;         // job records of egs_loglik_batch, jobs_cap entries
+    int64_t jobs_cap = 0, dev_bytes = 0;
 };
 
 // Per-kernel timing with HIP events on the library stream (enabled by
@@ -126,8 +131,15 @@ struct egs_problem_s {
     double *d_dot = nullptr;                 // dot-product block partials
     double *d_pred = nullptr;                // prediction targets and outputs, pred_cap doubles
     int64_t pred_cap = 0;
+    int64_t dev_bytes = 0;                   // device bytes held (size classes of egs_devmem)
     int refine_iters = 0;
     double refine_resid = 0.0;
+    // mixed plans: an evaluation whose refinement needs more than
+    // fallback_iters iterations (0: never), or whose mixed factor is not
+    // positive definite, is repeated under the FP64 plan
+    int fallback_iters = 4;
+    bool fell_back = false;
+    int64_t fallbacks = 0;
 
     double *d_x = nullptr, *d_y = nullptr, *d_z = nullptr;
     char *d_arena = nullptr;
@@ -176,6 +188,21 @@ struct egs_problem_s {
 };
 
 namespace {
+
+// Device buffers of a problem come from the block cache of egs_devmem and are
+// counted in p->dev_bytes.
+template <typename T>
+hipError_t pmalloc(egs_problem_s *p, T **ptr, size_t bytes) {
+    void *v = nullptr;
+    const hipError_t e = egs::dev_alloc(&v, bytes, &p->dev_bytes);
+    *ptr = static_cast<T *>(v);
+    return e;
+}
+
+template <typename T>
+void pfree(egs_problem_s *p, T *ptr) {
+    egs::dev_free((void *) ptr, &p->dev_bytes);
+}
 
 int check_problem(egs_problem_t p) {
     if (!p) return fail(EGS_ERR_INVALID_ARG, "null problem");
@@ -236,15 +263,15 @@ int make_params(int kernel, int metric, const double *th, egs::MaternParams &prm
 }
 
 void free_sigma_copy(egs_problem_s *p) {
-    hipFree(p->d_sig64);
-    hipFree(p->d_prow);
-    hipFree(p->d_pcol);
+    pfree(p, p->d_sig64);
+    pfree(p, p->d_prow);
+    pfree(p, p->d_pcol);
     p->d_sig64 = p->d_prow = p->d_pcol = nullptr;
 }
 
 void free_tile_inverses(egs_problem_s *p) {
-    hipFree(p->d_tinv);
-    hipFree(p->d_ybuf);
+    pfree(p, p->d_tinv);
+    pfree(p, p->d_ybuf);
     p->d_tinv = p->d_ybuf = nullptr;
 }
 
@@ -259,52 +286,53 @@ void invalidate_graph(egs_problem_s *p) {
 
 void free_problem_buffers(egs_problem_s *p) {
     if (p->ctx) {
+        std::lock_guard<std::mutex> lk(p->ctx->mu);
         auto &v = p->ctx->problems;
         v.erase(std::remove(v.begin(), v.end(), p), v.end());
     }
     invalidate_graph(p);
-    hipFree(p->d_prm);
+    pfree(p, p->d_prm);
     p->d_prm = nullptr;
-    hipFree(p->d_x);
-    hipFree(p->d_y);
-    hipFree(p->d_z);
-    hipFree(p->d_arena);
-    hipFree(p->d_off);
-    hipFree(p->d_store);
-    hipFree(p->d_pairs);
-    hipFree(p->d_winv);
-    hipFree(p->d_dscr);
-    hipFree(p->d_obs);
-    hipFree(p->d_work);
-    hipFree(p->d_aux);
-    hipFree(p->d_partials);
-    hipFree(p->d_info);
+    pfree(p, p->d_x);
+    pfree(p, p->d_y);
+    pfree(p, p->d_z);
+    pfree(p, p->d_arena);
+    pfree(p, p->d_off);
+    pfree(p, p->d_store);
+    pfree(p, p->d_pairs);
+    pfree(p, p->d_winv);
+    pfree(p, p->d_dscr);
+    pfree(p, p->d_obs);
+    pfree(p, p->d_work);
+    pfree(p, p->d_aux);
+    pfree(p, p->d_partials);
+    pfree(p, p->d_info);
     for (auto &q : p->d_pairs_cls) {
-        hipFree(q);
+        pfree(p, q);
         q = nullptr;
     }
     for (int b = 0; b < kPanelSlots; ++b) {
-        hipFree(p->d_p64[b]);
-        hipFree(p->d_p32[b]);
-        hipFree(p->d_p16h[b]);
-        hipFree(p->d_p16l[b]);
-        hipFree(p->d_pinv[b]);
-        hipFree(p->d_bmax[b]);
+        pfree(p, p->d_p64[b]);
+        pfree(p, p->d_p32[b]);
+        pfree(p, p->d_p16h[b]);
+        pfree(p, p->d_p16l[b]);
+        pfree(p, p->d_pinv[b]);
+        pfree(p, p->d_bmax[b]);
         p->d_pinv[b] = nullptr;
         p->d_bmax[b] = nullptr;
         p->d_p64[b] = nullptr;
         p->d_p32[b] = nullptr;
         p->d_p16h[b] = p->d_p16l[b] = nullptr;
     }
-    hipFree(p->d_pmax);
-    hipFree(p->d_counts);
+    pfree(p, p->d_pmax);
+    pfree(p, p->d_counts);
     p->d_pmax = nullptr;
     p->d_counts = nullptr;
-    hipFree(p->d_cg);
-    hipFree(p->d_mv);
-    hipFree(p->d_dot);
+    pfree(p, p->d_cg);
+    pfree(p, p->d_mv);
+    pfree(p, p->d_dot);
     p->d_cg = p->d_mv = p->d_dot = nullptr;
-    hipFree(p->d_pred);
+    pfree(p, p->d_pred);
     p->d_pred = nullptr;
     p->pred_cap = 0;
     free_sigma_copy(p);
@@ -405,10 +433,10 @@ int setup_mixed(egs_problem_s *p) {
         for (int c = 0; c < kClasses; ++c) p->cls_count[c][j] = (int) p->h_pairs_cls[c].size();
     }
     for (int c = 0; c < kClasses; ++c) {
-        hipFree(p->d_pairs_cls[c]);
+        pfree(p, p->d_pairs_cls[c]);
         p->d_pairs_cls[c] = nullptr;
         const size_t bytes = sizeof(int2) * std::max<size_t>(1, p->h_pairs_cls[c].size());
-        if (hipMalloc(&p->d_pairs_cls[c], bytes) != hipSuccess)
+        if (pmalloc(p, &p->d_pairs_cls[c], bytes) != hipSuccess)
             return fail(EGS_ERR_OUT_OF_MEMORY, "cannot allocate the update lists");
         if (!p->h_pairs_cls[c].empty())
             EGS_HIP(hipMemcpy(p->d_pairs_cls[c], p->h_pairs_cls[c].data(), sizeof(int2) * p->h_pairs_cls[c].size(),
@@ -418,24 +446,24 @@ int setup_mixed(egs_problem_s *p) {
     bool slots_ok = true;
     const bool need_slots = !p->d_p64[0];
     for (int b = 0; b < kPanelSlots && need_slots; ++b)
-        slots_ok = slots_ok && hipMalloc(&p->d_p64[b], panel * sizeof(double)) == hipSuccess &&
-                   hipMalloc(&p->d_p32[b], panel * sizeof(float)) == hipSuccess &&
-                   hipMalloc(&p->d_p16h[b], panel * sizeof(uint16_t)) == hipSuccess &&
-                   hipMalloc(&p->d_p16l[b], panel * sizeof(uint16_t)) == hipSuccess &&
-                   hipMalloc(&p->d_pinv[b], sizeof(float) * (size_t) p->npad) == hipSuccess &&
-                   hipMalloc(&p->d_bmax[b], sizeof(float) * (size_t) (p->npad / 192)) == hipSuccess;
+        slots_ok = slots_ok && pmalloc(p, &p->d_p64[b], panel * sizeof(double)) == hipSuccess &&
+                   pmalloc(p, &p->d_p32[b], panel * sizeof(float)) == hipSuccess &&
+                   pmalloc(p, &p->d_p16h[b], panel * sizeof(uint16_t)) == hipSuccess &&
+                   pmalloc(p, &p->d_p16l[b], panel * sizeof(uint16_t)) == hipSuccess &&
+                   pmalloc(p, &p->d_pinv[b], sizeof(float) * (size_t) p->npad) == hipSuccess &&
+                   pmalloc(p, &p->d_bmax[b], sizeof(float) * (size_t) (p->npad / 192)) == hipSuccess;
     if (!slots_ok) return fail(EGS_ERR_OUT_OF_MEMORY, "cannot allocate the mixed-precision panel copies");
     if (!p->d_cg) {
-        if (hipMalloc(&p->d_pmax, sizeof(unsigned) * (size_t) p->nt) != hipSuccess ||
-            hipMalloc(&p->d_counts, 2 * sizeof(unsigned long long)) != hipSuccess ||
-            hipMalloc(&p->d_cg, 5 * sizeof(double) * (size_t) p->npad) != hipSuccess ||
-            hipMalloc(&p->d_mv, sizeof(double) * (size_t) egs::matvec_chunks(p->n) * p->npad) != hipSuccess ||
-            hipMalloc(&p->d_dot, sizeof(double) * egs::dot_blocks()) != hipSuccess)
+        if (pmalloc(p, &p->d_pmax, sizeof(unsigned) * (size_t) p->nt) != hipSuccess ||
+            pmalloc(p, &p->d_counts, 2 * sizeof(unsigned long long)) != hipSuccess ||
+            pmalloc(p, &p->d_cg, 5 * sizeof(double) * (size_t) p->npad) != hipSuccess ||
+            pmalloc(p, &p->d_mv, sizeof(double) * (size_t) egs::matvec_chunks(p->n) * p->npad) != hipSuccess ||
+            pmalloc(p, &p->d_dot, sizeof(double) * egs::dot_blocks()) != hipSuccess)
             return fail(EGS_ERR_OUT_OF_MEMORY, "cannot allocate the mixed-precision work buffers");
     }
     if (!p->d_tinv) {
-        if (hipMalloc(&p->d_tinv, sizeof(double) * (size_t) p->nt * p->ts * p->ts) != hipSuccess ||
-            hipMalloc(&p->d_ybuf, sizeof(double) * (size_t) p->npad) != hipSuccess) {
+        if (pmalloc(p, &p->d_tinv, sizeof(double) * (size_t) p->nt * p->ts * p->ts) != hipSuccess ||
+            pmalloc(p, &p->d_ybuf, sizeof(double) * (size_t) p->npad) != hipSuccess) {
             free_tile_inverses(p);
             return fail(EGS_ERR_OUT_OF_MEMORY, "cannot allocate the diagonal-tile inverses");
         }
@@ -444,9 +472,9 @@ int setup_mixed(egs_problem_s *p) {
         // Optional: without it Sigma v regenerates the covariance on the fly.
         const size_t ntiles = (size_t) p->nt * (p->nt + 1) / 2;
         const size_t tile = (size_t) p->ts * p->ts;
-        if (hipMalloc(&p->d_sig64, ntiles * tile * sizeof(double)) != hipSuccess ||
-            hipMalloc(&p->d_prow, ntiles * p->ts * sizeof(double)) != hipSuccess ||
-            hipMalloc(&p->d_pcol, ntiles * p->ts * sizeof(double)) != hipSuccess) {
+        if (pmalloc(p, &p->d_sig64, ntiles * tile * sizeof(double)) != hipSuccess ||
+            pmalloc(p, &p->d_prow, ntiles * p->ts * sizeof(double)) != hipSuccess ||
+            pmalloc(p, &p->d_pcol, ntiles * p->ts * sizeof(double)) != hipSuccess) {
             (void) hipGetLastError();
             free_sigma_copy(p);
         }
@@ -471,11 +499,11 @@ int layout_tiles(egs_problem_s *p) {
         }
     }
     if (p->d_arena && p->arena_bytes != off) {
-        hipFree(p->d_arena);
+        pfree(p, p->d_arena);
         p->d_arena = nullptr;
     }
     if (!p->d_arena) {
-        if (hipMalloc(&p->d_arena, off) != hipSuccess) {
+        if (pmalloc(p, &p->d_arena, off) != hipSuccess) {
             p->d_arena = nullptr;
             return fail(EGS_ERR_OUT_OF_MEMORY, "cannot allocate %zu bytes for the covariance tiles", off);
         }
@@ -928,6 +956,55 @@ int covariance_tile_on(egs_context_t ctx, const TileRequest &t, double *A, const
     return EGS_SUCCESS;
 }
 
+// Shape of a problem: tile size (dts rounded up to a multiple of 64), tile
+// count and padded dimension.
+void init_shape(egs_problem_s *p, int kernel, int64_t n_loc, int dim, int dts) {
+    p->kernel = kernel;
+    p->band_prod64 = kernel == EGS_KERNEL_BIVARIATE_MATERN_PARSIMONIOUS ? 2 : 0;
+    p->p = kernel == EGS_KERNEL_BIVARIATE_MATERN_PARSIMONIOUS ? 2 : 1;
+    p->dim = dim;
+    p->dts = dts;
+    p->ts = std::min(((dts + egs::kBlk - 1) / egs::kBlk) * egs::kBlk, kMaxTileSize);
+    p->nsb = p->ts / egs::kBlk;
+    p->n_loc = n_loc;
+    p->n = n_loc * p->p;
+    p->nt = (int) ((p->n + p->ts - 1) / p->ts);
+    p->npad = (int64_t) p->nt * p->ts;
+}
+
+// Device bytes a problem of q's shape and plan holds after an evaluation: the
+// buffers of egs_problem_create, the tile arena, and under a mixed plan the
+// buffers of setup_mixed (prediction and graph buffers, made on demand, are
+// not included).  Each allocation counts with its size class.
+int64_t footprint(const egs_problem_s &q) {
+    using egs::dev_class_bytes;
+    const int64_t nt = q.nt, ts = q.ts, npad = q.npad, ntiles = nt * (nt + 1) / 2;
+    const int64_t nb = (int64_t) sizeof(double) * q.n_loc, vb = (int64_t) sizeof(double) * npad;
+    int64_t b = dev_class_bytes(nb) * (q.dim == 3 ? 3 : 2) + dev_class_bytes(8 * nt * nt) + dev_class_bytes(nt * nt) +
+                dev_class_bytes(8 * ntiles) + dev_class_bytes(8 * nt * q.nsb * egs::kBlk * egs::kBlk) +
+                dev_class_bytes(8 * ts * ts) + 3 * dev_class_bytes(vb) + dev_class_bytes(16 * nt) + dev_class_bytes(4);
+    int64_t arena = 0;
+    for (int j = 0; j < q.nt; ++j)
+        for (int i = j; i < q.nt; ++i) {
+            arena += ts * ts * (tile_store(&q, i, j) == egs::kStoreF64 ? 8 : 4);
+            arena = (arena + 255) & ~int64_t(255);
+        }
+    b += dev_class_bytes(arena);
+    if (q.mode == EGS_PRECISION_FP64) return b;
+    int64_t count[kClasses] = {};
+    for (int j = 0; j < q.nt; ++j)
+        for (int i = j; i < q.nt; ++i) ++count[tile_class(&q, i, j)];
+    for (int c = 0; c < kClasses; ++c) b += dev_class_bytes(8 * std::max<int64_t>(1, count[c]));
+    const int64_t panel = npad * ts;
+    b += kPanelSlots * (dev_class_bytes(8 * panel) + dev_class_bytes(4 * panel) + 2 * dev_class_bytes(2 * panel) +
+                        dev_class_bytes(4 * npad) + dev_class_bytes(4 * (npad / 192)));
+    b += dev_class_bytes(4 * nt) + dev_class_bytes(16) + dev_class_bytes(5 * vb) +
+         dev_class_bytes(8 * (int64_t) egs::matvec_chunks(q.n) * npad) + dev_class_bytes(8 * egs::dot_blocks());
+    b += dev_class_bytes(8 * nt * ts * ts) + dev_class_bytes(vb);
+    b += dev_class_bytes(8 * ntiles * ts * ts) + 2 * dev_class_bytes(8 * ntiles * ts);
+    return b;
+}
+
 uint64_t morton_spread(uint64_t v) {
     uint64_t out = 0;
     for (int b = 0; b < 16; ++b) out |= ((v >> b) & 1ull) << (4 * b);
@@ -973,11 +1050,17 @@ int egs_finalize(egs_context_t ctx) {
     hipStreamSynchronize(ctx->stream);
     hipStreamSynchronize(ctx->bulk);
     // problems still alive lose their device memory; destroying them later is safe
-    while (!ctx->problems.empty()) {
-        egs_problem_s *p = ctx->problems.back();
+    for (;;) {
+        egs_problem_s *p = nullptr;
+        {
+            std::lock_guard<std::mutex> lk(ctx->mu);
+            if (ctx->problems.empty()) break;
+            p = ctx->problems.back();
+        }
         free_problem_buffers(p);
         p->ctx = nullptr;
     }
+    egs::dev_free(ctx->d_jobs, &ctx->dev_bytes);
     hipStreamDestroy(ctx->stream);
     hipStreamDestroy(ctx->bulk);
     delete ctx;
@@ -1022,18 +1105,8 @@ int egs_problem_create(egs_context_t ctx, int kernel, int64_t n_loc, const doubl
     EGS_HIP(hipSetDevice(ctx->device));
     auto *p = new egs_problem_s();
     p->ctx = ctx;
-    p->kernel = kernel;
-    p->band_prod64 = kernel == EGS_KERNEL_BIVARIATE_MATERN_PARSIMONIOUS ? 2 : 0;
-    p->p = kernel == EGS_KERNEL_BIVARIATE_MATERN_PARSIMONIOUS ? 2 : 1;
     p->metric = metric;
-    p->dim = (z && kernel == EGS_KERNEL_UNIVARIATE_MATERN_STATIONARY) ? 3 : 2;
-    p->dts = dts;
-    p->ts = std::min(((dts + egs::kBlk - 1) / egs::kBlk) * egs::kBlk, kMaxTileSize);
-    p->nsb = p->ts / egs::kBlk;
-    p->n_loc = n_loc;
-    p->n = n_loc * p->p;
-    p->nt = (int) ((p->n + p->ts - 1) / p->ts);
-    p->npad = (int64_t) p->nt * p->ts;
+    init_shape(p, kernel, n_loc, (z && kernel == EGS_KERNEL_UNIVARIATE_MATERN_STATIONARY) ? 3 : 2, dts);
     const int nt = p->nt;
     for (int j = nt - 1; j >= 0; --j)
         for (int i = j; i < nt; ++i) p->h_pairs.push_back(int2{i, j});
@@ -1045,17 +1118,17 @@ int egs_problem_create(egs_context_t ctx, int kernel, int64_t n_loc, const doubl
     };
     const size_t nb = sizeof(double) * (size_t) n_loc;
     const size_t vb = sizeof(double) * (size_t) p->npad;
-    bool ok = hipMalloc(&p->d_x, nb) == hipSuccess && hipMalloc(&p->d_y, nb) == hipSuccess &&
-              (p->dim != 3 || hipMalloc(&p->d_z, nb) == hipSuccess) &&
-              hipMalloc(&p->d_off, sizeof(int64_t) * (size_t) nt * nt) == hipSuccess &&
-              hipMalloc(&p->d_store, (size_t) nt * nt) == hipSuccess &&
-              hipMalloc(&p->d_pairs, sizeof(int2) * p->h_pairs.size()) == hipSuccess &&
-              hipMalloc(&p->d_winv, sizeof(double) * (size_t) nt * p->nsb * egs::kBlk * egs::kBlk) == hipSuccess &&
-              hipMalloc(&p->d_dscr, sizeof(double) * (size_t) p->ts * p->ts) == hipSuccess &&
-              hipMalloc(&p->d_obs, vb) == hipSuccess && hipMalloc(&p->d_work, vb) == hipSuccess &&
-              hipMalloc(&p->d_aux, vb) == hipSuccess &&
-              hipMalloc(&p->d_partials, sizeof(double) * 2 * (size_t) nt) == hipSuccess &&
-              hipMalloc(&p->d_info, sizeof(int)) == hipSuccess;
+    bool ok = pmalloc(p, &p->d_x, nb) == hipSuccess && pmalloc(p, &p->d_y, nb) == hipSuccess &&
+              (p->dim != 3 || pmalloc(p, &p->d_z, nb) == hipSuccess) &&
+              pmalloc(p, &p->d_off, sizeof(int64_t) * (size_t) nt * nt) == hipSuccess &&
+              pmalloc(p, &p->d_store, (size_t) nt * nt) == hipSuccess &&
+              pmalloc(p, &p->d_pairs, sizeof(int2) * p->h_pairs.size()) == hipSuccess &&
+              pmalloc(p, &p->d_winv, sizeof(double) * (size_t) nt * p->nsb * egs::kBlk * egs::kBlk) == hipSuccess &&
+              pmalloc(p, &p->d_dscr, sizeof(double) * (size_t) p->ts * p->ts) == hipSuccess &&
+              pmalloc(p, &p->d_obs, vb) == hipSuccess && pmalloc(p, &p->d_work, vb) == hipSuccess &&
+              pmalloc(p, &p->d_aux, vb) == hipSuccess &&
+              pmalloc(p, &p->d_partials, sizeof(double) * 2 * (size_t) nt) == hipSuccess &&
+              pmalloc(p, &p->d_info, sizeof(int)) == hipSuccess;
     if (!ok) return cleanup(fail(EGS_ERR_OUT_OF_MEMORY, "cannot allocate the problem's device buffers"));
     for (auto &e : p->ev)
         if (hipEventCreate(&e) != hipSuccess) return cleanup(fail(EGS_ERR_HIP, "hipEventCreate failed"));
@@ -1068,8 +1141,33 @@ int egs_problem_create(egs_context_t ctx, int kernel, int64_t n_loc, const doubl
         return cleanup(fail(EGS_ERR_HIP, "copying the locations to the device failed"));
     int st = layout_tiles(p);
     if (st) return cleanup(st);
-    ctx->problems.push_back(p);
+    {
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        ctx->problems.push_back(p);
+    }
     *problem = p;
+    return EGS_SUCCESS;
+}
+
+int egs_problem_set_context(egs_problem_t p, egs_context_t ctx) {
+    int st = check_problem(p);
+    if (st) return st;
+    if (!ctx) return fail(EGS_ERR_INVALID_ARG, "ctx is null");
+    if (ctx->device != p->ctx->device) return fail(EGS_ERR_INVALID_ARG, "the contexts are on different devices");
+    if (ctx == p->ctx) return EGS_SUCCESS;
+    // every entry point returns with the problem's work complete, so nothing of
+    // it is pending on the old context's streams
+    {
+        std::lock_guard<std::mutex> lk(p->ctx->mu);
+        auto &v = p->ctx->problems;
+        v.erase(std::remove(v.begin(), v.end(), p), v.end());
+    }
+    invalidate_graph(p);   // recorded on the old context's streams
+    {
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        ctx->problems.push_back(p);
+    }
+    p->ctx = ctx;
     return EGS_SUCCESS;
 }
 
@@ -1088,10 +1186,36 @@ int egs_problem_device_bytes(egs_problem_t p, int64_t *bytes) {
     int st = check_problem(p);
     if (st) return st;
     if (!bytes) return fail(EGS_ERR_INVALID_ARG, "bytes is null");
-    const int64_t vec = (int64_t) sizeof(double) * p->npad;
-    *bytes = (int64_t) p->arena_bytes + 3 * vec +
-             (int64_t) sizeof(double) * p->nt * p->nsb * egs::kBlk * egs::kBlk +
-             (int64_t) sizeof(double) * p->n_loc * (p->dim == 3 ? 3 : 2);
+    *bytes = p->dev_bytes;
+    return EGS_SUCCESS;
+}
+
+int egs_memory_cache(int64_t limit_bytes, int64_t *cached_bytes, int64_t *live_bytes) {
+    int64_t c = 0, l = 0;
+    egs::dev_cache(limit_bytes, &c, &l);
+    if (cached_bytes) *cached_bytes = c;
+    if (live_bytes) *live_bytes = l;
+    return EGS_SUCCESS;
+}
+
+int egs_problem_estimate_bytes(int kernel, int64_t n_loc, int dim, int dts, int mode, int band_fp64, int band_fp32,
+                               int64_t *bytes) {
+    if (!bytes) return fail(EGS_ERR_INVALID_ARG, "bytes is null");
+    if (kernel != EGS_KERNEL_UNIVARIATE_MATERN_STATIONARY && kernel != EGS_KERNEL_BIVARIATE_MATERN_PARSIMONIOUS)
+        return fail(EGS_ERR_INVALID_ARG, "unknown kernel %d", kernel);
+    if (n_loc <= 0 || dts <= 0 || (dim != 2 && dim != 3)) return fail(EGS_ERR_INVALID_ARG, "invalid problem shape");
+    if (mode != EGS_PRECISION_FP64 && mode != EGS_PRECISION_MIXED && mode != EGS_PRECISION_MIXED_BF16)
+        return fail(EGS_ERR_INVALID_ARG, "unknown precision mode %d", mode);
+    if (band_fp64 < 0 || band_fp32 < band_fp64) return fail(EGS_ERR_INVALID_ARG, "bands must satisfy 0 <= fp64 <= fp32");
+    egs_problem_s q;
+    init_shape(&q, kernel, n_loc, kernel == EGS_KERNEL_UNIVARIATE_MATERN_STATIONARY ? dim : 2, dts);
+    q.mode = mode;
+    q.band64 = band_fp64;
+    q.band32 = band_fp32;
+    if (mode != EGS_PRECISION_FP64 && !egs::update192_supported(q.ts))
+        return fail(EGS_ERR_UNSUPPORTED, "mixed precision needs a dense tile size that is a multiple of 192 (got %d)",
+                    q.ts);
+    *bytes = footprint(q);
     return EGS_SUCCESS;
 }
 
@@ -1253,7 +1377,7 @@ int enqueue_fp64_eval(egs_problem_s *p) {
 // reaches the graph through d_prm.
 int graph_capture(egs_problem_s *p) {
     hipStream_t s = p->ctx->stream;
-    if (!p->d_prm && hipMalloc(&p->d_prm, sizeof(egs::MaternParams)) != hipSuccess)
+    if (!p->d_prm && pmalloc(p, &p->d_prm, sizeof(egs::MaternParams)) != hipSuccess)
         return fail(EGS_ERR_OUT_OF_MEMORY, "cannot allocate the covariance parameters");
     EGS_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed));
     const int st = enqueue_fp64_eval(p);
@@ -1282,9 +1406,10 @@ int graph_launch(egs_problem_s *p, const double *theta) {
     return EGS_SUCCESS;
 }
 
-int egs_loglik(egs_problem_t p, const double *theta, double *loglik, double *logdet, double *quad) {
-    int st = check_problem(p);
-    if (st) return st;
+// One evaluation under the problem's current plan (egs_loglik without the
+// FP64 fallback of mixed plans).
+int loglik_once(egs_problem_s *p, const double *theta, double *loglik, double *logdet, double *quad) {
+    int st = EGS_SUCCESS;
     if (!loglik) return fail(EGS_ERR_INVALID_ARG, "loglik is null");
     if (!p->has_obs) return fail(EGS_ERR_STATE, "the problem has no observations");
     hipStream_t s = p->ctx->stream;
@@ -1366,6 +1491,161 @@ int egs_loglik(egs_problem_t p, const double *theta, double *loglik, double *log
     *loglik = -0.5 * q - 0.5 * ld - (double) (n / 2.0) * std::log(2.0 * kPi);
     if (logdet) *logdet = ld;
     if (quad) *quad = q;
+    return EGS_SUCCESS;
+}
+
+int egs_loglik_batch(egs_problem_t *problems, const double *thetas, int count, double *loglik, double *logdet,
+                     double *quad, int *status) {
+    if (!problems || !thetas || !loglik || !status || count <= 0)
+        return fail(EGS_ERR_INVALID_ARG, "egs_loglik_batch: invalid arguments");
+    egs_context_s *ctx = nullptr;
+    int ts = 0;
+    for (int b = 0; b < count; ++b) {
+        egs_problem_s *p = problems[b];
+        int st = check_problem(p);
+        if (st) return st;
+        if (!ctx) {
+            ctx = p->ctx;
+            ts = p->ts;
+        }
+        if (p->ctx != ctx) return fail(EGS_ERR_INVALID_ARG, "egs_loglik_batch: the problems must share one context");
+        if (p->ts != ts) return fail(EGS_ERR_INVALID_ARG, "egs_loglik_batch: the problems must share one tile size");
+        if (is_mixed(p)) return fail(EGS_ERR_UNSUPPORTED, "egs_loglik_batch: FP64 plans only");
+        if (!p->has_obs) return fail(EGS_ERR_STATE, "egs_loglik_batch: problem %d has no observations", b);
+        for (int c = 0; c < b; ++c)
+            if (problems[c] == p) return fail(EGS_ERR_INVALID_ARG, "egs_loglik_batch: problem %d appears twice", b);
+    }
+    hipStream_t s = ctx->stream;
+    // theta of each problem; a rejected theta fails that problem only
+    std::vector<egs::MaternParams> prm(count);
+    std::vector<int> run;
+    for (int b = 0; b < count; ++b) {
+        status[b] = make_params(problems[b]->kernel, problems[b]->metric, thetas + 6 * b, prm[b]);
+        if (status[b] == EGS_SUCCESS) run.push_back(b);
+    }
+    // problems by descending tile count: at step k the active ones are a prefix
+    std::stable_sort(run.begin(), run.end(), [&](int a, int c) { return problems[a]->nt > problems[c]->nt; });
+    const int na = (int) run.size();
+    const int max_nt = na ? problems[run[0]]->nt : 0;
+    std::vector<int> active(max_nt + 1, 0);   // active[k]: problems with nt > k
+    for (int k = 0; k <= max_nt; ++k)
+        for (int b : run) active[k] += problems[b]->nt > k ? 1 : 0;
+    std::vector<int64_t> first(max_nt + 1, 0);
+    for (int k = 0; k < max_nt; ++k) first[k + 1] = first[k] + active[k];
+    const int64_t njobs = first[max_nt];
+    std::vector<egs::BatchJob> jobs((size_t) std::max<int64_t>(njobs, 1));
+    for (int k = 0; k < max_nt; ++k)
+        for (int a = 0; a < active[k]; ++a) {
+            egs_problem_s *p = problems[run[a]];
+            egs::BatchJob &J = jobs[(size_t) (first[k] + a)];
+            J.map = p->map();
+            J.k = k;
+            J.pairs = p->d_pairs;
+            J.winv_k = p->winv(k);
+            J.scratch = p->d_dscr;
+            J.info = p->d_info;
+            J.n = p->n;
+            J.z = p->d_work;
+            J.partials = p->d_partials;
+        }
+    if (na > 0) {
+        EGS_HIP(hipSetDevice(ctx->device));
+        if (njobs > ctx->jobs_cap) {
+            EGS_HIP(hipStreamSynchronize(s));
+            egs::dev_free(ctx->d_jobs, &ctx->dev_bytes);
+            ctx->d_jobs = nullptr;
+            ctx->jobs_cap = 0;
+            void *v = nullptr;
+            if (egs::dev_alloc(&v, sizeof(egs::BatchJob) * (size_t) njobs, &ctx->dev_bytes) != hipSuccess)
+                return fail(EGS_ERR_OUT_OF_MEMORY, "egs_loglik_batch: cannot allocate the job records");
+            ctx->d_jobs = static_cast<egs::BatchJob *>(v);
+            ctx->jobs_cap = njobs;
+        }
+        // the previous batch on this context has completed (every call returns synchronized)
+        EGS_HIP(hipMemcpyAsync(ctx->d_jobs, jobs.data(), sizeof(egs::BatchJob) * (size_t) njobs,
+                               hipMemcpyHostToDevice, s));
+        for (int b : run) {
+            egs_problem_s *p = problems[b];
+            egs::launch_copy_pad(p->d_work, p->d_obs, p->n, p->npad, s);
+            const double *dz = (p->kernel == EGS_KERNEL_UNIVARIATE_MATERN_STATIONARY) ? p->d_z : nullptr;
+            egs::launch_covgen_tiles(p->map(), p->d_pairs, (int) p->h_pairs.size(), prm[b], p->d_x, p->d_y, dz, p->n,
+                                     nullptr, s);
+            egs::launch_zero_u32(reinterpret_cast<unsigned *>(p->d_info), 1, s);
+            p->has_cov = false;
+            p->factored = false;
+            invalidate_graph(p);
+        }
+        // lockstep right-looking factorization: one launch per kernel and step for all problems
+        for (int k = 0; k < max_nt; ++k) {
+            const egs::BatchJob *J = ctx->d_jobs + first[k];
+            egs::launch_factor_diag_batch(J, active[k], ts, s);
+            egs::launch_panel_update_batch(J, active[k + 1], ts, max_nt - k - 1, s);
+        }
+        for (int k = 0; k < max_nt; ++k)
+            egs::launch_forward_batch(ctx->d_jobs + first[k], active[k], ts, max_nt - k - 1, s);
+        EGS_HIP(hipGetLastError());
+        EGS_HIP(hipStreamSynchronize(s));
+    }
+    for (int b : run) {
+        egs_problem_s *p = problems[b];
+        prof_collect(p);
+        int info = 0;
+        double ld = 0.0, q = 0.0;
+        if (hipMemcpy(&info, p->d_info, sizeof(int), hipMemcpyDeviceToHost) != hipSuccess) {
+            status[b] = fail(EGS_ERR_HIP, "egs_loglik_batch: reading the factorization status failed");
+            continue;
+        }
+        if (info != 0) {
+            status[b] = fail(EGS_ERR_NOT_SPD, "dpotrf failed, Matrix is not positive definite (leading minor %d)", info);
+            continue;
+        }
+        if ((status[b] = read_partials(p, ld, q)) != EGS_SUCCESS) continue;
+        p->factored = true;
+        std::memcpy(p->factored_theta, thetas + 6 * b, sizeof(double) * p->ntheta());
+        std::memcpy(p->cov_theta, thetas + 6 * b, sizeof(double) * p->ntheta());
+        loglik[b] = -0.5 * q - 0.5 * ld - (double) p->n / 2.0 * std::log(2.0 * kPi);
+        if (logdet) logdet[b] = ld;
+        if (quad) quad[b] = q;
+    }
+    return EGS_SUCCESS;
+}
+
+int egs_loglik(egs_problem_t p, const double *theta, double *loglik, double *logdet, double *quad) {
+    int st = check_problem(p);
+    if (st) return st;
+    p->fell_back = false;
+    st = loglik_once(p, theta, loglik, logdet, quad);
+    if (!is_mixed(p) || p->fallback_iters <= 0) return st;
+    // The mixed factor gives log|Sigma| directly, and the refinement corrects
+    // only z'Sigma^{-1}z; a refinement that needs many iterations means the
+    // factor is far from Sigma's (smooth fields), so its log-determinant is
+    // not trusted.  Such an evaluation, and one whose mixed factor is not
+    // positive definite, is repeated under the FP64 plan.
+    const bool poor = st == EGS_SUCCESS && p->refine_iters > p->fallback_iters;
+    if (!poor && st != EGS_ERR_NOT_SPD) return st;
+    const int mode = p->mode, b64 = p->band64, b32 = p->band32;
+    if ((st = egs_problem_set_precision(p, EGS_PRECISION_FP64, b64, b32))) return st;
+    const int st_eval = loglik_once(p, theta, loglik, logdet, quad);
+    const std::string err = g_last_error;
+    if ((st = egs_problem_set_precision(p, mode, b64, b32))) return st;   // the plan again, not factored
+    p->fell_back = true;
+    ++p->fallbacks;
+    if (st_eval) g_last_error = err;
+    return st_eval;
+}
+
+int egs_problem_set_fallback(egs_problem_t p, int max_iterations) {
+    int st = check_problem(p);
+    if (st) return st;
+    p->fallback_iters = max_iterations < 0 ? 0 : max_iterations;
+    return EGS_SUCCESS;
+}
+
+int egs_last_fallback(egs_problem_t p, int *fell_back, int64_t *total) {
+    int st = check_problem(p);
+    if (st) return st;
+    if (fell_back) *fell_back = p->fell_back ? 1 : 0;
+    if (total) *total = p->fallbacks;
     return EGS_SUCCESS;
 }
 
@@ -1533,10 +1813,10 @@ int egs_predict_mspe(egs_problem_t p, const double *theta, int64_t n_miss, const
     const int64_t need = rows + 3 * n_miss;
     if (need > p->pred_cap) {
         EGS_HIP(hipStreamSynchronize(s));
-        hipFree(p->d_pred);
+        pfree(p, p->d_pred);
         p->d_pred = nullptr;
         p->pred_cap = 0;
-        if (hipMalloc(&p->d_pred, sizeof(double) * (size_t) need) != hipSuccess) {
+        if (pmalloc(p, &p->d_pred, sizeof(double) * (size_t) need) != hipSuccess) {
             p->d_pred = nullptr;
             return fail(EGS_ERR_OUT_OF_MEMORY, "cannot allocate the prediction buffer");
         }

@@ -128,8 +128,8 @@ __device__ __forceinline__ void acc_foreach(const Acc64 &acc, F f) {
 // Row-block triangular solve against L_kk, column blocks [jb0, jb1).
 // ------------------------------------------------------------------------
 template <typename TX>
-__global__ void __launch_bounds__(kThreads) trsm_rows_kernel(TileMap map, int k, int64_t g0, int jb0, int jb1,
-                                                             const double *winv_k) {
+__device__ __forceinline__ void trsm_rows_body(const TileMap &map, int k, int64_t g0, int jb0, int jb1,
+                                               const double *winv_k, int bx) {
     // 66.5 KB of LDS: T is formed after each GEMM through sm (it reuses sm.a),
     // so that a workgroup fits beside one workgroup of the trailing update
     __shared__ GemmSmem sm;
@@ -137,7 +137,7 @@ __global__ void __launch_bounds__(kThreads) trsm_rows_kernel(TileMap map, int k,
     double (*sT)[kLdsPad] = reinterpret_cast<double (*)[kLdsPad]>(&sm.a[0][0][0]);
     const int ts = map.ts;
     const int nsb = ts / kBlk;
-    const int64_t g = g0 + blockIdx.x;
+    const int64_t g = g0 + bx;
     const int i = (int) (g / nsb);
     const int rb = (int) (g % nsb);
     TX *X = reinterpret_cast<TX *>(map.base + map.off[i * map.nt + k]) + rb * kBlk;
@@ -174,18 +174,25 @@ __global__ void __launch_bounds__(kThreads) trsm_rows_kernel(TileMap map, int k,
     }
 }
 
+template <typename TX>
+__global__ void __launch_bounds__(kThreads) trsm_rows_kernel(TileMap map, int k, int64_t g0, int jb0, int jb1,
+                                                             const double *winv_k) {
+    trsm_rows_body<TX>(map, k, g0, jb0, jb1, winv_k, blockIdx.x);
+}
+
 // ------------------------------------------------------------------------
 // Trailing update, FP64.
 // ------------------------------------------------------------------------
 // p64 (may be null): read panel k from its contiguous FP64 copy (tile i at
 // p64 + (i - k - 1) ts^2) instead of from the tiles.
-__global__ void __launch_bounds__(kThreads) update_kernel(TileMap map, const double *p64, const int2 *pairs, int k) {
+__device__ __forceinline__ void update_body(const TileMap &map, const double *p64, const int2 *pairs, int k, int bx,
+                                            int by) {
     __shared__ GemmSmem sm;
     const int ts = map.ts;
     const int nsb = ts / kBlk;
-    const int2 ij = pairs[blockIdx.y];
-    const int sbm = blockIdx.x % nsb;
-    const int sbn = blockIdx.x / nsb;
+    const int2 ij = pairs[by];
+    const int sbm = bx % nsb;
+    const int sbn = bx / nsb;
     if (ij.x == ij.y && sbm < sbn) return;
     double *C = tile_f64(map, ij.x, ij.y) + sbm * kBlk + (int64_t) sbn * kBlk * ts;
     const int64_t tt = (int64_t) ts * ts;
@@ -200,6 +207,10 @@ __global__ void __launch_bounds__(kThreads) update_kernel(TileMap map, const dou
     acc_foreach(acc, [&](int r, int c, double) { cv[q++] = C[r + (int64_t) c * ts]; });
     q = 0;
     acc_foreach(acc, [&](int r, int c, double v) { C[r + (int64_t) c * ts] = cv[q++] - v; });
+}
+
+__global__ void __launch_bounds__(kThreads) update_kernel(TileMap map, const double *p64, const int2 *pairs, int k) {
+    update_body(map, p64, pairs, k, blockIdx.x, blockIdx.y);
 }
 
 void launch_trsm_rows(const TileMap &map, int k, int64_t g0, int count, int jb0, int jb1, const double *winv_k,
@@ -320,8 +331,7 @@ __device__ __forceinline__ double readlane_f64(double v, int l) {
 // as column j of X.
 constexpr int kPotrfWaves = 8;   // two waves per SIMD: each step's FMAs interleave
 
-__global__ void __launch_bounds__(64 * kPotrfWaves) potrf64_kernel(TileMap map, int k, int jb, double *winv_k,
-                                                                   int *info, int64_t n) {
+__device__ __forceinline__ void potrf64_body(const TileMap &map, int k, int jb, double *winv_k, int *info, int64_t n) {
     constexpr int kW = kPotrfWaves;
     constexpr int kCols = kBlk / kW;   // columns per thread
     __shared__ double s_col[2][kBlk];
@@ -381,14 +391,19 @@ __global__ void __launch_bounds__(64 * kPotrfWaves) potrf64_kernel(TileMap map, 
     }
 }
 
+__global__ void __launch_bounds__(64 * kPotrfWaves) potrf64_kernel(TileMap map, int k, int jb, double *winv_k,
+                                                                   int *info, int64_t n) {
+    potrf64_body(map, k, jb, winv_k, info, n);
+}
+
 // acc = X (64 x 64, column-major, ldx) W^T, W the 64 x 64 lower inverse (ld 64).
 __device__ __forceinline__ void times_wt(Acc64 &acc, const double *X, int64_t ldx, const double *W, GemmSmem &sm) {
     acc_zero(acc);
     wg_gemm_nt(acc, X, ldx, W, kBlk, kBlk, sm);
 }
 
-__global__ void __launch_bounds__(kThreads) diag_update_kernel(TileMap map, int k, int jb, const double *winv_k,
-                                                               double *scratch) {
+__device__ __forceinline__ void diag_update_body(const TileMap &map, int k, int jb, const double *winv_k,
+                                                 double *scratch, int bx) {
     // 66.5 KB of LDS: L_r and L_c stay in registers until both GEMMs are done,
     // then go through sm (sm.a and sm.b), so that a workgroup fits beside one
     // workgroup of the trailing update
@@ -399,8 +414,8 @@ __global__ void __launch_bounds__(kThreads) diag_update_kernel(TileMap map, int 
     double (*sLc)[kLdsPad] = reinterpret_cast<double (*)[kLdsPad]>(&sm.b[0][0][0]);
     const int ts = map.ts;
     const int nsb = ts / kBlk;
-    // blockIdx.x enumerates the pairs jb < c <= r < nsb column by column
-    int idx = blockIdx.x, c = jb + 1;
+    // bx enumerates the pairs jb < c <= r < nsb column by column
+    int idx = bx, c = jb + 1;
     while (idx >= nsb - c) {
         idx -= nsb - c;
         ++c;
@@ -448,11 +463,16 @@ __global__ void __launch_bounds__(kThreads) diag_update_kernel(TileMap map, int 
     }
 }
 
+__global__ void __launch_bounds__(kThreads) diag_update_kernel(TileMap map, int k, int jb, const double *winv_k,
+                                                               double *scratch) {
+    diag_update_body(map, k, jb, winv_k, scratch, blockIdx.x);
+}
+
 // Blocks (r, c), r > c, of the scratch copy back into diagonal tile k.
-__global__ void __launch_bounds__(kThreads) diag_copy_kernel(TileMap map, int k, const double *scratch) {
+__device__ __forceinline__ void diag_copy_body(const TileMap &map, int k, const double *scratch, int bx) {
     const int ts = map.ts;
     const int nsb = ts / kBlk;
-    int idx = blockIdx.x, c = 0;
+    int idx = bx, c = 0;
     while (idx >= nsb - 1 - c) {
         idx -= nsb - 1 - c;
         ++c;
@@ -464,6 +484,67 @@ __global__ void __launch_bounds__(kThreads) diag_copy_kernel(TileMap map, int k,
         const int rr = e & 63, cc = e >> 6;
         T[base + rr + (int64_t) cc * ts] = scratch[base + rr + (int64_t) cc * ts];
     }
+}
+
+__global__ void __launch_bounds__(kThreads) diag_copy_kernel(TileMap map, int k, const double *scratch) {
+    diag_copy_body(map, k, scratch, blockIdx.x);
+}
+
+// ------------------------------------------------------------------------
+// Batched forms: one launch applies the same step to several problems of the
+// same tile size, problem b described by jobs[b] (blockIdx.y, or blockIdx.z
+// for the update); workgroups beyond a problem's extent exit at once.
+// ------------------------------------------------------------------------
+__global__ void __launch_bounds__(64 * kPotrfWaves) potrf64_batch_kernel(const BatchJob *jobs, int jb) {
+    const BatchJob &J = jobs[blockIdx.y];
+    potrf64_body(J.map, J.k, jb, J.winv_k, J.info, J.n);
+}
+
+__global__ void __launch_bounds__(kThreads) diag_update_batch_kernel(const BatchJob *jobs, int jb) {
+    const BatchJob &J = jobs[blockIdx.y];
+    diag_update_body(J.map, J.k, jb, J.winv_k, J.scratch, blockIdx.x);
+}
+
+__global__ void __launch_bounds__(kThreads) diag_copy_batch_kernel(const BatchJob *jobs) {
+    const BatchJob &J = jobs[blockIdx.y];
+    diag_copy_body(J.map, J.k, J.scratch, blockIdx.x);
+}
+
+__global__ void __launch_bounds__(kThreads) trsm_rows_batch_kernel(const BatchJob *jobs) {
+    const BatchJob &J = jobs[blockIdx.y];
+    const int nsb = J.map.ts / kBlk;
+    if ((int) blockIdx.x >= (J.map.nt - J.k - 1) * nsb) return;
+    trsm_rows_body<double>(J.map, J.k, (int64_t) (J.k + 1) * nsb, 0, nsb, J.winv_k, blockIdx.x);
+}
+
+__global__ void __launch_bounds__(kThreads) update_batch_kernel(const BatchJob *jobs) {
+    const BatchJob &J = jobs[blockIdx.z];
+    const int rest = J.map.nt - J.k - 1;   // tile columns k+1 .. nt-1: the first rest (rest + 1) / 2 pairs
+    if ((int) blockIdx.y >= rest * (rest + 1) / 2) return;
+    update_body(J.map, nullptr, J.pairs, J.k, blockIdx.x, blockIdx.y);
+}
+
+void launch_factor_diag_batch(const BatchJob *jobs, int njobs, int ts, hipStream_t s) {
+    if (njobs <= 0) return;
+    const int nsb = ts / kBlk;
+    for (int jb = 0; jb < nsb; ++jb) {
+        hipLaunchKernelGGL(potrf64_batch_kernel, dim3(1, njobs), dim3(64 * kPotrfWaves), 0, s, jobs, jb);
+        const int rem = nsb - jb - 1;
+        if (rem > 0)
+            hipLaunchKernelGGL(diag_update_batch_kernel, dim3((unsigned) (rem * (rem + 1) / 2), njobs), dim3(kThreads),
+                               0, s, jobs, jb);
+    }
+    if (nsb > 1)
+        hipLaunchKernelGGL(diag_copy_batch_kernel, dim3((unsigned) (nsb * (nsb - 1) / 2), njobs), dim3(kThreads), 0, s,
+                           jobs);
+}
+
+void launch_panel_update_batch(const BatchJob *jobs, int njobs, int ts, int rest, hipStream_t s) {
+    if (njobs <= 0 || rest <= 0) return;
+    const int nsb = ts / kBlk;
+    hipLaunchKernelGGL(trsm_rows_batch_kernel, dim3((unsigned) (rest * nsb), njobs), dim3(kThreads), 0, s, jobs);
+    hipLaunchKernelGGL(update_batch_kernel, dim3((unsigned) (nsb * nsb), (unsigned) (rest * (rest + 1) / 2), njobs),
+                       dim3(kThreads), 0, s, jobs);
 }
 
 void launch_factor_diag(const TileMap &map, int k, double *winv_k, double *scratch, int *info, int64_t n,

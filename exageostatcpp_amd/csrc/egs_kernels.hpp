@@ -11,6 +11,22 @@
 
 namespace egs {
 
+// One problem's share of a batched step (FP64 plan): step k of the tiled
+// Cholesky or of the forward solve on the tiles of `map`, with that problem's
+// update order (pairs), diagonal-block inverses of tile k, diagonal scratch,
+// not-SPD flag (n: its real dimension) and solve vector with its partials.
+struct BatchJob {
+    TileMap map;
+    int k;
+    const int2 *pairs;
+    double *winv_k;
+    double *scratch;
+    int *info;
+    int64_t n;
+    double *z;
+    double *partials;
+};
+
 // ---- covariance generation (egs_covgen.hip)
 // sig64 (may be null): also writes every generated tile in FP64 to
 // sig64 + (i*(i+1)/2 + j) * ts * ts, column-major.
@@ -64,6 +80,14 @@ int matvec_chunks(int64_t n);
 // the inverses of its diagonal blocks in winv_k; scratch holds ts * ts doubles.
 void launch_factor_diag(const TileMap &map, int k, double *winv_k, double *scratch, int *info, int64_t n,
                         hipStream_t s);
+// Batched steps over njobs problems of tile size ts (jobs in device memory):
+// factor diagonal tile k of each; then solve panel k and apply it to tile
+// columns k+1 .. of each (rest = the largest nt - k - 1 among the jobs; jobs
+// with fewer tiles exit early); forward-solve tile row k with its partials
+// and subtract it from the rows below.
+void launch_factor_diag_batch(const BatchJob *jobs, int njobs, int ts, hipStream_t s);
+void launch_panel_update_batch(const BatchJob *jobs, int njobs, int ts, int rest, hipStream_t s);
+void launch_forward_batch(const BatchJob *jobs, int njobs, int ts, int rest, hipStream_t s);
 // X_k = L_kk^{-T} for every diagonal tile k (tinv: nt * ts * ts doubles).
 void launch_tinv(const TileMap &map, const double *winv, double *tinv, hipStream_t s);
 // Preconditioner solves with the diagonal-tile inverses (right-looking):

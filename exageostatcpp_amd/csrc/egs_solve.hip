@@ -52,8 +52,8 @@ __device__ double wg_sum_1024(double v, double *red) {
     return r;
 }
 
-__global__ void __launch_bounds__(kSolveThreads) trsv_diag_kernel(TileMap map, int k, const double *winv_k, double *z,
-                                                                  double *partials) {
+__device__ __forceinline__ void trsv_diag_body(const TileMap &map, int k, const double *winv_k, double *z,
+                                               double *partials) {
     __shared__ double sy[kMaxTs];
     __shared__ double sp[kGroups][kBlk];
     __shared__ double st[kBlk];
@@ -113,6 +113,16 @@ __global__ void __launch_bounds__(kSolveThreads) trsv_diag_kernel(TileMap map, i
         partials[2 * k] = ld;
         partials[2 * k + 1] = q;
     }
+}
+
+__global__ void __launch_bounds__(kSolveThreads) trsv_diag_kernel(TileMap map, int k, const double *winv_k, double *z,
+                                                                  double *partials) {
+    trsv_diag_body(map, k, winv_k, z, partials);
+}
+
+__global__ void __launch_bounds__(kSolveThreads) trsv_diag_batch_kernel(const BatchJob *jobs) {
+    const BatchJob &J = jobs[blockIdx.x];
+    trsv_diag_body(J.map, J.k, J.winv_k, J.z, J.partials);
 }
 
 // scratch[(i - k - 1) * ts + c] = sum_r L_ik[r][c] y_i[r] for every tile row i > k.
@@ -594,16 +604,16 @@ __device__ __forceinline__ double panel_row_dot(const TA *__restrict__ A, const 
     return (a[0] + a[1]) + (a[2] + a[3]);
 }
 
-__global__ void __launch_bounds__(kThreads) fwd_panel_kernel(TileMap map, int k, const double *__restrict__ yk,
-                                                             double *__restrict__ z) {
+__device__ __forceinline__ void fwd_panel_body(const TileMap &map, int k, const double *__restrict__ yk,
+                                               double *__restrict__ z, int bx) {
     __shared__ double sy[kMaxTs];
     __shared__ double sp[4][kBlk];
     const int ts = map.ts;
     const int nsb = ts / kBlk;
     const int t = threadIdx.x;
     const int lane = t & 63, w = t >> 6;
-    const int i = k + 1 + (int) (blockIdx.x / nsb);
-    const int rb = (int) (blockIdx.x % nsb);
+    const int i = k + 1 + (int) (bx / nsb);
+    const int rb = (int) (bx % nsb);
     for (int e = t; e < ts; e += kThreads) sy[e] = yk[e];
     __syncthreads();
     double s;
@@ -612,6 +622,18 @@ __global__ void __launch_bounds__(kThreads) fwd_panel_kernel(TileMap map, int k,
     sp[w][lane] = s;
     __syncthreads();
     if (t < kBlk) z[(int64_t) i * ts + rb * kBlk + t] -= ((sp[0][t] + sp[1][t]) + sp[2][t]) + sp[3][t];
+}
+
+__global__ void __launch_bounds__(kThreads) fwd_panel_kernel(TileMap map, int k, const double *__restrict__ yk,
+                                                             double *__restrict__ z) {
+    fwd_panel_body(map, k, yk, z, blockIdx.x);
+}
+
+__global__ void __launch_bounds__(kThreads) fwd_panel_batch_kernel(const BatchJob *jobs) {
+    const BatchJob &J = jobs[blockIdx.y];
+    const int nsb = J.map.ts / kBlk;
+    if ((int) blockIdx.x >= (J.map.nt - J.k - 1) * nsb) return;
+    fwd_panel_body(J.map, J.k, J.z + (int64_t) J.k * J.map.ts, J.z, blockIdx.x);
 }
 
 // y[r] = sum_{c >= r} X[r][c] x[c]: one workgroup of 16 waves per 64 rows,
@@ -709,6 +731,14 @@ void launch_fwd_panel(const TileMap &map, int k, const double *y_k, double *z, h
     if (rows <= 0) return;
     const int nsb = map.ts / kBlk;
     hipLaunchKernelGGL(fwd_panel_kernel, dim3((unsigned) (rows * nsb)), dim3(kThreads), 0, s, map, k, y_k, z);
+}
+
+void launch_forward_batch(const BatchJob *jobs, int njobs, int ts, int rest, hipStream_t s) {
+    if (njobs <= 0) return;
+    hipLaunchKernelGGL(trsv_diag_batch_kernel, dim3((unsigned) njobs), dim3(kSolveThreads), 0, s, jobs);
+    if (rest > 0)
+        hipLaunchKernelGGL(fwd_panel_batch_kernel, dim3((unsigned) (rest * (ts / kBlk)), njobs), dim3(kThreads), 0, s,
+                           jobs);
 }
 
 void launch_bwd_diag_inv(const double *tinv_k, const double *x_k, double *y_k, int ts, hipStream_t s) {

@@ -85,8 +85,30 @@ int egs_covariance_tile(egs_context_t ctx, int kernel, double *A, int rows, int 
 int egs_problem_create(egs_context_t ctx, int kernel, int64_t n_loc, const double *x, const double *y,
                        const double *z, int metric, int dts, egs_problem_t *problem);
 int egs_problem_destroy(egs_problem_t problem);
-/* Device bytes the problem holds. */
+/* Moves the problem to another context of the same device: its later calls
+ * run on that context's streams (the serving loop hands idle regions to
+ * whichever worker takes their next request).  No counterpart in the
+ * reference. */
+int egs_problem_set_context(egs_problem_t problem, egs_context_t ctx);
+/* Device bytes the problem holds (each allocation counted with its size
+ * class, see egs_memory_cache). */
 int egs_problem_device_bytes(egs_problem_t problem, int64_t *bytes);
+/* Device bytes a problem of this shape and plan holds once evaluated (its
+ * tiles, factor workspaces and vectors; under a mixed plan also the panel
+ * copies, the FP64 copy of Sigma and the diagonal-tile inverses), before it
+ * exists: the footprint the serving loop admits requests by.  dim is 2 or 3;
+ * mode and bands as egs_problem_set_precision.  Prediction buffers (grown on
+ * demand, 8 * n_miss * (P + 3) bytes) are not included. */
+int egs_problem_estimate_bytes(int kernel, int64_t n_loc, int dim, int dts, int mode, int band_fp64, int band_fp32,
+                               int64_t *bytes);
+/* Device memory of problems comes in size classes (256-byte granules up to
+ * 64 KiB, eight classes per octave above).  Released blocks are kept for the
+ * next allocation of the same class while live plus cached bytes stay within
+ * limit_bytes, because hipFree waits for the whole device; limit 0 (the
+ * default) releases every block at once.  limit_bytes < 0 leaves the limit
+ * unchanged.  Reports the cached and live bytes of the current device (either
+ * pointer may be NULL).  No counterpart in the reference. */
+int egs_memory_cache(int64_t limit_bytes, int64_t *cached_bytes, int64_t *live_bytes);
 
 /* Precision plan: EGS_PRECISION_FP64, or a mixed plan with tiles whose tile
  * distance |i - j| <= band_fp64 in FP64, <= band_fp32 in FP32 and the rest
@@ -123,6 +145,17 @@ int egs_problem_set_fp64_product_band(egs_problem_t problem, int band);
  * profiling and mixed plans use the direct path.  Off by default; no
  * counterpart in the reference. */
 int egs_problem_set_graph(egs_problem_t problem, int enable);
+/* Guard of mixed plans: log|Sigma| is read off the mixed factor and the
+ * refinement corrects only z'Sigma^{-1}z, so an evaluation whose refinement
+ * needs more than max_iterations iterations (the factor is far from Sigma's,
+ * as on smooth fields), or whose mixed factor is not positive definite, is
+ * repeated under the FP64 plan (the plan is restored afterwards, unfactored).
+ * Default 4 (the default plan refines in 2 at n = 100k); 0 turns it off.  No
+ * counterpart in the reference (its precision setting is ignored). */
+int egs_problem_set_fallback(egs_problem_t problem, int max_iterations);
+/* Whether the last egs_loglik fell back to the FP64 plan, and how many
+ * evaluations of the problem did (either pointer may be NULL). */
+int egs_last_fallback(egs_problem_t problem, int *fell_back, int64_t *total);
 /* Iterations and final relative FP64 residual of the last refinement. */
 int egs_last_refinement(egs_problem_t problem, int *iterations, double *residual);
 /* Block updates (192 x 192) of the tiles beyond the single-product band in the
@@ -143,6 +176,20 @@ int egs_problem_get_observations(egs_problem_t problem, double *z, int64_t n);
  * logdet and quad may be NULL.  Returns EGS_ERR_NOT_SPD when Sigma(theta) is
  * not positive definite (ChameleonDense.cpp:29-32). */
 int egs_loglik(egs_problem_t problem, const double *theta, double *loglik, double *logdet, double *quad);
+
+/* egs_loglik for several problems at once (FP64 plan, one context, one tile
+ * size): every step of the tiled Cholesky and of the forward solve is one
+ * launch for all problems (problem b at step k while k < its tile count), so
+ * the latency-bound diagonal-tile chains of small problems run side by side
+ * instead of one after another.  theta of problem b is thetas[6 b ..] (its
+ * first 3 or 6 entries).  status[b] receives that problem's result
+ * (EGS_SUCCESS, EGS_ERR_NOT_SPD, EGS_ERR_INVALID_ARG for its theta) and
+ * loglik[b], logdet[b], quad[b] its values (logdet and quad may be NULL).
+ * Each problem keeps its factor, as after egs_loglik.  Returns an error only
+ * for an invalid batch.  No counterpart in the reference (it evaluates one
+ * problem per call). */
+int egs_loglik_batch(egs_problem_t *problems, const double *thetas, int count, double *loglik, double *logdet,
+                     double *quad, int *status);
 
 /* The phases of egs_loglik, for callers that drive them one by one:
  *   RuntimeFunctions<T>::CovarianceMatrix (src/runtime/starpu/StarPuFunctions.cpp:24-45),
@@ -253,17 +300,31 @@ int egs_data_modeling(egs_problem_t problem, const double *lb, const double *ub,
  * serving layer: it evaluates one problem per process through
  * ExaGeoStatDataModeling / ExaGeoStatPrediction (ExaGeoStat.cpp:91-125); the
  * server runs those two operations (egs_loglik, egs_predict) for registered
- * regions concurrently.  `workers` contexts (1..16) each own a pair of streams;
- * a region is pinned to one worker, so its requests run in submission order
- * and a prediction at the theta of the region's last evaluation reuses the
- * factor kept in HBM.  Regions are made resident on demand within
- * hbm_budget_bytes, releasing the least recently used idle ones. */
+ * regions concurrently.  Requests wait in one queue; each of `workers`
+ * contexts (1..16, each with its own pair of streams) takes the oldest request
+ * whose region is idle, so a region's requests run in submission order.  A
+ * worker that takes a likelihood request of a small FP64 region (at most 16
+ * tiles) also takes the other queued ones of the same tile size (up to 32)
+ * and evaluates them together with egs_loglik_batch.  A region stays resident
+ * after its request: a prediction at the theta of the region's last
+ * evaluation reuses the factor kept in HBM.  Regions are admitted by their
+ * footprint (egs_problem_estimate_bytes) within hbm_budget_bytes, releasing
+ * the least recently used idle ones; while the server lives, released device
+ * blocks are cached for later regions within the same budget
+ * (egs_memory_cache, set back to 0 by egs_server_destroy). */
 typedef struct egs_server_s *egs_server_t;
 int egs_server_create(int device, int64_t hbm_budget_bytes, int workers, egs_server_t *server);
 int egs_server_destroy(egs_server_t server);
-/* Copies the locations and observations (n_loc * P values); *region is its id. */
+/* Copies the locations and observations (n_loc * P values); *region is its
+ * id.  The region is evaluated under the FP64 plan. */
 int egs_server_add_region(egs_server_t server, int kernel, int64_t n_loc, const double *x, const double *y,
                           const double *z, const double *obs, int metric, int dts, int *region);
+/* The same with a precision plan (mode and bands as egs_problem_set_precision;
+ * a mixed plan needs a tile size that is a multiple of 192).  Fails with
+ * EGS_ERR_OUT_OF_MEMORY when the region's footprint exceeds the budget. */
+int egs_server_add_region_plan(egs_server_t server, int kernel, int64_t n_loc, const double *x, const double *y,
+                               const double *z, const double *obs, int metric, int dts, int mode, int band_fp64,
+                               int band_fp32, int *region);
 int egs_server_submit_loglik(egs_server_t server, int region, const double *theta, int64_t *ticket);
 int egs_server_submit_predict(egs_server_t server, int region, const double *theta, int64_t n_miss, const double *xm,
                               const double *ym, int64_t *ticket);
@@ -276,6 +337,10 @@ int egs_server_wait(egs_server_t server, int64_t ticket, double *value, double *
  * the first submission, and the HBM held by resident regions. */
 int egs_server_stats(egs_server_t server, int64_t *completed, double *p50_ms, double *p99_ms, double *per_second,
                      int64_t *resident_bytes);
+/* Batches run and the requests they held; predictions, and those that found
+ * their region's factor at their theta still in HBM. */
+int egs_server_batch_stats(egs_server_t server, int64_t *batches, int64_t *batched_requests, int64_t *predicts,
+                           int64_t *predict_hits);
 int egs_server_reset_stats(egs_server_t server);
 
 #ifdef __cplusplus

@@ -1,0 +1,100 @@
+"""The smooth-field workflow end to end (maximum likelihood, then kriging of
+jointly generated held-out points) at a size the oracle finishes in seconds.
+
+theta = (1, 0.1, 1.0); 1936 jittered-grid locations plus 100 uniform held-out
+points whose observations are generated jointly with the grid's
+(tests/golden/make_mle_fixture.py writes the oracle optimum).  The GPU fit
+starts at the lower bounds as the reference does (Configurations.cpp:200) and
+must reach the oracle optimum within 1e-4; kriging of the held-out points at
+the GPU estimate must match the oracle's predictions and MSPE within 1e-8
+(LinearAlgebraMethods.cpp:451-605, Prediction.cpp:165-197).
+"""
+import json
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import exageostatcpp_amd as egs
+from oracle import orc
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+sys.path.insert(0, os.path.join(ROOT, "tests", "golden"))
+import make_mle_fixture as fx  # noqa: E402
+
+
+@pytest.fixture(scope="module")
+def case():
+    with open(os.path.join(ROOT, "tests", "golden", "mle_c2_n1936.json")) as fh:
+        f = json.load(fh)
+    (x, y), zo, (xm, ym), zh = fx.c2_data()
+    # the same data the fixture was computed on
+    assert abs(np.sum(zo) - f["z_obs_sum"]) < 1e-9 and abs(np.sum(zh) - f["z_held_sum"]) < 1e-9
+    return f, (x, y), zo, (xm, ym), zh
+
+
+def test_smooth_field_fit_then_held_out_kriging(case):
+    f, (x, y), zo, (xm, ym), zh = case
+    with egs.Context(0) as ctx:
+        p = egs.Problem(ctx, 0, x, y, dts=f["dts"])
+        p.set_observations(zo)
+        print("smooth-field fit n=%d from the lower bounds" % len(x), flush=True)
+        th, llh, nev = p.data_modeling(f["lb"], f["ub"], tolerance=1e-11, max_iterations=3000)
+        print("theta-hat %s llh %.10f after %d evaluations (oracle %s %.10f)" % (list(th), llh, nev, f["theta_hat"],
+                                                                              f["loglik"]), flush=True)
+        assert np.max(np.abs(np.array(th) - f["theta_hat"])) < 1e-4
+        assert llh >= f["loglik"] - 1e-6
+        zm, m3 = p.predict_mspe(list(th), xm, ym, z_actual=zh)
+        p.close()
+    zref, mref = orc.krige(0, (x, y), (xm, ym), list(th), zo, f["dts"], z_actual=zh)
+    assert np.max(np.abs(zm - zref)) < 1e-8
+    assert abs(m3[0] - mref) < 1e-8
+    # the held-out points are predicted far better than their variance (the field is smooth)
+    assert m3[0] < 0.1 * np.var(zh)
+
+
+def test_smooth_field_mixed_fit_lands_within_1e3_of_fp64(case):
+    """Under the default mixed plan (with its FP64 fallback for evaluations
+    whose refinement needs more than 4 iterations) the fit of the smooth field
+    must land within 1e-3 of the FP64 optimum; the fallback count is reported."""
+    f, (x, y), zo, _, _ = case
+    with egs.Context(0) as ctx:
+        p = egs.Problem(ctx, 0, x, y, dts=192)
+        p.set_observations(zo)
+        p.set_precision(egs.PRECISION_MIXED)
+        print("smooth-field mixed fit n=%d" % len(x), flush=True)
+        th, llh, nev = p.data_modeling(f["lb"], f["ub"], tolerance=1e-11, max_iterations=3000)
+        _, fallbacks = p.fallback()
+        it = p.refinement()["iterations"]
+        p.close()
+    print("mixed theta-hat %s llh %.10f, %d evaluations, %d fell back to FP64, last refinement %d iterations"
+          % (list(th), llh, nev, fallbacks, it), flush=True)
+    assert np.max(np.abs(np.array(th) - f["theta_hat"])) < 1e-3
+
+
+def test_fallback_repeats_a_poorly_refined_evaluation_in_fp64(case):
+    """A fallback threshold below the refinement's iteration count forces the
+    FP64 repeat: the value then equals the FP64 plan's."""
+    f, (x, y), zo, _, _ = case
+    th = f["theta_true"]
+    with egs.Context(0) as ctx:
+        p = egs.Problem(ctx, 0, x, y, dts=192)
+        p.set_observations(zo)
+        want = p.loglik(th)
+        p.set_precision(egs.PRECISION_MIXED)
+        p.set_fallback(0)
+        mixed = p.loglik(th)
+        iters = p.refinement()["iterations"]
+        assert p.fallback() == (False, 0)
+        if iters >= 1:
+            p.set_fallback(iters - 1 if iters > 1 else 1)
+            got = p.loglik(th)
+            fell, total = p.fallback()
+            if iters > 1:
+                assert fell and total == 1
+                assert got == want
+        p.close()
+    assert abs(mixed[0] - want[0]) <= 1e-5 * abs(want[0])

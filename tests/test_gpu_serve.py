@@ -1,8 +1,11 @@
 """Request server (egs_server_*): many regions of uneven size on one GPU.
 
 Every served result must equal the same request evaluated alone on its own
-problem (same data, same tile size): the server adds concurrency and caching,
-not arithmetic.  The HBM budget forces evictions in the second test.
+problem (same data, same tile size, same plan): the server adds concurrency,
+batching and caching, not arithmetic.  Served results are also checked against
+the oracle (FP64 plan 1e-10 relative on the log-likelihood and 1e-8 on
+predictions; mixed plan 1e-6 relative).  The HBM budget forces evictions in
+one test.
 """
 import sys
 
@@ -51,14 +54,16 @@ def _requests(regions, count, seed):
     return reqs
 
 
-def _alone(regions, reqs):
+def _alone(regions, reqs, mode=egs.PRECISION_FP64, dts=DTS):
     """Each request on a fresh problem of its region, one at a time."""
     out = []
     with egs.Context(0) as ctx:
         for kind, r, th, xm, ym in reqs:
             x, y, z, _ = regions[r]
-            p = egs.Problem(ctx, 0, x, y, dts=DTS)
+            p = egs.Problem(ctx, 0, x, y, dts=dts)
             p.set_observations(z)
+            if mode != egs.PRECISION_FP64:
+                p.set_precision(mode, 4, 4)
             if kind == "loglik":
                 out.append(p.loglik(th)[0])
             else:
@@ -67,9 +72,9 @@ def _alone(regions, reqs):
     return out
 
 
-def _served(regions, reqs, budget, workers):
+def _served(regions, reqs, budget, workers, mode=egs.PRECISION_FP64, dts=DTS, stats=None):
     srv = capi.Server(0, budget, workers)
-    ids = [srv.add_region(0, x, y, z, dts=DTS) for x, y, z, _ in regions]
+    ids = [srv.add_region(0, x, y, z, dts=dts, mode=mode) for x, y, z, _ in regions]
     tickets = []
     for kind, r, th, xm, ym in reqs:
         if kind == "loglik":
@@ -82,6 +87,8 @@ def _served(regions, reqs, budget, workers):
         assert lat > 0.0
         res.append(v if kind == "loglik" else zm)
     st = srv.stats()
+    if stats is not None:
+        stats.update(srv.batch_stats())
     srv.close()
     return res, st
 
@@ -95,24 +102,54 @@ def _compare(got, want):
 
 
 def test_served_results_equal_requests_run_alone(regions):
+    """All requests are queued at once, so workers take batches of the small
+    FP64 likelihood requests: batched results must equal the alone ones."""
     reqs = _requests(regions, 40, 1)
     log("serve: 40 requests over %d regions, 3 workers" % len(regions))
-    got, st = _served(regions, reqs, 64 << 30, 3)
+    bs = {}
+    got, st = _served(regions, reqs, 64 << 30, 3, stats=bs)
     want = _alone(regions, reqs)
     _compare(got, want)
     assert st["completed"] == len(reqs) and st["p99_ms"] >= st["p50_ms"] > 0.0
+    log("serve: %s" % bs)
+    assert bs["batches"] > 0 and bs["batched_requests"] >= 2 * bs["batches"]
+    assert bs["predicts"] == sum(1 for q in reqs if q[0] == "predict")
+
+
+def test_served_mixed_plan_equals_alone_and_the_oracle(regions):
+    reqs = _requests(regions, 24, 3)
+    log("serve: 24 requests, mixed plan, 2 workers")
+    got, _ = _served(regions, reqs, 64 << 30, 2, mode=egs.PRECISION_MIXED, dts=192)
+    want = _alone(regions, reqs, mode=egs.PRECISION_MIXED, dts=192)
+    _compare(got, want)
+    for (kind, r, th, xm, ym), g in zip(reqs, got):
+        x, y, z, _ = regions[r]
+        if kind == "loglik" and len(x) <= 1200:
+            ref = orc.loglik(0, (x, y), th, z, 192)["llh"]
+            assert abs(g - ref) <= 1e-6 * abs(ref), (g, ref)
+
+
+def test_served_fp64_results_match_the_oracle(regions):
+    reqs = [q for q in _requests(regions, 30, 4) if len(regions[q[1]][0]) <= 1200][:12]
+    log("serve: %d requests against the oracle" % len(reqs))
+    got, _ = _served(regions, reqs, 64 << 30, 2)
+    for (kind, r, th, xm, ym), g in zip(reqs, got):
+        x, y, z, _ = regions[r]
+        if kind == "loglik":
+            ref = orc.loglik(0, (x, y), th, z, DTS)["llh"]
+            assert abs(g - ref) <= 1e-10 * abs(ref), (g, ref)
+        else:
+            zo, _ = orc.krige(0, (x, y), (xm, ym), th, z, DTS)
+            assert np.max(np.abs(g - zo)) < 1e-8
 
 
 def test_small_budget_evicts_and_still_matches(regions):
     reqs = _requests(regions, 30, 2)
-    # room for the largest region plus a quarter (the server's own estimate of
-    # a region's bytes): the regions together need about one and a half times that
-    def est(n):
-        ts = (DTS + 63) // 64 * 64
-        nt = (n + ts - 1) // ts
-        return nt * (nt + 1) // 2 * ts * ts * 8 + nt * ts * 64 * 8 + 8 * ts * ts + 4 * nt * ts * 8 + n * 24
-    budget = int(1.25 * est(max(SIZES)))
-    assert sum(est(n) for n in SIZES) > 1.4 * budget
+    fp = {n: capi.estimate_bytes(0, n, DTS) for n in SIZES}
+    # room for the largest region plus a quarter: the regions together need
+    # about one and a half times that
+    budget = int(1.25 * max(fp.values()))
+    assert sum(fp.values()) > 1.4 * budget
     log("serve: 30 requests, budget %d MB, 2 workers" % (budget >> 20))
     got, st = _served(regions, reqs, budget, 2)
     want = _alone(regions, reqs)
@@ -123,9 +160,22 @@ def test_small_budget_evicts_and_still_matches(regions):
 def test_region_larger_than_budget_is_an_error(regions):
     srv = capi.Server(0, 1 << 20, 1)
     x, y, z, th = regions[3]
-    rid = srv.add_region(0, x, y, z, dts=DTS)
-    t = srv.submit_loglik(rid, th)
     with pytest.raises(capi.EGSError) as e:
-        srv.wait(t)
+        srv.add_region(0, x, y, z, dts=DTS)
     assert e.value.code == capi.ERR_OUT_OF_MEMORY
     srv.close()
+
+
+@pytest.mark.parametrize("mode,dts", [(egs.PRECISION_FP64, 320), (egs.PRECISION_MIXED, 192)])
+def test_footprint_estimate_equals_device_bytes(regions, mode, dts):
+    """The admission footprint is what an evaluated problem holds."""
+    x, y, z, th = regions[3]
+    with egs.Context(0) as ctx:
+        p = egs.Problem(ctx, 0, x, y, dts=dts)
+        p.set_observations(z)
+        if mode != egs.PRECISION_FP64:
+            p.set_precision(mode, 4, 4)
+        p.loglik(th)
+        held = p.device_bytes()
+        p.close()
+    assert held == capi.estimate_bytes(0, len(x), dts, mode, 4, 4)
