@@ -66,6 +66,8 @@ EXPORTS = {
     "egs_synchronize": ([_vp], _int),
     "egs_covariance_tile": ([_vp, _int, _dp, _int, _int, _int, _int, _dp, _dp, _dp, _i64, _dp, _dp, _dp, _i64, _dp,
                              _int], _int),
+    "egs_covariance_tile_noalloc": ([_vp, _int, _dp, _int, _int, _int, _int, _dp, _dp, _dp, _i64, _dp, _dp, _dp, _i64,
+                                     _dp, _int], _int),
     "egs_problem_create": ([_vp, _int, _i64, _dp, _dp, _dp, _int, _int, ctypes.POINTER(_vp)], _int),
     "egs_problem_destroy": ([_vp], _int),
     "egs_problem_set_context": ([_vp, _vp], _int),

@@ -47,6 +47,29 @@ int fail(int code, const char *fmt, ...) {
 constexpr double kPi = 3.141592653589793;   // Definitions.hpp:45
 constexpr int kMaxTileSize = 2048;
 
+// Device scratch of egs_covariance_tile_noalloc: the output tile and the
+// location slices it stages, at most kScratchDoubles doubles (32 MiB), held by
+// every GPU the library initializes until the process ends.  The array's name
+// is a host-side stand-in; host code uses only the address that
+// hipGetSymbolAddress returns for the current device (g_scratch).
+constexpr int64_t kScratchDoubles = int64_t(1) << 22;
+static __device__ __align__(256) double scratch[kScratchDoubles];
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+double *g_scratch[kMaxDevices] = {};
+
+// Resolves this device's copy of the scratch array once (the device is current).
+int resolve_scratch(int device) {
+    std::lock_guard<std::mutex> lk(g_scratch_mutex);
+    if (g_scratch[device]) return EGS_SUCCESS;
+    void *ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+        return fail(EGS_ERR_HIP, "hipGetSymbolAddress(scratch) failed: %s", hipGetErrorString(e));
+    g_scratch[device] = static_cast<double *>(ptr);
+    return EGS_SUCCESS;
+}
+
 }  // namespace
 
 namespace egs {
@@ -1031,7 +1054,10 @@ int egs_init(int device, egs_context_t *ctx) {
     if (hipGetDeviceCount(&count) != hipSuccess || count <= 0)
         return fail(EGS_ERR_NO_DEVICE, "no HIP device is visible");
     if (device < 0 || device >= count) return fail(EGS_ERR_INVALID_ARG, "device %d out of range [0, %d)", device, count);
+    if (device >= kMaxDevices) return fail(EGS_ERR_UNSUPPORTED, "device %d beyond the library's %d", device, kMaxDevices);
     EGS_HIP(hipSetDevice(device));
+    const int sst = resolve_scratch(device);
+    if (sst) return sst;
     auto *c = new egs_context_s();
     c->device = device;
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
@@ -1090,6 +1116,25 @@ int egs_covariance_tile(egs_context_t ctx, int kernel, double *A, int rows, int 
     if (st) return st;
     if (e != hipSuccess) return fail(EGS_ERR_HIP, "covariance tile: %s", hipGetErrorString(e));
     return EGS_SUCCESS;
+}
+
+int egs_covariance_tile_noalloc(egs_context_t ctx, int kernel, double *A, int rows, int cols, int row_off, int col_off,
+                                const double *x1, const double *y1, const double *z1, int64_t n1, const double *x2,
+                                const double *y2, const double *z2, int64_t n2, const double *theta, int metric) {
+    TileRequest t;
+    int st = check_tile_request(ctx, kernel, A, rows, cols, row_off, col_off, x1, y1, z1, n1, x2, y2, z2, n2, theta,
+                                metric, t);
+    if (st || rows == 0 || cols == 0) return st;
+    if (t.scratch_doubles() > kScratchDoubles)
+        return fail(EGS_ERR_INVALID_ARG,
+                    "egs_covariance_tile_noalloc: the tile and its location slices need %lld doubles; the static "
+                    "scratch holds %lld (rows * cols + 3 * (row and column locations))",
+                    (long long) t.scratch_doubles(), (long long) kScratchDoubles);
+    EGS_HIP(hipSetDevice(ctx->device));
+    double *buf = g_scratch[ctx->device];   // resolved by egs_init on this device
+    if (!buf) return fail(EGS_ERR_STATE, "egs_covariance_tile_noalloc: the device scratch is not resolved");
+    // every element of the tile is written, so nothing in the scratch needs clearing
+    return covariance_tile_on(ctx, t, A, x1, y1, z1, x2, y2, z2, buf);
 }
 
 int egs_problem_create(egs_context_t ctx, int kernel, int64_t n_loc, const double *x, const double *y,

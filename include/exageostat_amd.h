@@ -76,6 +76,19 @@ int egs_covariance_tile(egs_context_t ctx, int kernel, double *A, int rows, int 
                         const double *x2, const double *y2, const double *z2, int64_t n2,
                         const double *theta, int metric);
 
+/* The same tile with no device allocation, for callers that cannot allocate
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  No counterpart in the reference.  The output tile and the
+ * location slices it reads (rows / P and cols / P locations) are staged in a
+ * static device array of 2^22 doubles (32 MiB) that every GPU the library
+ * initializes holds until the process ends: rows * cols + 3 * (ceil(rows / P)
+ * + ceil(cols / P)) must fit, else EGS_ERR_INVALID_ARG.  Calls that overlap on one
+ * GPU share that array and race; results equal egs_covariance_tile's. */
+int egs_covariance_tile_noalloc(egs_context_t ctx, int kernel, double *A, int rows, int cols, int row_off,
+                                int col_off, const double *x1, const double *y1, const double *z1, int64_t n1,
+                                const double *x2, const double *y2, const double *z2, int64_t n2,
+                                const double *theta, int metric);
+
 /* ---- problems (a set of locations and its device-resident Sigma) ---------
  * Replaces LinearAlgebraMethods<T>::InitiateDescriptors
  * (src/linear-algebra-solvers/LinearAlgebraMethods.cpp:35-118): the covariance
