@@ -207,6 +207,12 @@ def kernel_table(ks, steps):
     return out
 
 
+# Tile size of the serving regions: a multiple of 192, as mixed plans need, and
+# the size the observations are generated with, so that an FP64 evaluation at
+# the generating theta repeats the generation's factorization.
+SERVE_DTS = 384
+
+
 def serve_workload(egs, args):
     """The serving mix: args.serve_requests requests; 80% likelihood evaluations,
     each for a new region with n log-uniform in [500, 20000] (jittered-grid
@@ -235,7 +241,7 @@ def serve_workload(egs, args):
         for reg in regions:
             n, th, seed = reg[0], reg[1], reg[2]
             x, y = egs.generate_locations(n, seed=seed)
-            p = egs.Problem(ctx, "UnivariateMaternStationary", x, y, dts=320)
+            p = egs.Problem(ctx, "UnivariateMaternStationary", x, y, dts=SERVE_DTS)
             reg[3], reg[4], reg[5] = x, y, p.generate_observations(th, seed=seed)
             p.close()
     egs.capi.memory_cache(0)
@@ -250,7 +256,7 @@ def serve_run(egs, args, reqs, regions, arrivals, mode):
     import numpy as np
 
     mixed = mode != egs.PRECISION_FP64
-    dts = 384 if mixed else 320   # mixed plans need a multiple of 192
+    dts = SERVE_DTS
     srv = egs.capi.Server(0, args.serve_budget_gb << 30, args.serve_workers)
     ids = [srv.add_region(0, r[3], r[4], r[5], dts=dts, mode=mode, band_fp64=args.band_fp64,
                           band_fp32=args.band_fp32) for r in regions]
@@ -272,10 +278,14 @@ def serve_run(egs, args, reqs, regions, arrivals, mode):
             tickets.append(srv.submit_predict(ids[r], th, xm, ym))
         else:
             tickets.append(srv.submit_loglik(ids[r], th))
-    out = []
+    out, failed = [], []
     for i, t in enumerate(tickets):
-        v, zm, _ = srv.wait(t)
-        out.append(zm if reqs[i][0] == "predict" else v)
+        try:
+            v, zm, _ = srv.wait(t)
+            out.append(zm if reqs[i][0] == "predict" else v)
+        except egs.EGSError as e:   # e.g. a covariance that is not positive definite at that theta
+            out.append(None)
+            failed.append((i, str(e)))
     wall = time.perf_counter() - start
     st = srv.stats()
     bs = srv.batch_stats()
@@ -284,9 +294,14 @@ def serve_run(egs, args, reqs, regions, arrivals, mode):
         "%d of %d predictions on a cached factor" % ("mixed" if mixed else "fp64", len(tickets), args.serve_rate,
                                                     st["p50_ms"], st["p99_ms"], bs["batches"], bs["batched_requests"],
                                                     bs["predict_hits"], bs["predicts"]))
-    return {"plan": ("mixed (%d, %d), dts %d" % (args.band_fp64, args.band_fp32, dts)) if mixed else "fp64, dts 320",
+    if failed:
+        log("serve: %d requests failed, first: %s" % (len(failed), failed[0][1]))
+    return {"plan": ("mixed (%d, %d) with its FP64 fallback, dts %d" % (args.band_fp64, args.band_fp32, dts)) if mixed
+            else "fp64, dts %d" % dts,
+            "failed": len(failed), "failure_example": failed[0][1] if failed else None,
             "p50_ms": st["p50_ms"], "p99_ms": st["p99_ms"], "completed_per_s": len(tickets) / wall,
             "batches": bs["batches"], "batched_requests": bs["batched_requests"],
+            "regions_switched_to_fp64": bs["regions_switched_to_fp64"],
             "predictions_on_cached_factor": [bs["predict_hits"], bs["predicts"]]}, out
 
 
@@ -300,9 +315,9 @@ def serve_phase(egs, args):
     reqs, regions, arrivals = serve_workload(egs, args)
     fp64, r64 = serve_run(egs, args, reqs, regions, arrivals, egs.PRECISION_FP64)
     mixed, rmx = serve_run(egs, args, reqs, regions, arrivals, egs.PRECISION_MIXED)
-    llh_err = max(abs(a - b) / abs(b) for (k, *_), a, b in zip(reqs, rmx, r64) if k == "loglik")
-    pred_err = max((float(np.max(np.abs(a - b))) for (k, *_), a, b in zip(reqs, rmx, r64) if k == "predict"),
-                   default=0.0)
+    both = [(k, a, b) for (k, *_), a, b in zip(reqs, rmx, r64) if a is not None and b is not None]
+    llh_err = max((abs(a - b) / abs(b) for k, a, b in both if k == "loglik"), default=0.0)
+    pred_err = max((float(np.max(np.abs(a - b))) for k, a, b in both if k == "predict"), default=0.0)
     sizes = [r[0] for r in regions]
     return {"requests": len(reqs), "rate_per_s": args.serve_rate, "regions": len(regions),
             "n_range": [min(sizes), max(sizes)], "workers": args.serve_workers,
@@ -482,7 +497,8 @@ def main():
                          "(negative: never)")
     ap.add_argument("--fp64-product-band", type=int, default=None,
                     help="FP64-band tiles beyond this band take split FP16 products into FP64 storage "
-                         "(negative: none; default: the library's, 1 for univariate and 2 for bivariate kernels)")
+                         "(negative: every tile of the FP64 band takes FP64 products; default: the library's, 0 for "
+                         "univariate and 2 for bivariate kernels)")
     ap.add_argument("--single-tau", type=float, default=2.0 ** -16,
                     help="... when their panel rows' scales multiply to <= tau times the panel's largest squared")
     ap.add_argument("--plain-bf16", action="store_true", help="single BF16 product instead of the split one")

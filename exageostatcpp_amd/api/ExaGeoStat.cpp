@@ -745,20 +745,35 @@ T ExaGeoStat<T>::ExaGeoStatDataModeling(configurations::Configurations &aConfigu
         }
         return -llh;
     };
-    if (free_idx.empty()) {
-        objective(nullptr, &st);
-    } else {
-        const int nf = static_cast<int>(free_idx.size());
+    const int nf = static_cast<int>(free_idx.size());
+    int used = 0;
+    auto search = [&](const std::vector<double> &start, int budget) {
         std::vector<double> xf(nf), lf(nf), uf(nf);
         for (int j = 0; j < nf; ++j) {
-            xf[j] = x[free_idx[j]];
+            xf[j] = start[free_idx[j]];
             lf[j] = lb[free_idx[j]];
             uf[j] = ub[free_idx[j]];
         }
         double fmin = 0.0;
         int nev = 0, reason = 0;
-        egs_bobyqa(nf, xf.data(), lf.data(), uf.data(), aConfigurations.GetTolerance(),
-                   aConfigurations.GetMaxMleIterations(), 0.0, objective, &st, &fmin, &nev, &reason);
+        egs_bobyqa(nf, xf.data(), lf.data(), uf.data(), aConfigurations.GetTolerance(), budget, 0.0, objective, &st,
+                   &fmin, &nev, &reason);
+        used += nev;
+    };
+    if (free_idx.empty()) objective(nullptr, &st);
+    else search(x, aConfigurations.GetMaxMleIterations());
+    // Under MIXED precision the search ends with an FP64 polish from the mixed
+    // optimum (as egs_data_modeling does): the reported optimum is FP64's
+    configurations::Configurations c64 = aConfigurations;
+    if (!st.failed && nf > 0 && PrecisionMode(aConfigurations) != EGS_PRECISION_FP64 &&
+        used < aConfigurations.GetMaxMleIterations()) {
+        c64.SetPrecision(common::DOUBLE);
+        st.conf = &c64;
+        const std::vector<double> start = st.best_theta;
+        st.theta = start;
+        st.best = -INFINITY;
+        search(start, aConfigurations.GetMaxMleIterations() - used);
+        st.conf = &aConfigurations;
     }
     if (st.failed) throw std::runtime_error(st.error);
     aConfigurations.SetStartingTheta(st.best_theta);

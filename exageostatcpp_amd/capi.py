@@ -89,8 +89,8 @@ EXPORTS = {
     "egs_trsm_vector": ([_vp, _int, ctypes.c_double, _dp, _i64], _int),
     "egs_last_timings": ([_vp, _dp, _dp, _dp], _int),
     "egs_last_refinement": ([_vp, ctypes.POINTER(_int), _dp], _int),
-    "egs_problem_set_fallback": ([_vp, _int], _int),
-    "egs_last_fallback": ([_vp, ctypes.POINTER(_int), ctypes.POINTER(_i64)], _int),
+    "egs_problem_set_fallback": ([_vp, _int, ctypes.c_double], _int),
+    "egs_last_fallback": ([_vp, ctypes.POINTER(_int), ctypes.POINTER(_i64), _dp], _int),
     "egs_last_product_counts": ([_vp, ctypes.POINTER(_i64), ctypes.POINTER(_i64)], _int),
     "egs_problem_profile": ([_vp, _int], _int),
     "egs_problem_kernel_stats": ([_vp, _int, ctypes.POINTER(_i64), _dp, _dp], _int),
@@ -112,7 +112,7 @@ EXPORTS = {
     "egs_server_stats": ([_vp, ctypes.POINTER(_i64), _dp, _dp, _dp, ctypes.POINTER(_i64)], _int),
     "egs_server_reset_stats": ([_vp], _int),
     "egs_server_batch_stats": ([_vp, ctypes.POINTER(_i64), ctypes.POINTER(_i64), ctypes.POINTER(_i64),
-                                ctypes.POINTER(_i64)], _int),
+                                ctypes.POINTER(_i64), ctypes.POINTER(_i64)], _int),
 }
 
 OBJECTIVE_FN = ctypes.CFUNCTYPE(ctypes.c_double, _dp, _vp)
@@ -320,16 +320,18 @@ class Problem:
         _check(lib().egs_last_refinement(self.handle, ctypes.byref(it), ctypes.byref(res)))
         return {"iterations": it.value, "residual": res.value}
 
-    def set_fallback(self, max_iterations):
-        """Mixed plans: repeat an evaluation under FP64 when the refinement needs
-        more than max_iterations iterations or the mixed factor is not SPD (0: never)."""
-        _check(lib().egs_problem_set_fallback(self.handle, int(max_iterations)))
+    def set_fallback(self, max_iterations=4, tolerance=1e-5):
+        """Mixed plans: repeat an evaluation under FP64 when its estimated relative
+        llh error exceeds tolerance, its refinement needs more than max_iterations
+        iterations, or the mixed factor is not SPD (0 turns a test off)."""
+        _check(lib().egs_problem_set_fallback(self.handle, int(max_iterations), float(tolerance)))
 
     def fallback(self):
-        """(the last evaluation fell back to FP64, evaluations that did)."""
-        f, t = _int(), _i64()
-        _check(lib().egs_last_fallback(self.handle, ctypes.byref(f), ctypes.byref(t)))
-        return bool(f.value), t.value
+        """(the last evaluation fell back to FP64, evaluations that did,
+        error estimate of the last mixed evaluation)."""
+        f, t, e = _int(), _i64(), ctypes.c_double()
+        _check(lib().egs_last_fallback(self.handle, ctypes.byref(f), ctypes.byref(t), ctypes.byref(e)))
+        return bool(f.value), t.value, e.value
 
     def set_graph(self, enable=True):
         """Replay FP64-plan evaluations from a captured hipGraph."""
@@ -496,10 +498,11 @@ class Server:
         _check(lib().egs_server_reset_stats(self.handle))
 
     def batch_stats(self):
-        b, br, p, ph = _i64(), _i64(), _i64(), _i64()
+        b, br, p, ph, fr = _i64(), _i64(), _i64(), _i64(), _i64()
         _check(lib().egs_server_batch_stats(self.handle, ctypes.byref(b), ctypes.byref(br), ctypes.byref(p),
-                                            ctypes.byref(ph)))
-        return {"batches": b.value, "batched_requests": br.value, "predicts": p.value, "predict_hits": ph.value}
+                                            ctypes.byref(ph), ctypes.byref(fr)))
+        return {"batches": b.value, "batched_requests": br.value, "predicts": p.value, "predict_hits": ph.value,
+                "regions_switched_to_fp64": fr.value}
 
 
 def loglik_batch(problems, thetas):

@@ -502,7 +502,7 @@ BobyqaResult Solver::run(double *x) {
         if (!(ub_[i] > lb_[i]) || !std::isfinite(lb_[i]) || !std::isfinite(ub_[i]))
             return {BobyqaStatus::kInvalid, kInf, 0};
         const double xi = std::min(ub_[i], std::max(lb_[i], x[i]));
-        double step = 0.25 * (ub_[i] - lb_[i]);
+        double step = opt_.step_scale * (ub_[i] - lb_[i]);
         if (ub_[i] - xi < step && ub_[i] > xi) step = 0.75 * (ub_[i] - xi);
         if (xi - lb_[i] < step && xi > lb_[i]) step = 0.75 * (xi - lb_[i]);
         scale_[i] = step;

@@ -48,6 +48,7 @@ struct BobyqaOptions {
     double ftol_abs = 0.0;   // stop when an improvement of the best value is below this (0: off)
     int maxeval = 1000;      // objective evaluations, counting the initial design
     double rho_end = 1e-7;   // final trust-region radius, relative to the initial step
+    double step_scale = 0.25;   // initial step as a fraction of ub - lb (NLopt's default 0.25)
 };
 
 struct BobyqaResult {

@@ -161,6 +161,11 @@ struct egs_problem_s {
     // fallback_iters iterations (0: never), or whose mixed factor is not
     // positive definite, is repeated under the FP64 plan
     int fallback_iters = 4;
+    // ... or whose estimated relative log-likelihood error exceeds fallback_tol:
+    // |q0 - q| / (2 max(|llh|, 1)), q0 = z'(L L^T)^{-1} z with the mixed factor
+    // and q = z'Sigma^{-1} z refined (see egs_loglik)
+    double fallback_tol = 1e-5;
+    double quad_unrefined = 0.0, error_estimate = 0.0;
     bool fell_back = false;
     int64_t fallbacks = 0;
 
@@ -862,6 +867,8 @@ int refine_solve(egs_problem_s *p, const double *b) {
     int st = host_dot(p, b, b, bb);
     if (st) return st;
     apply_factor_inverse(p, b, X);
+    // b' (L L^T)^{-1} b with the factor alone: the quadratic form before refinement
+    if ((st = host_dot(p, b, X, p->quad_unrefined))) return st;
     st = sigma_times(p, X, R, b);
     if (st) return st;
     st = host_dot(p, R, R, rr);
@@ -1063,6 +1070,7 @@ int egs_init(int device, egs_context_t *ctx) {
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
         hipStreamCreateWithFlags(&c->bulk, hipStreamNonBlocking) != hipSuccess) {
         if (c->stream) hipStreamDestroy(c->stream);
+        if (c->bulk) hipStreamDestroy(c->bulk);
         delete c;
         return fail(EGS_ERR_HIP, "hipStreamCreate failed");
     }
@@ -1453,13 +1461,17 @@ int graph_launch(egs_problem_s *p, const double *theta) {
 
 // One evaluation under the problem's current plan (egs_loglik without the
 // FP64 fallback of mixed plans).
-int loglik_once(egs_problem_s *p, const double *theta, double *loglik, double *logdet, double *quad) {
+static int loglik_once(egs_problem_s *p, const double *theta, double *loglik, double *logdet, double *quad) {
     int st = EGS_SUCCESS;
     if (!loglik) return fail(EGS_ERR_INVALID_ARG, "loglik is null");
     if (!p->has_obs) return fail(EGS_ERR_STATE, "the problem has no observations");
     hipStream_t s = p->ctx->stream;
     if (p->use_graph && !is_mixed(p) && !p->prof.on && !p->graph_exec) {
-        // first use: capture; if the runtime refuses, evaluate directly from now on
+        // first use: capture (after checking theta, which the replay needs
+        // valid: the capture itself leaves the tiles unchanged); if the
+        // runtime refuses, evaluate directly from now on
+        egs::MaternParams check;
+        if ((st = make_params(p->kernel, p->metric, theta, check))) return st;
         st = graph_capture(p);
         if (st == EGS_ERR_HIP) {
             (void) hipGetLastError();
@@ -1659,14 +1671,27 @@ int egs_loglik(egs_problem_t p, const double *theta, double *loglik, double *log
     int st = check_problem(p);
     if (st) return st;
     p->fell_back = false;
+    double q_local = 0.0;
+    if (!quad) quad = &q_local;
     st = loglik_once(p, theta, loglik, logdet, quad);
-    if (!is_mixed(p) || p->fallback_iters <= 0) return st;
+    if (!is_mixed(p)) return st;
     // The mixed factor gives log|Sigma| directly, and the refinement corrects
     // only z'Sigma^{-1}z; a refinement that needs many iterations means the
     // factor is far from Sigma's (smooth fields), so its log-determinant is
     // not trusted.  Such an evaluation, and one whose mixed factor is not
     // positive definite, is repeated under the FP64 plan.
-    const bool poor = st == EGS_SUCCESS && p->refine_iters > p->fallback_iters;
+    // With w = L^{-1} z and S = L^{-1} Sigma L^{-T} - I, the mixed factor's
+    // log-determinant is off by log det(I + S) ~ tr S, and q0 - q = w'w -
+    // w'(I + S)^{-1} w ~ w'S w, whose mean is tr S when w ~ N(0, I) (z drawn
+    // from the model): one probe of the log-determinant error that the
+    // refinement already paid for.
+    if (st == EGS_SUCCESS && quad)
+        p->error_estimate = std::fabs(p->quad_unrefined - *quad) / (2.0 * std::max(std::fabs(*loglik), 1.0));
+    else if (st == EGS_SUCCESS)
+        p->error_estimate = 0.0;
+    if (p->fallback_iters <= 0 && p->fallback_tol <= 0.0) return st;   // the guard is off
+    const bool poor = st == EGS_SUCCESS && ((p->fallback_iters > 0 && p->refine_iters > p->fallback_iters) ||
+                                            (p->fallback_tol > 0.0 && p->error_estimate > p->fallback_tol));
     if (!poor && st != EGS_ERR_NOT_SPD) return st;
     const int mode = p->mode, b64 = p->band64, b32 = p->band32;
     if ((st = egs_problem_set_precision(p, EGS_PRECISION_FP64, b64, b32))) return st;
@@ -1679,18 +1704,21 @@ int egs_loglik(egs_problem_t p, const double *theta, double *loglik, double *log
     return st_eval;
 }
 
-int egs_problem_set_fallback(egs_problem_t p, int max_iterations) {
+int egs_problem_set_fallback(egs_problem_t p, int max_iterations, double tolerance) {
     int st = check_problem(p);
     if (st) return st;
+    if (std::isnan(tolerance)) return fail(EGS_ERR_INVALID_ARG, "the fallback tolerance is NaN");
     p->fallback_iters = max_iterations < 0 ? 0 : max_iterations;
+    p->fallback_tol = tolerance < 0.0 ? 0.0 : tolerance;
     return EGS_SUCCESS;
 }
 
-int egs_last_fallback(egs_problem_t p, int *fell_back, int64_t *total) {
+int egs_last_fallback(egs_problem_t p, int *fell_back, int64_t *total, double *error_estimate) {
     int st = check_problem(p);
     if (st) return st;
     if (fell_back) *fell_back = p->fell_back ? 1 : 0;
     if (total) *total = p->fallbacks;
+    if (error_estimate) *error_estimate = p->error_estimate;
     return EGS_SUCCESS;
 }
 
@@ -1814,9 +1842,25 @@ int egs_generate_locations(int64_t n, int dim, double *x, double *y, double *z) 
     return EGS_SUCCESS;
 }
 
+static int generate_observations_fp64(egs_problem_s *p, const double *theta, int seed, double *z_out);
+
 int egs_generate_observations(egs_problem_t p, const double *theta, int seed, double *z_out) {
     int st = check_problem(p);
     if (st) return st;
+    if (!is_mixed(p)) return generate_observations_fp64(p, theta, seed, z_out);
+    // the data comes from the FP64 factor whatever the plan: FP64 for the
+    // generation, then the plan again (unfactored)
+    const int mode = p->mode, b64 = p->band64, b32 = p->band32;
+    if ((st = egs_problem_set_precision(p, EGS_PRECISION_FP64, b64, b32))) return st;
+    const int st_gen = generate_observations_fp64(p, theta, seed, z_out);
+    const std::string err = g_last_error;
+    if ((st = egs_problem_set_precision(p, mode, b64, b32))) return st;
+    if (st_gen) g_last_error = err;
+    return st_gen;
+}
+
+static int generate_observations_fp64(egs_problem_s *p, const double *theta, int seed, double *z_out) {
+    int st = EGS_SUCCESS;
     hipStream_t s = p->ctx->stream;
     // iseed = {seed, seed, seed, 1} (LinearAlgebraMethods.cpp:348) as one 48-bit word
     const uint64_t sd = (uint64_t) (uint32_t) seed;
@@ -1840,10 +1884,9 @@ int egs_generate_observations(egs_problem_t p, const double *theta, int seed, do
     return EGS_SUCCESS;
 }
 
-int egs_predict_mspe(egs_problem_t p, const double *theta, int64_t n_miss, const double *xm, const double *ym,
-                     const double *zm, double *zmiss, const double *z_actual, double *mspe3) {
-    int st = check_problem(p);
-    if (st) return st;
+static int predict_once(egs_problem_s *p, const double *theta, int64_t n_miss, const double *xm, const double *ym,
+                        const double *zm, double *zmiss, const double *z_actual, double *mspe3) {
+    int st = EGS_SUCCESS;
     if (!theta || !xm || !ym || !zmiss || n_miss <= 0) return fail(EGS_ERR_INVALID_ARG, "invalid prediction request");
     if (!p->has_obs) return fail(EGS_ERR_STATE, "the problem has no observations");
     const bool three_d = p->dim == 3;
@@ -1926,6 +1969,26 @@ int egs_predict_mspe(egs_problem_t p, const double *theta, int64_t n_miss, const
     return EGS_SUCCESS;
 }
 
+int egs_predict_mspe(egs_problem_t p, const double *theta, int64_t n_miss, const double *xm, const double *ym,
+                     const double *zm, double *zmiss, const double *z_actual, double *mspe3) {
+    int st = check_problem(p);
+    if (st) return st;
+    p->fell_back = false;
+    st = predict_once(p, theta, n_miss, xm, ym, zm, zmiss, z_actual, mspe3);
+    if (st != EGS_ERR_NOT_SPD || !is_mixed(p) || (p->fallback_iters <= 0 && p->fallback_tol <= 0.0)) return st;
+    // a mixed factor that is not positive definite: the prediction under the
+    // FP64 plan, then the plan again (unfactored), as egs_loglik does
+    const int mode = p->mode, b64 = p->band64, b32 = p->band32;
+    if ((st = egs_problem_set_precision(p, EGS_PRECISION_FP64, b64, b32))) return st;
+    const int st_pred = predict_once(p, theta, n_miss, xm, ym, zm, zmiss, z_actual, mspe3);
+    const std::string err = g_last_error;
+    if ((st = egs_problem_set_precision(p, mode, b64, b32))) return st;
+    p->fell_back = true;
+    ++p->fallbacks;
+    if (st_pred) g_last_error = err;
+    return st_pred;
+}
+
 int egs_predict(egs_problem_t p, const double *theta, int64_t n_miss, const double *xm, const double *ym,
                 const double *zm, double *zmiss, const double *z_actual, double *mspe) {
     double m3[3] = {-1.0, 0.0, 0.0};
@@ -1993,20 +2056,20 @@ int egs_data_modeling(egs_problem_t p, const double *lb, const double *ub, const
         }
         return v;
     };
-    if (free_idx.empty()) {
-        llh_at(x);
-    } else {
-        const int nf = static_cast<int>(free_idx.size());
+    const int nf = static_cast<int>(free_idx.size());
+    // one BOBYQA search from `start` with the given initial step, at most budget evaluations
+    auto search = [&](const std::vector<double> &start, double step_scale, int budget) {
         std::vector<double> xf(nf), lf(nf), hf(nf);
         for (int j = 0; j < nf; ++j) {
-            xf[j] = x[free_idx[j]];
+            xf[j] = start[free_idx[j]];
             lf[j] = lo[free_idx[j]];
             hf[j] = hi[free_idx[j]];
         }
         egs::BobyqaOptions opt;
         opt.ftol_abs = tolerance > 0.0 ? tolerance : 0.0;
-        opt.maxeval = max_iterations;
-        std::vector<double> th(x);
+        opt.maxeval = budget;
+        opt.step_scale = step_scale;
+        std::vector<double> th(start);
         egs::bobyqa_minimize(
             nf, xf.data(), lf.data(), hf.data(),
             [&](const double *v) {
@@ -2014,6 +2077,22 @@ int egs_data_modeling(egs_problem_t p, const double *lb, const double *ub, const
                 return -llh_at(th);
             },
             opt);
+    };
+    if (free_idx.empty()) llh_at(x);
+    else search(x, 0.25, max_iterations);
+    // Under a mixed plan the search ends with an FP64 polish: the mixed
+    // log-likelihood carries an error of order 1e-7 relative that varies with
+    // theta, which can stop BOBYQA early along the flat ridges of smooth fields;
+    // a second search with the FP64 plan from the mixed optimum (smaller first
+    // steps) finds the FP64 optimum.  The reported theta and value are FP64's.
+    if (is_mixed(p) && nf > 0 && eval_status == EGS_SUCCESS && count < max_iterations) {
+        const int mode = p->mode, b64 = p->band64, b32 = p->band32;
+        int st_plan = egs_problem_set_precision(p, EGS_PRECISION_FP64, b64, b32);
+        if (st_plan) return st_plan;
+        const std::vector<double> start(theta, theta + nt);
+        best = -INFINITY;
+        search(start, 0.05, max_iterations - count);
+        if ((st_plan = egs_problem_set_precision(p, mode, b64, b32))) return st_plan;
     }
     if (nevals) *nevals = count;
     if (eval_status != EGS_SUCCESS) {

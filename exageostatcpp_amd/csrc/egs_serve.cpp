@@ -106,7 +106,7 @@ struct egs_server_s {
     bool stop = false;
     std::vector<double> latencies;
     double t_first = -1.0, t_last = 0.0;
-    int64_t batches = 0, batched_requests = 0, predicts = 0, predict_hits = 0;
+    int64_t batches = 0, batched_requests = 0, predicts = 0, predict_hits = 0, switched_regions = 0;
 
     bool batchable(const Region &r) const {
         return r.mode == EGS_PRECISION_FP64 && r.nt <= kBatchTiles;
@@ -219,6 +219,7 @@ struct egs_server_s {
             std::vector<Result> res(nb);
             std::vector<int> ready;
             std::vector<char> hit(nb, 0);
+            bool switched = false;   // a mixed region fell back and now runs FP64
             for (int b = 0; b < nb; ++b) {
                 Region &r = *regs[b];
                 const bool had = r.prob != nullptr;
@@ -254,6 +255,15 @@ struct egs_server_s {
                 int st;
                 if (rq.kind == kLoglik) {
                     st = egs_loglik(r.prob, rq.theta, &res[0].value, nullptr, nullptr);
+                    int fell = 0;
+                    if (r.mode != EGS_PRECISION_FP64 && egs_last_fallback(r.prob, &fell, nullptr, nullptr) == EGS_SUCCESS &&
+                        fell) {
+                        // the mixed factor was not good enough for this region's covariance:
+                        // later requests use the FP64 plan directly (the problem stays
+                        // within the footprint it was admitted with)
+                        if (egs_problem_set_precision(r.prob, EGS_PRECISION_FP64, r.band64, r.band32) == EGS_SUCCESS)
+                            switched = true;
+                    }
                 } else {
                     res[0].zmiss.assign(rq.xm.size() * (size_t) r.p, 0.0);
                     st = egs_predict(r.prob, rq.theta, (int64_t) rq.xm.size(), rq.xm.data(), rq.ym.data(), nullptr,
@@ -269,6 +279,10 @@ struct egs_server_s {
                     ++batches;
                     batched_requests += nb;
                 }
+                if (switched) {
+                    regs[0]->mode = EGS_PRECISION_FP64;
+                    ++switched_regions;
+                }
                 for (int b = 0; b < nb; ++b) {
                     Request &rq = batch[b];
                     Region &r = *regs[b];
@@ -280,11 +294,11 @@ struct egs_server_s {
                         predict_hits += hit[b];
                     }
                     // the problem holds the factor of its last successful request
-                    if (r.prob && res[b].status == EGS_SUCCESS) {
+                    if (r.prob && res[b].status == EGS_SUCCESS && !(switched && b == 0)) {
                         r.evaluated = true;
                         std::memcpy(r.last_theta, rq.theta, sizeof(r.last_theta));
                     } else {
-                        r.evaluated = false;
+                        r.evaluated = false;   // failed, or re-planned to FP64 (unfactored)
                     }
                     Result &out = res[b];
                     out.done = true;
@@ -391,8 +405,11 @@ int egs_server_add_region_plan(egs_server_t s, int kernel, int64_t n_loc, const 
     r.n_loc = n_loc;
     r.ts = std::min(((dts + 63) / 64) * 64, 2048);
     r.nt = (int) ((n_loc * r.p + r.ts - 1) / r.ts);
+    // a mixed plan whose FP64 band covers every tile stores nothing in lower
+    // precision: such a region runs the FP64 plan (and may join batches)
+    if (mode != EGS_PRECISION_FP64 && r.nt - 1 <= band_fp64) r.mode = EGS_PRECISION_FP64;
     int64_t fp = 0;
-    const int st = egs_problem_estimate_bytes(kernel, n_loc, r.dim, dts, mode, band_fp64, band_fp32, &fp);
+    int st = egs_problem_estimate_bytes(kernel, n_loc, r.dim, dts, mode, band_fp64, band_fp32, &fp);
     if (st != EGS_SUCCESS) return st;
     // plus the prediction buffer of a 100-point request
     r.footprint = fp + 8 * 100 * (r.p + 3) + 4096;
@@ -478,9 +495,10 @@ int egs_server_stats(egs_server_t s, int64_t *completed, double *p50_ms, double 
 }
 
 int egs_server_batch_stats(egs_server_t s, int64_t *batches, int64_t *batched_requests, int64_t *predicts,
-                           int64_t *predict_hits) {
+                           int64_t *predict_hits, int64_t *fp64_regions) {
     if (!s) return serve_fail(EGS_ERR_INVALID_ARG, "egs_server_batch_stats: null server");
     std::lock_guard<std::mutex> lk(s->m);
+    if (fp64_regions) *fp64_regions = s->switched_regions;
     if (batches) *batches = s->batches;
     if (batched_requests) *batched_requests = s->batched_requests;
     if (predicts) *predicts = s->predicts;
@@ -494,7 +512,7 @@ int egs_server_reset_stats(egs_server_t s) {
     s->latencies.clear();
     s->t_first = -1.0;
     s->t_last = 0.0;
-    s->batches = s->batched_requests = s->predicts = s->predict_hits = 0;
+    s->batches = s->batched_requests = s->predicts = s->predict_hits = s->switched_regions = 0;
     return EGS_SUCCESS;
 }
 

@@ -158,17 +158,24 @@ int egs_problem_set_fp64_product_band(egs_problem_t problem, int band);
  * profiling and mixed plans use the direct path.  Off by default; no
  * counterpart in the reference. */
 int egs_problem_set_graph(egs_problem_t problem, int enable);
-/* Guard of mixed plans: log|Sigma| is read off the mixed factor and the
- * refinement corrects only z'Sigma^{-1}z, so an evaluation whose refinement
- * needs more than max_iterations iterations (the factor is far from Sigma's,
- * as on smooth fields), or whose mixed factor is not positive definite, is
- * repeated under the FP64 plan (the plan is restored afterwards, unfactored).
- * Default 4 (the default plan refines in 2 at n = 100k); 0 turns it off.  No
+/* Guard of mixed plans.  log|Sigma| is read off the mixed factor L and the
+ * refinement corrects only z'Sigma^{-1}z.  With S = L^{-1} Sigma L^{-T} - I the
+ * log-determinant is off by about tr S, and the change of the quadratic form
+ * under refinement, q0 - q with q0 = z'(L L^T)^{-1} z, is a one-probe estimate
+ * of it (w'S w with w = L^{-1} z); its half over max(|loglik|, 1) estimates the
+ * relative log-likelihood error.  An evaluation is repeated under the FP64
+ * plan (the mixed plan restored afterwards, unfactored) when that estimate
+ * exceeds `tolerance`, when the refinement needs more than max_iterations
+ * iterations, or when the mixed factor is not positive definite (the last
+ * also for egs_predict / egs_predict_mspe).  Defaults:
+ * 4 iterations and 1e-5 (the default plan's estimate at n = 100k on the
+ * exponential field is of order 1e-7); 0 turns either test off.  No
  * counterpart in the reference (its precision setting is ignored). */
-int egs_problem_set_fallback(egs_problem_t problem, int max_iterations);
-/* Whether the last egs_loglik fell back to the FP64 plan, and how many
- * evaluations of the problem did (either pointer may be NULL). */
-int egs_last_fallback(egs_problem_t problem, int *fell_back, int64_t *total);
+int egs_problem_set_fallback(egs_problem_t problem, int max_iterations, double tolerance);
+/* Whether the last egs_loglik fell back to the FP64 plan, how many
+ * evaluations of the problem did, and the error estimate of its last mixed
+ * evaluation (any pointer may be NULL). */
+int egs_last_fallback(egs_problem_t problem, int *fell_back, int64_t *total, double *error_estimate);
 /* Iterations and final relative FP64 residual of the last refinement. */
 int egs_last_refinement(egs_problem_t problem, int *iterations, double *residual);
 /* Block updates (192 x 192) of the tiles beyond the single-product band in the
@@ -263,7 +270,8 @@ int egs_generate_locations(int64_t n, int dim, double *x, double *y, double *z);
 /* LinearAlgebraMethods<T>::GenerateObservationsVector
  * (LinearAlgebraMethods.cpp:333-449): z = L(theta) e with e = dlarnv(3,
  * {seed, seed, seed, 1}); the result is written to z_out (length n_loc * P, may
- * be NULL) and becomes the problem's observations. */
+ * be NULL) and becomes the problem's observations.  L is the FP64 factor under
+ * every plan (a mixed plan is restored afterwards, unfactored). */
 int egs_generate_observations(egs_problem_t problem, const double *theta, int seed, double *z_out);
 
 /* ---- kriging ---------------------------------------------------------------
@@ -304,7 +312,10 @@ int egs_bobyqa(int n, double *x, const double *lb, const double *ub, double ftol
  * theta receives the best point, *loglik the best value (NLopt's
  * last_optimum_value), *nevals the evaluation count (may be NULL).  A failed
  * evaluation (not positive definite, invalid theta) ends the search with its
- * status, as the reference's exception does. */
+ * status, as the reference's exception does.  Under a mixed plan the search
+ * ends with an FP64 polish within the same evaluation budget: a second search
+ * under the FP64 plan from the mixed optimum (first steps 0.05 (ub - lb)),
+ * whose optimum is the one returned (the mixed plan is restored). */
 int egs_data_modeling(egs_problem_t problem, const double *lb, const double *ub, const double *start, double tolerance,
                       int max_iterations, double *theta, double *loglik, int *nevals);
 
@@ -324,7 +335,10 @@ int egs_data_modeling(egs_problem_t problem, const double *lb, const double *ub,
  * footprint (egs_problem_estimate_bytes) within hbm_budget_bytes, releasing
  * the least recently used idle ones; while the server lives, released device
  * blocks are cached for later regions within the same budget
- * (egs_memory_cache, set back to 0 by egs_server_destroy). */
+ * (egs_memory_cache, set back to 0 by egs_server_destroy).  A mixed-plan
+ * region whose tiles all lie in the FP64 band runs the FP64 plan; one whose
+ * evaluation falls back to FP64 (egs_problem_set_fallback) keeps the FP64
+ * plan for its later requests. */
 typedef struct egs_server_s *egs_server_t;
 int egs_server_create(int device, int64_t hbm_budget_bytes, int workers, egs_server_t *server);
 int egs_server_destroy(egs_server_t server);
@@ -351,9 +365,10 @@ int egs_server_wait(egs_server_t server, int64_t ticket, double *value, double *
 int egs_server_stats(egs_server_t server, int64_t *completed, double *p50_ms, double *p99_ms, double *per_second,
                      int64_t *resident_bytes);
 /* Batches run and the requests they held; predictions, and those that found
- * their region's factor at their theta still in HBM. */
+ * their region's factor at their theta still in HBM; mixed-plan regions that
+ * switched to the FP64 plan after an evaluation fell back to it. */
 int egs_server_batch_stats(egs_server_t server, int64_t *batches, int64_t *batched_requests, int64_t *predicts,
-                           int64_t *predict_hits);
+                           int64_t *predict_hits, int64_t *fp64_regions);
 int egs_server_reset_stats(egs_server_t server);
 
 #ifdef __cplusplus

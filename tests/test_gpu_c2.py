@@ -57,9 +57,11 @@ def test_smooth_field_fit_then_held_out_kriging(case):
 
 
 def test_smooth_field_mixed_fit_lands_within_1e3_of_fp64(case):
-    """Under the default mixed plan (with its FP64 fallback for evaluations
-    whose refinement needs more than 4 iterations) the fit of the smooth field
-    must land within 1e-3 of the FP64 optimum; the fallback count is reported."""
+    """The default mixed plan's fit of the smooth field: the mixed search, then
+    the FP64 polish from its optimum, must land within 1e-3 of the FP64 optimum
+    (the polish makes it the FP64 optimum; the mixed search alone stopped near
+    (2.08, 0.142, 1.004), about 1 away along the flat sigma^2-range ridge, with
+    per-evaluation errors of only ~1e-4 absolute)."""
     f, (x, y), zo, _, _ = case
     with egs.Context(0) as ctx:
         p = egs.Problem(ctx, 0, x, y, dts=192)
@@ -67,34 +69,44 @@ def test_smooth_field_mixed_fit_lands_within_1e3_of_fp64(case):
         p.set_precision(egs.PRECISION_MIXED)
         print("smooth-field mixed fit n=%d" % len(x), flush=True)
         th, llh, nev = p.data_modeling(f["lb"], f["ub"], tolerance=1e-11, max_iterations=3000)
-        _, fallbacks = p.fallback()
-        it = p.refinement()["iterations"]
+        _, fallbacks, _ = p.fallback()
         p.close()
-    print("mixed theta-hat %s llh %.10f, %d evaluations, %d fell back to FP64, last refinement %d iterations"
-          % (list(th), llh, nev, fallbacks, it), flush=True)
+    print("mixed + FP64 polish: theta-hat %s llh %.10f, %d evaluations, %d fell back to FP64"
+          % (list(th), llh, nev, fallbacks), flush=True)
     assert np.max(np.abs(np.array(th) - f["theta_hat"])) < 1e-3
+    assert llh >= f["loglik"] - 1e-6
 
 
-def test_fallback_repeats_a_poorly_refined_evaluation_in_fp64(case):
-    """A fallback threshold below the refinement's iteration count forces the
-    FP64 repeat: the value then equals the FP64 plan's."""
+def test_fallback_repeats_a_poor_mixed_evaluation_in_fp64(case):
+    """Without the guard the mixed value stands and its error estimate is
+    reported (positive: the refinement changed the quadratic form); with a
+    tolerance below that estimate the evaluation is repeated under the FP64
+    plan and equals it bit for bit.  The estimate against the actual error is
+    printed (measured: 3 to 100 times the actual relative error, once 9 times
+    under it)."""
     f, (x, y), zo, _, _ = case
-    th = f["theta_true"]
     with egs.Context(0) as ctx:
         p = egs.Problem(ctx, 0, x, y, dts=192)
         p.set_observations(zo)
-        want = p.loglik(th)
         p.set_precision(egs.PRECISION_MIXED)
-        p.set_fallback(0)
-        mixed = p.loglik(th)
-        iters = p.refinement()["iterations"]
-        assert p.fallback() == (False, 0)
-        if iters >= 1:
-            p.set_fallback(iters - 1 if iters > 1 else 1)
-            got = p.loglik(th)
-            fell, total = p.fallback()
-            if iters > 1:
-                assert fell and total == 1
-                assert got == want
+        p.set_fallback(0, 0.0)
+        rows = []
+        for th in ([1.0, 0.1, 1.0], [2.0, 0.14, 1.0], [0.5, 0.05, 2.0]):
+            mixed = p.loglik(th)[0]
+            fell, total, est = p.fallback()
+            assert not fell and total == 0
+            rows.append((th, mixed, est))
+        want = []
+        p.set_precision(egs.PRECISION_FP64)
+        for th, _, _ in rows:
+            want.append(p.loglik(th)[0])
+        p.set_precision(egs.PRECISION_MIXED)
+        for (th, mixed, est), w in zip(rows, want):
+            err = abs(mixed - w) / abs(w)
+            print("theta %s: mixed %.10f fp64 %.10f rel err %.2e, estimate %.2e" % (th, mixed, w, err, est), flush=True)
+            assert est > 0.0
+            p.set_fallback(0, 0.5 * est)
+            got = p.loglik(th)[0]
+            fell, _, _ = p.fallback()
+            assert fell and got == w
         p.close()
-    assert abs(mixed[0] - want[0]) <= 1e-5 * abs(want[0])

@@ -39,6 +39,7 @@ def test_mixed_loglik_within_1e6(ctx, data4k):
     p = egs.Problem(ctx, 0, x, y, dts=192)
     p.set_observations(z)
     p.set_precision(egs.PRECISION_MIXED, 2, 10)
+    p.set_fallback(0, 0.0)   # the mixed arithmetic itself, no FP64 repeat
     llh, ld, q = p.loglik(th)
     rf = p.refinement()
     rel = abs(llh - ref["llh"]) / abs(ref["llh"])
@@ -63,6 +64,7 @@ def test_mixed_plan_is_equivariant_under_power_of_two_variance(ctx, data4k, log2
     p = egs.Problem(ctx, 0, x, y, dts=192)
     p.set_observations(z)
     p.set_precision(egs.PRECISION_MIXED, 1, 3)
+    p.set_fallback(0, 0.0)
     _, ld1, q1 = p.loglik(th)
     _, ldc, qc = p.loglik([th[0] * c, th[1], th[2]])
     log("  2^%d: logdet shift %.12f (n ln c %.12f), quad ratio %.15f" % (log2c, ldc - ld1, n * np.log(c), qc * c / q1))
@@ -76,6 +78,7 @@ def test_mixed_plain_bf16_refines_quad(ctx, data4k):
     p = egs.Problem(ctx, 0, x, y, dts=192)
     p.set_observations(z)
     p.set_precision(egs.PRECISION_MIXED_BF16, 2, 10)
+    p.set_fallback(0, 0.0)
     llh, ld, q = p.loglik(th)
     rf = p.refinement()
     log("  llh %.10f oracle %.10f rel %.2e refine %s" % (llh, ref["llh"], abs(llh - ref["llh"]) / abs(ref["llh"]), rf))
@@ -95,6 +98,7 @@ def test_mixed_fp64_only_band_equals_fp64(ctx):
     p = egs.Problem(ctx, 0, x, y, dts=192)
     p.set_observations(z)
     p.set_precision(egs.PRECISION_MIXED, 100, 100)
+    p.set_fallback(0, 0.0)
     p.set_fp64_product_band(None)
     llh, _, _ = p.loglik(th)
     assert abs(llh - ref["llh"]) <= 1e-10 * abs(ref["llh"])
@@ -169,6 +173,7 @@ def test_single_product_selection_pins_both_ends(ctx, data4k):
     p = egs.Problem(ctx, 0, x, y, dts=192)
     p.set_observations(z)
     p.set_precision(egs.PRECISION_MIXED, 1, 3)
+    p.set_fallback(0, 0.0)
     p.set_single_product(-1)
     split_only = p.loglik(th)
     assert p.product_counts() == {"split": 0, "single": 0}
@@ -205,10 +210,14 @@ def test_default_mixed_plan_at_100k_within_1e6_of_fp64(ctx):
     p.set_precision(egs.PRECISION_MIXED)
     log("n=100000: mixed evaluations")
     lm, dm, qm = p.loglik(th)
+    fell, _, est = p.fallback()
     rel = abs(lm - l64) / abs(l64)
-    log("  fp64 %.10f mixed %.10f rel %.2e quad rel %.2e" % (l64, lm, rel, abs(qm - q64) / q64))
+    log("  fp64 %.10f mixed %.10f rel %.2e quad rel %.2e, error estimate %.2e" % (l64, lm, rel, abs(qm - q64) / q64,
+                                                                                 est))
+    assert not fell   # the default plan's own result, not the FP64 repeat
     assert rel <= 1e-6
     assert abs(qm - q64) <= 1e-10 * q64
+    p.set_fallback(0, 0.0)
     _, dc, qc = p.loglik([th[0] * 2.0 ** -10, th[1], th[2]])
     assert abs((dc - dm) - 100000 * np.log(2.0 ** -10)) <= 1e-10 * abs(100000 * np.log(2.0 ** -10))
     assert abs(qc * 2.0 ** -10 - qm) <= 1e-12 * qm

@@ -369,10 +369,14 @@ def test_univariate_predict_mspe_leaves_per_variable_errors_zero(ctx):
 def test_covariance_tile_interior_offsets_match_oracle(ctx, kernel, theta):
     """A tile away from the origin of both location sets (the plugin stages only
     the locations it touches): equal to the oracle's tile at 1e-13 relative,
-    including a last tile whose row count is odd for the bivariate kernel."""
+    including ragged tiles.  Bivariate tiles have even row and column counts,
+    as every tile of the reference's even dense tile size does (for an odd row
+    count its loop writes row i+1 of the last pair into the next column,
+    BivariateMaternParsimonious.cpp:80-95)."""
     x, y = orc.generate_locations(900, seed=6)
     xs, ys = x[::-1].copy(), y[::-1].copy()
-    for rows, cols, r0, c0 in ((96, 64, 320, 640), (17, 33, 1000 if kernel else 500, 12), (5, 3, 0, 0)):
+    ragged = ((18, 34, 1000, 12), (6, 2, 0, 0)) if kernel else ((17, 33, 500, 12), (5, 3, 0, 0))
+    for rows, cols, r0, c0 in ((96, 64, 320, 640),) + ragged:
         got = ctx.covariance_tile(kernel, rows, cols, r0, c0, (x, y), (xs, ys), theta)
         ref = orc.covariance_tile(kernel, rows, cols, r0, c0, (x, y), (xs, ys), theta)
         assert np.max(np.abs(got - ref)) <= 1e-13 * np.max(np.abs(ref))

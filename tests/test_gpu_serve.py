@@ -55,17 +55,26 @@ def _requests(regions, count, seed):
 
 
 def _alone(regions, reqs, mode=egs.PRECISION_FP64, dts=DTS):
-    """Each request on a fresh problem of its region, one at a time."""
+    """Each request on a fresh problem of its region, one at a time, with the
+    server's plan policy: a mixed region whose tiles all lie in the FP64 band
+    (4) runs FP64, and a mixed region keeps FP64 after an evaluation of it fell
+    back to FP64."""
     out = []
+    switched = set()
+    ts = (dts + 63) // 64 * 64
     with egs.Context(0) as ctx:
         for kind, r, th, xm, ym in reqs:
             x, y, z, _ = regions[r]
             p = egs.Problem(ctx, 0, x, y, dts=dts)
             p.set_observations(z)
-            if mode != egs.PRECISION_FP64:
+            nt = (len(x) + ts - 1) // ts
+            mixed = mode != egs.PRECISION_FP64 and nt - 1 > 4 and r not in switched
+            if mixed:
                 p.set_precision(mode, 4, 4)
             if kind == "loglik":
                 out.append(p.loglik(th)[0])
+                if mixed and p.fallback()[0]:
+                    switched.add(r)
             else:
                 out.append(p.predict(th, xm, ym)[0])
             p.close()
