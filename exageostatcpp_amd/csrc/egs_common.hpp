@@ -49,6 +49,21 @@ __device__ __forceinline__ int xcd_remap(int w, int n) {
     return x * q + min(x, r) + i;
 }
 
+// The same idea for blocks of uneven cost: groups of g consecutive logical
+// blocks (one tile's blocks) are dealt to the XCDs in turn, XCD x taking
+// groups x, x + 8, x + 16, ...  A group's blocks meet in one L2, and every
+// XCD gets a share of each part of the order, so a run of expensive tiles
+// does not all land on one XCD (with xcd_remap's contiguous ranges the last
+// XCD took every tile at the end of the list).  Workgroups past the last full
+// round of 8 g keep their index.  A bijection of [0, n).
+__device__ __forceinline__ int xcd_deal(int w, int n, int g) {
+    constexpr int kXcd = 8;
+    const int full = n / (kXcd * g) * (kXcd * g);
+    if (w >= full) return w;
+    const int x = w % kXcd, i = w / kXcd;
+    return (i / g) * (kXcd * g) + x * g + i % g;
+}
+
 typedef double f64x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 

@@ -319,7 +319,10 @@ __global__ void __launch_bounds__(kNTw, kMode == kModeSplit ? 1 : 2) update192_w
     const int ts = map.ts;
     const int nsb = ts / kBM;
     const int nsub = nsb * nsb;
-    const int bid = xcd_remap((int) blockIdx.x, (int) gridDim.x);
+    // per-block choice: split blocks (about 2.3x the cost of single ones)
+    // gather where the list ends (the columns next to the panel), so whole
+    // tiles are dealt to the XCDs in turn
+    const int bid = xcd_deal((int) blockIdx.x, (int) gridDim.x, nsub);
     const int pair = bid / nsub;
     const int sub = bid % nsub;
     const int sbm = sub % nsb, sbn = sub / nsb;
