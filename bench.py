@@ -156,6 +156,63 @@ def measured_traffic(kernel, args):
     return t["fetch_bytes_per_launch"] + t["write_bytes_per_launch"], t.get("algorithmic_bytes_per_launch")
 
 
+def _counter_mean_kib(path, name_regex):
+    """Mean Counter_Value (KiB) per dispatch of the kernels matching
+    name_regex in a rocprofv3 --pmc CSV, and the number of dispatches."""
+    import csv
+    import re
+
+    per = {}
+    with open(path) as fh:
+        for r in csv.DictReader(fh):
+            if re.search(name_regex, r["Kernel_Name"]):
+                per[r["Dispatch_Id"]] = per.get(r["Dispatch_Id"], 0.0) + float(r["Counter_Value"])
+    if not per:
+        raise SystemExit("no dispatch of %s in %s" % (name_regex, path))
+    return sum(per.values()) / len(per), len(per)
+
+
+def traffic_summary(args):
+    """Writes profiles/update_16_select_n<N>k_traffic.json from four rocprofv3
+    --pmc runs under DIR (f: FETCH_SIZE and w: WRITE_SIZE of the bench
+    workload, cf and cw: the same counters of --copy-probe's 1 GiB copy), each
+    in DIR/<x>/<x>_counter_collection.csv.  The copy calibrates the counters
+    to bytes; the algorithmic bytes per launch are the per-block class's tile
+    reads and writes (one FP32 tile read and written per tile update) over the
+    schedule's launches of that class."""
+    d, rx = args.traffic_summary, args.traffic_kernel
+    f_kib, nf = _counter_mean_kib(os.path.join(d, "f", "f_counter_collection.csv"), rx)
+    w_kib, nw = _counter_mean_kib(os.path.join(d, "w", "w_counter_collection.csv"), rx)
+    cf_kib, _ = _counter_mean_kib(os.path.join(d, "cf", "cf_counter_collection.csv"), "copy")
+    cw_kib, _ = _counter_mean_kib(os.path.join(d, "cw", "cw_counter_collection.csv"), "copy")
+    gib = float(2 ** 30)
+    cal_f, cal_w = gib / (cf_kib * 1024.0), gib / (cw_kib * 1024.0)
+    # the per-block class: tiles with i - j > max(band_fp32, single_band);
+    # launches per step: column k+1 (critical), column k+2, columns k+3.. (bulk)
+    nt = -(-args.n // args.dts)
+    beyond = max(args.band_fp32, args.single_band)
+    launches, updates = 0, 0
+    for k in range(nt - 1):
+        for lo, hi in ((k + 1, k + 2), (k + 2, k + 3), (k + 3, nt)):
+            cnt = sum(max(0, nt - j - beyond - 1) for j in range(lo, min(hi, nt)))
+            if cnt:
+                launches += 1
+                updates += cnt
+    alg = updates * 2.0 * 4.0 * args.dts * args.dts / launches
+    out = {"kernel": "update_16_select", "kernel_name_regex": rx, "n": args.n, "dts": args.dts,
+           "band_fp64": args.band_fp64, "band_fp32": args.band_fp32, "single_band": args.single_band,
+           "single_tau": tau_json(args), "fp64_product_band": product_band(args, bivariate=False),
+           "operands": operands(args),
+           "fetch_bytes_per_launch": f_kib * 1024.0 * cal_f, "write_bytes_per_launch": w_kib * 1024.0 * cal_w,
+           "algorithmic_bytes_per_launch": alg, "launches_per_factorization": launches,
+           "tile_updates_per_factorization": updates, "calibration": {"fetch": cal_f, "write": cal_w},
+           "raw_kib_per_dispatch": {"fetch": f_kib, "write": w_kib}, "dispatches": {"fetch": nf, "write": nw}}
+    path = os.path.join(ROOT, "profiles", "update_16_select_n%dk_traffic.json" % (args.n // 1000))
+    with open(path, "w") as fh:
+        json.dump(out, fh, indent=1)
+    print(json.dumps(out))
+
+
 def timed(egs, ctx, prob, theta, steps, warmup, label):
     """Warm-up, then `steps` timed evaluations without per-kernel timing, then
     one more evaluation with HIP-event timing of every launch (kernel table and
@@ -365,18 +422,26 @@ def bivariate_gcd(egs, args):
         prob.set_single_product(args.single_band, args.single_tau)
         if args.fp64_product_band is not None:
             prob.set_fp64_product_band(args.fp64_product_band)
+        # the plan's own arithmetic (guard off), then one evaluation under the
+        # default guard: its error estimate and whether it repeats in FP64
+        prob.set_fallback(0, 0.0)
         try:
             mixed = timed(egs, ctx, prob, theta, args.steps, args.warmup, "bivariate mixed (%d, %d)" % (b64, b32))
         except egs.NotPositiveDefinite as e:
             log("bivariate mixed (%d, %d): %s" % (b64, b32, e))
             out["mixed"].append({"plan": [b64, b32], "error": str(e)})
+            prob.set_fallback()
             continue
+        prob.set_fallback()
+        prob.loglik(theta)
+        fell, _, est = prob.fallback()
         out["mixed"].append({"plan": [b64, b32], "evals_per_s": mixed["evals_per_s"],
                              "ms_per_step": mixed["ms_per_step"], "loglik": mixed["loglik"],
                              "rel_llh_err_vs_fp64": abs(mixed["loglik"] - fp64["loglik"]) / abs(fp64["loglik"]),
                              "refinement": mixed["refinement"],
-        "block_updates_beyond_single_band": mixed["product_counts"],
-                             "speedup_vs_fp64": mixed["evals_per_s"] / fp64["evals_per_s"]})
+                             "block_updates_beyond_single_band": mixed["product_counts"],
+                             "speedup_vs_fp64": mixed["evals_per_s"] / fp64["evals_per_s"],
+                             "guard": {"error_estimate": est, "repeats_in_fp64": bool(fell)}})
     prob.close()
     ctx.close()
     print(json.dumps(out))
@@ -441,8 +506,10 @@ def mle_compare(egs, args):
     prob.generate_observations(theta, seed=0)
     out = {"workload": "maximum-likelihood fit, n=%d, theta_true=%s, dts=%d, bounds %s..%s, start theta_true, "
                        "ftol %g, rho_end %g" % (n, theta, args.dts, lb, ub, args.mle_ftol, args.mle_rho_end)}
+    out["z_sum"] = float(sum(prob.get_observations()))   # identical data across separate runs
     fp64_prob = None
-    for label, mode in (("fp64", None), ("mixed", egs.PRECISION_MIXED)):
+    plans = [pl for pl in (("fp64", None), ("mixed", egs.PRECISION_MIXED)) if args.mle_plans in ("both", pl[0])]
+    for label, mode in plans:
         if mode is not None:
             fp64_prob = egs.Problem(ctx, "UnivariateMaternStationary", x, y, dts=args.dts)
             fp64_prob.set_observations(prob.get_observations())
@@ -456,6 +523,8 @@ def mle_compare(egs, args):
             count[0] += 1
             try:
                 llh = prob.loglik(list(v))[0]
+                if mode is not None and prob.fallback()[0]:
+                    fallback[0] += 1   # the library's guard repeated it under the FP64 plan
             except egs.NotPositiveDefinite:
                 fallback[0] += 1
                 llh = fp64_prob.loglik(list(v))[0]
@@ -470,13 +539,15 @@ def mle_compare(egs, args):
         out[label] = {"theta_hat": list(th), "loglik": -fmin, "evaluations": nev, "fp64_fallbacks": fallback[0],
                       "seconds": dt}
         log("%s: theta %s llh %.10f, %d evaluations, %.1f s" % (label, list(th), -fmin, nev, dt))
-    d = [abs(a - b) for a, b in zip(out["fp64"]["theta_hat"], out["mixed"]["theta_hat"])]
-    out["max_abs_theta_diff"] = max(d)
-    out["fp64_loglik_at_mixed_theta_hat"] = fp64_prob.loglik(out["mixed"]["theta_hat"])[0]
-    out["mixed_plan"] = {"band_fp64": args.band_fp64, "band_fp32": args.band_fp32, "single_band": args.single_band,
-                         "single_tau": tau_json(args),
-                         "fp64_product_band": product_band(args, bivariate=False)}
-    fp64_prob.close()
+    if "fp64" in out and "mixed" in out:
+        d = [abs(a - b) for a, b in zip(out["fp64"]["theta_hat"], out["mixed"]["theta_hat"])]
+        out["max_abs_theta_diff"] = max(d)
+    if "mixed" in out:
+        out["fp64_loglik_at_mixed_theta_hat"] = fp64_prob.loglik(out["mixed"]["theta_hat"])[0]
+        out["mixed_plan"] = {"band_fp64": args.band_fp64, "band_fp32": args.band_fp32,
+                             "single_band": args.single_band, "single_tau": tau_json(args),
+                             "fp64_product_band": product_band(args, bivariate=False)}
+        fp64_prob.close()
     prob.close()
     ctx.close()
     print(json.dumps(out))
@@ -504,7 +575,7 @@ def main():
     ap.add_argument("--plain-bf16", action="store_true", help="single BF16 product instead of the split one")
     ap.add_argument("--cpu-sample", type=int, default=20000,
                     help="n of the host sample (each evaluation a few seconds on 16 host cores)")
-    ap.add_argument("--cpu-reps", type=int, default=3, help="timed host evaluations (median reported)")
+    ap.add_argument("--cpu-reps", type=int, default=5, help="timed host evaluations (median reported)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--mle", action="store_true",
                     help="fit theta with the FP64 and the mixed plan on the same data and report both estimates "
@@ -521,6 +592,8 @@ def main():
                          "held-out points")
     ap.add_argument("--c2-iterations", type=int, default=100)
     ap.add_argument("--mle-n", type=int, default=40000, help="n of the --mle fits")
+    ap.add_argument("--mle-plans", choices=("both", "fp64", "mixed"), default="both",
+                    help="which --mle fits to run (separate runs see identical data: z_sum in the output)")
     ap.add_argument("--theta", type=float, nargs=3, default=None)
     ap.add_argument("--maxeval", type=int, default=500)
     ap.add_argument("--no-serve", action="store_true", help="skip the serving phase")
@@ -531,7 +604,14 @@ def main():
     ap.add_argument("--serve-only", action="store_true", help="only the serving phase")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a 1 GiB device-to-device copy (calibrates FETCH_SIZE/WRITE_SIZE counters)")
+    ap.add_argument("--traffic-summary", metavar="DIR", default=None,
+                    help="write the dominant kernel's HBM traffic summary under profiles/ from rocprofv3 FETCH_SIZE "
+                         "and WRITE_SIZE runs in DIR/{f,w,cf,cw} (no GPU needed)")
+    ap.add_argument("--traffic-kernel", default="update192_w_kernelIDF16_Li2E",
+                    help="kernel-name regex of --traffic-summary")
     args = ap.parse_args()
+    if args.traffic_summary:
+        return traffic_summary(args)
     if args.copy_probe:
         import torch
 

@@ -104,7 +104,7 @@ struct Profiler {
 };
 
 // Update classes of a mixed plan (see tile_class).
-constexpr int kClasses = 5;
+constexpr int kClasses = 6;
 // Panel copies in flight: panel k is read by the bulk update of step k while
 // the critical stream writes panel k+1 and then panel k+2.
 constexpr int kPanelSlots = 3;
@@ -126,6 +126,11 @@ struct egs_problem_s {
     // bivariate kernel, whose smoother second field is more sensitive (band 1
     // measured 1.2e-6 relative there under plan (2, 2), band 0 2.5e-6)
     int band_prod64 = 0;
+    // The per-block choice also applies to the FP64 tiles that take split
+    // products (class 5).  Set at creation: on for univariate kernels (at
+    // n = 100k the error stays 2.3e-7 and the evaluation is 4.5% faster), off
+    // for the bivariate kernel (plan (8, 8): 1.2e-6 with it, 2.2e-7 without).
+    bool single_fp64 = true;
     double tau_single = 0x1p-16;
     // mixed plans: per-class update lists (descending column) and, for class c,
     // cls_count[c][j] = number of its tiles in columns >= j
@@ -438,10 +443,16 @@ uint8_t tile_store(const egs_problem_s *p, int i, int j) {
 
 // Update class of tile (i, j) under a mixed plan: 0 FP64, 1 FP32, 2 16-bit
 // products (split FP16, or one BF16 product under MIXED_BF16), 3 one FP16
-// product (beyond band_single under MIXED, per block when tau_single is finite).
+// product (beyond band_single under MIXED, per block when tau_single is
+// finite), 4 split FP16 products into an FP64 tile (the FP64 band beyond the
+// FP64-product band), 5 the same with one product per block where the panel
+// rows are small (class 4 tiles beyond band_single, univariate kernels).
 int tile_class(const egs_problem_s *p, int i, int j) {
     const int d = i - j;
-    if (d <= p->band64) return (p->mode == EGS_PRECISION_MIXED && d > p->band_prod64) ? 4 : 0;
+    if (d <= p->band64) {
+        if (p->mode != EGS_PRECISION_MIXED || d <= p->band_prod64) return 0;
+        return (p->single_fp64 && p->band_single >= 0 && d > p->band_single) ? 5 : 4;
+    }
     if (d <= p->band32) return 1;
     if (p->mode == EGS_PRECISION_MIXED && p->band_single >= 0 && d > p->band_single) return 3;
     return 2;
@@ -634,7 +645,7 @@ constexpr int kShortUpdate = 8;
 void launch_mixed_update(egs_problem_s *p, int k, int jcol_lo, int jcol_hi, hipStream_t st, bool critical = false) {
     const egs::TileMap m = p->map();
     static const int prof_id[kClasses] = {EGS_PROFILE_UPDATE, EGS_PROFILE_UPDATE_F32, EGS_PROFILE_UPDATE_16,
-                                          EGS_PROFILE_UPDATE_16_SELECT, EGS_PROFILE_UPDATE_16};
+                                          EGS_PROFILE_UPDATE_16_SELECT, EGS_PROFILE_UPDATE_16, EGS_PROFILE_UPDATE_16};
     for (int c = 0; c < kClasses; ++c) {
         // class-c tiles with column in [jcol_lo, jcol_hi): list entries [count[jcol_hi], count[jcol_lo])
         const int beg = p->cls_count[c][jcol_hi];
@@ -648,10 +659,12 @@ void launch_mixed_update(egs_problem_s *p, int k, int jcol_lo, int jcol_hi, hipS
         else if (c == 0) egs::launch_update192_panel(m, p->d_p64[slot], pr, end - beg, k, st);
         else if (c == 1) egs::launch_update192_f32(m, p->d_p32[slot], pr, end - beg, k, st);
         else {
+            const bool all_single = std::isinf(p->tau_single);
             const egs::Operands op =
-                c == 4 ? egs::kSplitF16C64
-                       : c == 3 ? (std::isinf(p->tau_single) ? egs::kSingleF16 : egs::kAdaptiveF16)
-                                : (p->mode == EGS_PRECISION_MIXED ? egs::kSplitF16 : egs::kSingleBF16);
+                c == 5 ? (all_single ? egs::kSingleF16C64 : egs::kAdaptiveF16C64)
+                : c == 4 ? egs::kSplitF16C64
+                : c == 3 ? (all_single ? egs::kSingleF16 : egs::kAdaptiveF16)
+                         : (p->mode == EGS_PRECISION_MIXED ? egs::kSplitF16 : egs::kSingleBF16);
             egs::launch_update192_16(m, p->d_p16h[slot], p->d_p16l[slot], p->d_pinv[slot], p->d_bmax[slot],
                                      p->d_pmax, (float) p->tau_single, pr, end - beg, k, op, p->d_counts, st);
         }
@@ -991,6 +1004,7 @@ int covariance_tile_on(egs_context_t ctx, const TileRequest &t, double *A, const
 void init_shape(egs_problem_s *p, int kernel, int64_t n_loc, int dim, int dts) {
     p->kernel = kernel;
     p->band_prod64 = kernel == EGS_KERNEL_BIVARIATE_MATERN_PARSIMONIOUS ? 2 : 0;
+    p->single_fp64 = kernel != EGS_KERNEL_BIVARIATE_MATERN_PARSIMONIOUS;
     p->p = kernel == EGS_KERNEL_BIVARIATE_MATERN_PARSIMONIOUS ? 2 : 1;
     p->dim = dim;
     p->dts = dts;

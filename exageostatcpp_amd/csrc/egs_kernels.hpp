@@ -133,8 +133,12 @@ void launch_update192_f32(const TileMap &map, const float *p32, const int2 *pair
 // between the two per 192 x 192 block (one product when bmax(A rows) *
 // bmax(B rows) <= tau * pmax[k]^2), or one BF16 product of unscaled values.
 // FP16 forms undo the row scales with inv_scale.
-// kSplitF16C64 is the split product into an FP64 tile.
-enum Operands { kSplitF16 = 0, kSingleF16 = 1, kAdaptiveF16 = 2, kSingleBF16 = 3, kSplitF16C64 = 4 };
+// The C64 forms update an FP64 tile: the split product, one product, or the
+// choice per block.
+enum Operands {
+    kSplitF16 = 0, kSingleF16 = 1, kAdaptiveF16 = 2, kSingleBF16 = 3, kSplitF16C64 = 4, kSingleF16C64 = 5,
+    kAdaptiveF16C64 = 6
+};
 // C -= A B^T from the 16-bit parts of panel k.  The single and per-block forms
 // add the number of 192 x 192 blocks they update with the split product to
 // counts[0] and with one product to counts[1].

@@ -591,6 +591,14 @@ void launch_update192_16(const TileMap &map, const uint16_t *p16h, const uint16_
         hipLaunchKernelGGL((update192_w_kernel<_Float16, kModeSplit, 2, true>), grid, dim3(kNTw), 0, s, map, p16h,
                            p16l, inv_scale, bmax, pmax, tau, pairs, k, nullptr);
         break;
+    case kSingleF16C64:
+        hipLaunchKernelGGL((update192_w_kernel<_Float16, kModeSingle, 2, true>), grid, dim3(kNTw), 0, s, map, p16h,
+                           p16l, inv_scale, bmax, pmax, tau, pairs, k, counts);
+        break;
+    case kAdaptiveF16C64:
+        hipLaunchKernelGGL((update192_w_kernel<_Float16, kModeAdaptive, 2, true>), grid, dim3(kNTw), 0, s, map, p16h,
+                           p16l, inv_scale, bmax, pmax, tau, pairs, k, counts);
+        break;
     default:
         hipLaunchKernelGGL((update192_w_kernel<__bf16, kModeSingle, 2>), grid, dim3(kNTw), 0, s, map, p16h, p16l,
                            nullptr, nullptr, nullptr, 0.0f, pairs, k, nullptr);

@@ -131,9 +131,11 @@ int egs_memory_cache(int64_t limit_bytes, int64_t *cached_bytes, int64_t *live_b
  * with residuals b - Sigma x evaluated in FP64 until ||r|| <= 1e-12 ||b||. */
 int egs_problem_set_precision(egs_problem_t problem, int mode, int band_fp64, int band_fp32);
 /* Under EGS_PRECISION_MIXED, a 192 x 192 block of a tile with tile distance
- * |i - j| > band (and > band_fp32) is updated with one FP16 product of the
- * row-scaled values instead of the split three (about 11 significant bits per
- * operand instead of 22) when the panel rows it reads are small:
+ * |i - j| > band that would take the split FP16 product (beyond band_fp32;
+ * for the univariate kernel also the FP64 tiles beyond the FP64-product band,
+ * see egs_problem_set_fp64_product_band) is updated with one FP16 product of
+ * the row-scaled values instead of the split three (about 11 significant bits
+ * per operand instead of 22) when the panel rows it reads are small:
  * smax(its A rows) * smax(its B rows) <= tau * smax(panel)^2, where smax is the
  * largest power-of-two row scale (2^(e+1) for a largest entry f 2^e).
  * tau = INFINITY takes every such block, tau = 0 none but all-zero ones;
