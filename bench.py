@@ -607,7 +607,7 @@ def main():
     ap.add_argument("--traffic-summary", metavar="DIR", default=None,
                     help="write the dominant kernel's HBM traffic summary under profiles/ from rocprofv3 FETCH_SIZE "
                          "and WRITE_SIZE runs in DIR/{f,w,cf,cw} (no GPU needed)")
-    ap.add_argument("--traffic-kernel", default="update192_w_kernelIDF16_Li2E",
+    ap.add_argument("--traffic-kernel", default="update192_w_kernelIDF16_Li2ELi2ELb0",
                     help="kernel-name regex of --traffic-summary")
     args = ap.parse_args()
     if args.traffic_summary:

@@ -285,7 +285,9 @@ __global__ void __launch_bounds__(kNT, 1) update192_f32_kernel(TileMap map, cons
 constexpr int kNTw = 256;
 constexpr int kKCw = 32;                  // K per chunk: two MFMA steps
 constexpr int kPlanesW = kKCw / 8;        // 16-byte vectors per row per chunk
-constexpr int kPlaneW = kBM + 4;          // rows per LDS plane, padded
+// rows per LDS plane, padded: at 194 x 16 B the four planes of a staged row
+// start 32 B apart in the banks, so eight lanes' 16-byte writes do not collide
+constexpr int kPlaneW = kBM + 2;
 constexpr int kVecsW = kBM * kPlanesW;    // 768 vectors per operand part per chunk
 constexpr int kPerThreadW = kVecsW / kNTw;   // 3
 
@@ -327,6 +329,7 @@ __global__ void __launch_bounds__(kNTw, kMode == kModeSplit ? 1 : 2) update192_w
     const int sub = bid % nsub;
     const int sbm = sub % nsb, sbn = sub / nsb;
     const int2 ij = pairs[pair];
+    if (ij.x == ij.y && sbm < sbn) return;   // a diagonal tile's upper blocks (uniform over the workgroup)
     const int64_t tsz = (int64_t) ts * ts;
     const int64_t aoff = (int64_t) (ij.x - k - 1) * tsz + (int64_t) sbm * kBM * ts;
     const int64_t boff = (int64_t) (ij.y - k - 1) * tsz + (int64_t) sbn * kBM * ts;

@@ -147,7 +147,9 @@ int egs_problem_set_precision(egs_problem_t problem, int mode, int band_fp64, in
 int egs_problem_set_single_product(egs_problem_t problem, int band, double tau);
 /* Under EGS_PRECISION_MIXED, tiles of the FP64 band (|i - j| <= band_fp64)
  * with |i - j| > band keep FP64 storage but are updated with split FP16
- * products accumulated in FP32 and added in FP64.  Default 0 for univariate
+ * products accumulated in FP32 and added in FP64 (for univariate kernels one
+ * product per block where egs_problem_set_single_product's test passes).
+ * Default 0 for univariate
  * kernels (only the diagonal tiles take FP64 products), 2 for the bivariate
  * kernel.  Negative: every tile of the FP64 band does.  Kept across
  * egs_problem_set_precision calls. */
