@@ -207,7 +207,7 @@ def traffic_summary(args):
            "algorithmic_bytes_per_launch": alg, "launches_per_factorization": launches,
            "tile_updates_per_factorization": updates, "calibration": {"fetch": cal_f, "write": cal_w},
            "raw_kib_per_dispatch": {"fetch": f_kib, "write": w_kib}, "dispatches": {"fetch": nf, "write": nw}}
-    path = os.path.join(ROOT, "profiles", "update_16_select_n%dk_traffic.json" % (args.n // 1000))
+    path = args.traffic_out or os.path.join(ROOT, "profiles", "update_16_select_n%dk_traffic.json" % (args.n // 1000))
     with open(path, "w") as fh:
         json.dump(out, fh, indent=1)
     print(json.dumps(out))
@@ -609,6 +609,7 @@ def main():
                          "and WRITE_SIZE runs in DIR/{f,w,cf,cw} (no GPU needed)")
     ap.add_argument("--traffic-kernel", default="update192_w_kernelIDF16_Li2ELi2ELb0",
                     help="kernel-name regex of --traffic-summary")
+    ap.add_argument("--traffic-out", default=None, help="output file of --traffic-summary (default: under profiles/)")
     args = ap.parse_args()
     if args.traffic_summary:
         return traffic_summary(args)
