@@ -733,8 +733,8 @@ def main():
             "frac_mfma_issued": achieved * mfma_per_flop / PEAK[prec],
             "note": "achieved and frac count algorithmic flops (2 ts^3 per tile update) over the launch time; "
                     "frac_mfma_issued weights them by the MFMAs each product takes (3 per split product, 1 per "
-                    "single one, from the block counts); algorithmic_bytes is each updated tile read and written "
-                    "once",
+                    "single one, from the block counts of every per-block launch, FP32 tiles and the FP64 band's "
+                    "together); algorithmic_bytes is each updated tile read and written once",
         },
         "fp64": {
             "evals_per_s": fp64["evals_per_s"], "ms_per_step": fp64["ms_per_step"], "steps": fp64_steps,

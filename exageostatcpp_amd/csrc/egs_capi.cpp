@@ -131,6 +131,11 @@ struct egs_problem_s {
     // n = 100k the error stays 2.3e-7 and the evaluation is 4.5% faster), off
     // for the bivariate kernel (plan (8, 8): 1.2e-6 with it, 2.2e-7 without).
     bool single_fp64 = true;
+    // The panel solve of FP32 rows runs on FP32 MFMA (FP32 accumulation)
+    // instead of FP64 MFMA.  Set at creation: on for univariate kernels
+    // (n = 100k: 1021 -> 991 ms, relative error 2.3e-7 -> 2.7e-7), off for
+    // the bivariate kernel (its sweep's errors grew to 1.4-3.0e-6).
+    bool f32_panel_solve = true;
     double tau_single = 0x1p-16;
     // mixed plans: per-class update lists (descending column) and, for class c,
     // cls_count[c][j] = number of its tiles in columns >= j
@@ -607,7 +612,7 @@ void factor_panel(egs_problem_s *p, int k) {
                 egs::launch_trsm_rows(m, k, (int64_t) (k + 1) * nsb, (last64 - k) * nsb, 0, nsb, wk, s, false);
             if (last64 + 1 < nt)
                 egs::launch_trsm_rows(m, k, (int64_t) (last64 + 1) * nsb, (nt - 1 - last64) * nsb, 0, nsb, wk, s,
-                                      true);
+                                      true, p->f32_panel_solve);
         }
         prof_end(p, pi);
         if (is_mixed(p)) {
@@ -1005,6 +1010,7 @@ void init_shape(egs_problem_s *p, int kernel, int64_t n_loc, int dim, int dts) {
     p->kernel = kernel;
     p->band_prod64 = kernel == EGS_KERNEL_BIVARIATE_MATERN_PARSIMONIOUS ? 2 : 0;
     p->single_fp64 = kernel != EGS_KERNEL_BIVARIATE_MATERN_PARSIMONIOUS;
+    p->f32_panel_solve = kernel != EGS_KERNEL_BIVARIATE_MATERN_PARSIMONIOUS;
     p->p = kernel == EGS_KERNEL_BIVARIATE_MATERN_PARSIMONIOUS ? 2 : 1;
     p->dim = dim;
     p->dts = dts;

@@ -181,6 +181,147 @@ __global__ void __launch_bounds__(kThreads) trsm_rows_kernel(TileMap map, int k,
 }
 
 // ------------------------------------------------------------------------
+// The same solve for FP32 panel rows with v_mfma_f32_16x16x4f32 (FP32
+// accumulation; L and W converted to FP32 as they are staged, 50 KB of LDS):
+// 1.8x the rate of the FP64 form, for the univariate kernel's mixed plans
+// (the bivariate field's error grew 3-14x with it).  FP32 result element v of
+// lane l is D[4 (l / 16) + v][l % 16].
+// ------------------------------------------------------------------------
+struct GemmSmem32 {
+    float a[2][kKc][kLdsPad];
+    float b[2][kKc][kLdsPad];
+};
+struct Acc32 {
+    f32x4 v[2][2];
+};
+
+__device__ void wg_gemm_nt_f32(Acc32 &acc, const float *__restrict__ A, int64_t lda, const double *__restrict__ B,
+                               int64_t ldb, int K, GemmSmem32 &sm) {
+    const int t = threadIdx.x;
+    const int lane = t & 63;
+    const int w = t >> 6;
+    const int mq = w & 1, nq = w >> 1;
+    const int lr = lane & 15, lk = lane >> 4;
+    const int srow = t & 63;
+    float ra[kStage], rb[kStage];
+#pragma unroll
+    for (int q = 0; q < kStage; ++q) {
+        const int col = (t >> 6) + q * (kThreads / kBlk);
+        ra[q] = A[srow + (int64_t) col * lda];
+        rb[q] = (float) B[srow + (int64_t) col * ldb];
+    }
+#pragma unroll
+    for (int q = 0; q < kStage; ++q) {
+        const int col = (t >> 6) + q * (kThreads / kBlk);
+        sm.a[0][col][srow] = ra[q];
+        sm.b[0][col][srow] = rb[q];
+    }
+    __syncthreads();
+    const int nch = K / kKc;
+    for (int c = 0; c < nch; ++c) {
+        const int buf = c & 1;
+        const bool more = c + 1 < nch;
+        if (more) {
+            const int64_t k0 = (int64_t) (c + 1) * kKc;
+#pragma unroll
+            for (int q = 0; q < kStage; ++q) {
+                const int col = (t >> 6) + q * (kThreads / kBlk);
+                ra[q] = A[srow + (k0 + col) * lda];
+                rb[q] = (float) B[srow + (k0 + col) * ldb];
+            }
+        }
+#pragma unroll
+        for (int kk = 0; kk < kKc; kk += 4) {
+            const float a0 = sm.a[buf][kk + lk][mq * 32 + lr];
+            const float a1 = sm.a[buf][kk + lk][mq * 32 + 16 + lr];
+            const float b0 = sm.b[buf][kk + lk][nq * 32 + lr];
+            const float b1 = sm.b[buf][kk + lk][nq * 32 + 16 + lr];
+            acc.v[0][0] = mfma_f32(b0, a0, acc.v[0][0]);
+            acc.v[0][1] = mfma_f32(b0, a1, acc.v[0][1]);
+            acc.v[1][0] = mfma_f32(b1, a0, acc.v[1][0]);
+            acc.v[1][1] = mfma_f32(b1, a1, acc.v[1][1]);
+        }
+        if (more) {
+#pragma unroll
+            for (int q = 0; q < kStage; ++q) {
+                const int col = (t >> 6) + q * (kThreads / kBlk);
+                sm.a[buf ^ 1][col][srow] = ra[q];
+                sm.b[buf ^ 1][col][srow] = rb[q];
+            }
+        }
+        __syncthreads();
+    }
+}
+
+template <typename F>
+__device__ __forceinline__ void acc32_foreach(const Acc32 &acc, F f) {
+    const int lane = threadIdx.x & 63;
+    const int w = threadIdx.x >> 6;
+    const int mq = w & 1, nq = w >> 1;
+#pragma unroll
+    for (int ni = 0; ni < 2; ++ni)
+#pragma unroll
+        for (int mi = 0; mi < 2; ++mi)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int r = mq * 32 + mi * 16 + (lane & 15);
+                const int c = nq * 32 + ni * 16 + 4 * (lane >> 4) + v;
+                f(r, c, acc.v[ni][mi][v]);
+            }
+}
+
+__global__ void __launch_bounds__(kThreads) trsm_rows_f32mfma_kernel(TileMap map, int k, int64_t g0, int jb0,
+                                                                     int jb1, const double *winv_k) {
+    __shared__ GemmSmem32 sm;
+    __shared__ float sW[kBlk][kLdsPad];
+    static_assert(sizeof(GemmSmem32::a) >= sizeof(float) * kBlk * kLdsPad, "T must fit in the staging buffer");
+    float (*sT)[kLdsPad] = reinterpret_cast<float (*)[kLdsPad]>(&sm.a[0][0][0]);
+    const int ts = map.ts;
+    const int nsb = ts / kBlk;
+    const int64_t g = g0 + blockIdx.x;
+    const int i = (int) (g / nsb);
+    const int rb = (int) (g % nsb);
+    float *X = reinterpret_cast<float *>(map.base + map.off[i * map.nt + k]) + rb * kBlk;
+    const double *L = tile_f64(map, k, k);
+    const int lane = threadIdx.x & 63;
+    const int w = threadIdx.x >> 6;
+    const int mq = w & 1, nq = w >> 1;
+    const int lr = lane & 15, lk = lane >> 4;
+    for (int jb = jb0; jb < jb1; ++jb) {
+        Acc32 acc;
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int b = 0; b < 2; ++b) acc.v[a][b] = f32x4{0.f, 0.f, 0.f, 0.f};
+        if (jb > 0) wg_gemm_nt_f32(acc, X, ts, L + jb * kBlk, ts, jb * kBlk, sm);
+        float *Xj = X + (int64_t) jb * kBlk * ts;
+        acc32_foreach(acc, [&](int r, int c, float v) { sT[r][c] = Xj[r + (int64_t) c * ts] - v; });
+        // W_jb (64 x 64 lower inverse, ld 64) as FP32 in LDS: sW[c][l] = W[c][l]
+        const double *W = winv_k + (int64_t) jb * kBlk * kBlk;
+        for (int e = threadIdx.x; e < kBlk * kBlk; e += kThreads) sW[e & 63][e >> 6] = (float) W[e];
+        __syncthreads();
+        Acc32 acc2;
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int b = 0; b < 2; ++b) acc2.v[a][b] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll 4
+        for (int kk = 0; kk < kBlk; kk += 4) {
+            const float a0 = sT[mq * 32 + lr][kk + lk];
+            const float a1 = sT[mq * 32 + 16 + lr][kk + lk];
+            const float b0 = sW[nq * 32 + lr][kk + lk];
+            const float b1 = sW[nq * 32 + 16 + lr][kk + lk];
+            acc2.v[0][0] = mfma_f32(b0, a0, acc2.v[0][0]);
+            acc2.v[0][1] = mfma_f32(b0, a1, acc2.v[0][1]);
+            acc2.v[1][0] = mfma_f32(b1, a0, acc2.v[1][0]);
+            acc2.v[1][1] = mfma_f32(b1, a1, acc2.v[1][1]);
+        }
+        acc32_foreach(acc2, [&](int r, int c, float v) { Xj[r + (int64_t) c * ts] = v; });
+        __syncthreads();
+    }
+}
+
+// ------------------------------------------------------------------------
 // Trailing update, FP64.
 // ------------------------------------------------------------------------
 // p64 (may be null): read panel k from its contiguous FP64 copy (tile i at
@@ -214,9 +355,11 @@ __global__ void __launch_bounds__(kThreads) update_kernel(TileMap map, const dou
 }
 
 void launch_trsm_rows(const TileMap &map, int k, int64_t g0, int count, int jb0, int jb1, const double *winv_k,
-                      hipStream_t s, bool x_f32) {
+                      hipStream_t s, bool x_f32, bool f32mfma) {
     if (count <= 0 || jb1 <= jb0) return;
-    if (x_f32)
+    if (x_f32 && f32mfma)
+        hipLaunchKernelGGL(trsm_rows_f32mfma_kernel, dim3(count), dim3(kThreads), 0, s, map, k, g0, jb0, jb1, winv_k);
+    else if (x_f32)
         hipLaunchKernelGGL(trsm_rows_kernel<float>, dim3(count), dim3(kThreads), 0, s, map, k, g0, jb0, jb1, winv_k);
     else
         hipLaunchKernelGGL(trsm_rows_kernel<double>, dim3(count), dim3(kThreads), 0, s, map, k, g0, jb0, jb1, winv_k);

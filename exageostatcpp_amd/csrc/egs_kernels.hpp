@@ -44,7 +44,7 @@ void launch_covgen_rect(double *out, int rows, int cols, int64_t row_off, int64_
 // (global 64-row block g lives in tile g / nsb at row (g % nsb) * 64), column
 // blocks [jb0, jb1).
 void launch_trsm_rows(const TileMap &map, int k, int64_t g0, int count, int jb0, int jb1, const double *winv_k,
-                      hipStream_t s, bool x_f32 = false);
+                      hipStream_t s, bool x_f32 = false, bool f32mfma = false);
 // Trailing update C_ij -= L_ik L_jk^T for the first npairs lower tiles of the
 // update order (all (i, j) with i >= j > k).
 void launch_update(const TileMap &map, const int2 *pairs, int npairs, int k, hipStream_t s);
