@@ -126,7 +126,9 @@ int egs_memory_cache(int64_t limit_bytes, int64_t *cached_bytes, int64_t *live_b
 /* Precision plan: EGS_PRECISION_FP64, or a mixed plan with tiles whose tile
  * distance |i - j| <= band_fp64 in FP64, <= band_fp32 in FP32 and the rest
  * stored in FP32 and updated with 16-bit products accumulated in FP32 (needs a
- * tile size that is a multiple of 192).  Under a mixed plan the factor's diagonal gives log|Sigma| and
+ * tile size that is a multiple of 192).  The panel solve of FP32-stored rows
+ * runs on FP32 MFMA for the univariate kernel and on FP64 MFMA for the
+ * bivariate one.  Under a mixed plan the factor's diagonal gives log|Sigma| and
  * z'Sigma^{-1}z comes from conjugate gradients preconditioned by the factor,
  * with residuals b - Sigma x evaluated in FP64 until ||r|| <= 1e-12 ||b||. */
 int egs_problem_set_precision(egs_problem_t problem, int mode, int band_fp64, int band_fp32);
