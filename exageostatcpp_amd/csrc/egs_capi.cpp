@@ -606,13 +606,18 @@ void factor_panel(egs_problem_s *p, int k) {
         if (!is_mixed(p)) {
             egs::launch_trsm_rows(m, k, (int64_t) (k + 1) * nsb, (nt - k - 1) * nsb, 0, nsb, wk, s);
         } else {
-            // rows within band64 of the diagonal are FP64 tiles, the rest FP32
+            // rows within band64 of the diagonal are FP64 tiles, the rest FP32;
+            // both in one launch when both are solved on FP64 MFMA
             const int last64 = std::min(nt - 1, k + p->band64);
-            if (last64 >= k + 1)
-                egs::launch_trsm_rows(m, k, (int64_t) (k + 1) * nsb, (last64 - k) * nsb, 0, nsb, wk, s, false);
-            if (last64 + 1 < nt)
-                egs::launch_trsm_rows(m, k, (int64_t) (last64 + 1) * nsb, (nt - 1 - last64) * nsb, 0, nsb, wk, s,
-                                      true, p->f32_panel_solve);
+            const int rows32 = std::max(0, nt - 1 - last64);
+            if (!p->f32_panel_solve) {
+                egs::launch_trsm_rows_mixed(m, k, (int64_t) (k + 1) * nsb, (last64 - k) * nsb, rows32 * nsb, wk, s);
+            } else {
+                if (last64 >= k + 1)
+                    egs::launch_trsm_rows(m, k, (int64_t) (k + 1) * nsb, (last64 - k) * nsb, 0, nsb, wk, s, false);
+                egs::launch_trsm_rows(m, k, (int64_t) (last64 + 1) * nsb, rows32 * nsb, 0, nsb, wk, s, true,
+                                      p->f32_panel_solve);
+            }
         }
         prof_end(p, pi);
         if (is_mixed(p)) {

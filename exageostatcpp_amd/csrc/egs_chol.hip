@@ -127,12 +127,11 @@ __device__ __forceinline__ void acc_foreach(const Acc64 &acc, F f) {
 // ------------------------------------------------------------------------
 // Row-block triangular solve against L_kk, column blocks [jb0, jb1).
 // ------------------------------------------------------------------------
+// 66.5 KB of LDS (sm): T is formed after each GEMM through sm (it reuses
+// sm.a), so that a workgroup fits beside one workgroup of the trailing update.
 template <typename TX>
 __device__ __forceinline__ void trsm_rows_body(const TileMap &map, int k, int64_t g0, int jb0, int jb1,
-                                               const double *winv_k, int bx) {
-    // 66.5 KB of LDS: T is formed after each GEMM through sm (it reuses sm.a),
-    // so that a workgroup fits beside one workgroup of the trailing update
-    __shared__ GemmSmem sm;
+                                               const double *winv_k, int bx, GemmSmem &sm) {
     static_assert(sizeof(GemmSmem::a) >= sizeof(double) * kBlk * kLdsPad, "T must fit in the staging buffer");
     double (*sT)[kLdsPad] = reinterpret_cast<double (*)[kLdsPad]>(&sm.a[0][0][0]);
     const int ts = map.ts;
@@ -177,7 +176,8 @@ __device__ __forceinline__ void trsm_rows_body(const TileMap &map, int k, int64_
 template <typename TX>
 __global__ void __launch_bounds__(kThreads) trsm_rows_kernel(TileMap map, int k, int64_t g0, int jb0, int jb1,
                                                              const double *winv_k) {
-    trsm_rows_body<TX>(map, k, g0, jb0, jb1, winv_k, blockIdx.x);
+    __shared__ GemmSmem sm;
+    trsm_rows_body<TX>(map, k, g0, jb0, jb1, winv_k, blockIdx.x, sm);
 }
 
 // ------------------------------------------------------------------------
@@ -321,6 +321,22 @@ __global__ void __launch_bounds__(kThreads) trsm_rows_f32mfma_kernel(TileMap map
     }
 }
 
+// FP64 rows [g0, g0 + count64) and FP32 rows after them in one launch, both
+// on FP64 MFMA, so that the two panel solves of a mixed plan run side by side
+// rather than one after the other (each is a chain of ts / 64 dependent steps
+// per workgroup).  Plans whose FP32 rows are solved on FP32 MFMA keep two
+// launches: that kernel's lighter workgroups (50 KB of LDS) would otherwise
+// carry the FP64 body's LDS and registers.
+__global__ void __launch_bounds__(kThreads) trsm_rows_mixed_kernel(TileMap map, int k, int64_t g0, int count64,
+                                                                   const double *winv_k) {
+    __shared__ GemmSmem sm;
+    const int nsb = map.ts / kBlk;
+    if ((int) blockIdx.x < count64)
+        trsm_rows_body<double>(map, k, g0, 0, nsb, winv_k, blockIdx.x, sm);
+    else
+        trsm_rows_body<float>(map, k, g0, 0, nsb, winv_k, blockIdx.x, sm);
+}
+
 // ------------------------------------------------------------------------
 // Trailing update, FP64.
 // ------------------------------------------------------------------------
@@ -363,6 +379,13 @@ void launch_trsm_rows(const TileMap &map, int k, int64_t g0, int count, int jb0,
         hipLaunchKernelGGL(trsm_rows_kernel<float>, dim3(count), dim3(kThreads), 0, s, map, k, g0, jb0, jb1, winv_k);
     else
         hipLaunchKernelGGL(trsm_rows_kernel<double>, dim3(count), dim3(kThreads), 0, s, map, k, g0, jb0, jb1, winv_k);
+}
+
+void launch_trsm_rows_mixed(const TileMap &map, int k, int64_t g0, int count64, int count32, const double *winv_k,
+                            hipStream_t s) {
+    if (count64 + count32 <= 0) return;
+    hipLaunchKernelGGL(trsm_rows_mixed_kernel, dim3((unsigned) (count64 + count32)), dim3(kThreads), 0, s, map, k,
+                       g0, count64, winv_k);
 }
 
 void launch_update(const TileMap &map, const int2 *pairs, int npairs, int k, hipStream_t s) {
@@ -657,7 +680,8 @@ __global__ void __launch_bounds__(kThreads) trsm_rows_batch_kernel(const BatchJo
     const BatchJob &J = jobs[blockIdx.y];
     const int nsb = J.map.ts / kBlk;
     if ((int) blockIdx.x >= (J.map.nt - J.k - 1) * nsb) return;
-    trsm_rows_body<double>(J.map, J.k, (int64_t) (J.k + 1) * nsb, 0, nsb, J.winv_k, blockIdx.x);
+    __shared__ GemmSmem sm;
+    trsm_rows_body<double>(J.map, J.k, (int64_t) (J.k + 1) * nsb, 0, nsb, J.winv_k, blockIdx.x, sm);
 }
 
 __global__ void __launch_bounds__(kThreads) update_batch_kernel(const BatchJob *jobs) {

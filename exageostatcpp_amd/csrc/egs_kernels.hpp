@@ -45,6 +45,10 @@ void launch_covgen_rect(double *out, int rows, int cols, int64_t row_off, int64_
 // blocks [jb0, jb1).
 void launch_trsm_rows(const TileMap &map, int k, int64_t g0, int count, int jb0, int jb1, const double *winv_k,
                       hipStream_t s, bool x_f32 = false, bool f32mfma = false);
+// Both panel solves of a mixed plan in one launch on FP64 MFMA: count64 FP64
+// row blocks from g0 on, then count32 FP32 ones.
+void launch_trsm_rows_mixed(const TileMap &map, int k, int64_t g0, int count64, int count32, const double *winv_k,
+                            hipStream_t s);
 // Trailing update C_ij -= L_ik L_jk^T for the first npairs lower tiles of the
 // update order (all (i, j) with i >= j > k).
 void launch_update(const TileMap &map, const int2 *pairs, int npairs, int k, hipStream_t s);
