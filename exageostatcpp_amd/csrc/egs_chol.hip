@@ -512,41 +512,60 @@ __device__ __forceinline__ void potrf64_body(const TileMap &map, int k, int jb, 
     for (int q = 0; q < kCols; ++q) y[q] = Ajj[i + (int64_t) (w + kW * q) * ts];
     if (w == 0) s_col[0][i] = y[0];
     __syncthreads();
-    for (int j = 0; j < kBlk; ++j) {
-        const int buf = j & 1;
-        const double piv = s_col[buf][j];
-        if (t == 0 && !(piv > 0.0)) {
-            const int64_t col = (int64_t) k * ts + (int64_t) jb * kBlk + j;
-            if (col < n) atomicCAS(info, 0, (int) (col + 1));
-        }
-        // 1 / sqrt(piv): the hardware estimate and two Newton steps (each
-        // doubles the correct bits; the chain is 7 dependent operations instead
-        // of a correctly rounded square root followed by a division)
-        double rs = __builtin_amdgcn_rsq(piv);
-        const double hp = 0.5 * piv;
-        rs = fma(rs, fma(-hp * rs, rs, 0.5), rs);
-        rs = fma(rs, fma(-hp * rs, rs, 0.5), rs);
-        const double sq = piv * rs;
-        if (t == 0) s_rs[j] = rs;
-        const double lo = i > j ? s_col[buf][i] * rs : 0.0;   // L_ij below the diagonal
-        const double r = lo * rs;
-        // column j of the unscaled X after its own step
-        const double xj0 = i == j ? 1.0 : -r;
-        // the LDS operands of every column at once: one wait for all of them
-        double gl[kCols];
+    // j = kW p + jj: with p unrolled, column c = w + kW q is known to lie
+    // before column j for q < p and after it for q > p; only q = p compares
+    // the wave index with jj
 #pragma unroll
-        for (int q = 0; q < kCols; ++q) gl[q] = s_col[buf][w + kW * q];
+    for (int p = 0; p < kCols; ++p) {
+#pragma unroll 1
+        for (int jj = 0; jj < kW; ++jj) {
+            const int j = kW * p + jj;
+            const int buf = jj & 1;
+            // every LDS operand of the step is read here, ahead of the pivot's
+            // arithmetic, so that the step waits on LDS once
+            const double piv = s_col[buf][j];
+            const double aij = s_col[buf][i];
+            double gl[kCols];
 #pragma unroll
-        for (int q = 0; q < kCols; ++q) {
-            const int c = w + kW * q;   // uniform across the wave
-            double g = gl[q];
-            if (c <= j) g = readlane_f64(y[q], j);
-            const double upd = fma(-r, g, y[q]);
-            if (c == j) Ajj[i + (int64_t) c * ts] = i > j ? lo : (i == j ? sq : 0.0);
-            y[q] = c == j ? xj0 : upd;
-            if (c == j + 1) s_col[buf ^ 1][i] = upd;
+            for (int q = p; q < kCols; ++q) gl[q] = s_col[buf][w + kW * q];
+            if (t == 0 && !(piv > 0.0)) {
+                const int64_t col = (int64_t) k * ts + (int64_t) jb * kBlk + j;
+                if (col < n) atomicCAS(info, 0, (int) (col + 1));
+            }
+            // 1 / sqrt(piv): the hardware estimate and two Newton steps (each
+            // doubles the correct bits; the chain is 7 dependent operations
+            // instead of a correctly rounded square root followed by a division)
+            double rs = __builtin_amdgcn_rsq(piv);
+            const double hp = 0.5 * piv;
+            rs = fma(rs, fma(-hp * rs, rs, 0.5), rs);
+            rs = fma(rs, fma(-hp * rs, rs, 0.5), rs);
+            const double sq = piv * rs;
+            if (t == 0) s_rs[j] = rs;
+            const double lo = i > j ? aij * rs : 0.0;   // L_ij below the diagonal
+            const double r = lo * rs;
+            // column j of the unscaled X after its own step
+            const double xj0 = i == j ? 1.0 : -r;
+#pragma unroll
+            for (int q = 0; q < kCols; ++q) {
+                if (q < p) {
+                    // c < j: X's column, operand X_jc from lane j
+                    y[q] = fma(-r, readlane_f64(y[q], j), y[q]);
+                } else if (q > p) {
+                    // c > j: A's column, operand A_cj from LDS
+                    y[q] = fma(-r, gl[q], y[q]);
+                    if (q == p + 1 && w == 0 && jj == kW - 1) s_col[buf ^ 1][i] = y[q];
+                } else {
+                    // c = w + kW p against j = kW p + jj: selects rather than
+                    // branches, so that y does not pass through branch joins
+                    const double g = w < jj ? readlane_f64(y[q], j) : gl[q];
+                    const double upd = fma(-r, g, y[q]);
+                    if (w == jj) Ajj[i + (int64_t) j * ts] = i > j ? lo : (i == j ? sq : 0.0);
+                    if (w == jj + 1) s_col[buf ^ 1][i] = upd;
+                    y[q] = w == jj ? xj0 : upd;   // c == j: L's column stored, X's column starts
+                }
+            }
+            __syncthreads();
         }
-        __syncthreads();
     }
     double *W = winv_k + (int64_t) jb * kBlk * kBlk;
     const double rsi = s_rs[i];
