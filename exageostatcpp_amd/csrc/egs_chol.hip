@@ -4,8 +4,8 @@
 //
 // Step k of the factorization:
 //   * the diagonal tile is factored right-looking in 64 x 64 blocks
-//     (potrf64_kernel, diag_update_kernel, diag_copy_kernel below), which also
-//     leaves the 64 x 64 inverses of its diagonal blocks;
+//     (diag_step_kernel, diag_copy_kernel below), which also leaves the
+//     64 x 64 inverses of its diagonal blocks;
 //   * the panel below the diagonal tile is solved by one workgroup per 64-row
 //     block, marching over the 64-column blocks of L_kk and applying the
 //     64 x 64 inverses (trsm_rows_kernel);
@@ -462,15 +462,21 @@ void launch_tinv(const TileMap &map, const double *winv, double *tinv, hipStream
 }
 
 // ------------------------------------------------------------------------
-// Diagonal tile, right-looking in 64 x 64 blocks.  Step jb:
-//   potrf64_kernel      factors block (jb, jb) (already updated) and inverts it;
-//   diag_update_kernel  one workgroup per block (r, c), jb < c <= r: forms
-//                       L_r = A_r,jb W_jb^T and L_c = A_c,jb W_jb^T and applies
-//                       A_rc -= L_r L_c^T; the diagonal workgroups (r, r) also
-//                       write L_r to a scratch copy of the tile (the tile's own
-//                       block (r, jb) is still being read by other workgroups);
+// Diagonal tile, right-looking in 64 x 64 blocks, one launch per block
+// column (diag_step_kernel).  Launch jb:
+//   workgroup 0         applies column jb - 1 to block (jb, jb), then factors
+//                       that block and inverts it (potrf64_body);
+//   workgroups 1 ..     apply column jb - 1 to the other blocks (r, c),
+//                       jb <= c <= r (diag_update_body): form
+//                       L_r = A_r,jb-1 W_jb-1^T and L_c = A_c,jb-1 W_jb-1^T and
+//                       apply A_rc -= L_r L_c^T; the diagonal blocks (r, r)
+//                       also write L_r to a scratch copy of the tile (the
+//                       tile's own block (r, jb - 1) is still being read by
+//                       other workgroups);
 //   diag_copy_kernel    at the end moves the solved blocks below the diagonal
 //                       from the scratch copy into the tile.
+// Block (jb, jb) needs only column jb - 1's update before it is factored, so
+// the rest of that update runs beside the factorization instead of before it.
 // ------------------------------------------------------------------------
 namespace {
 // value of v in lane l (l uniform across the wave)
@@ -495,7 +501,9 @@ __device__ __forceinline__ double readlane_f64(double v, int l) {
 // buffered) for c > j, the unscaled X_jc read from lane j for c <= j.  At step
 // j the owner wave of column j stores L's column j and restarts its register
 // as column j of X.
-constexpr int kPotrfWaves = 8;   // two waves per SIMD: each step's FMAs interleave
+// One wave per SIMD, the block size of the update workgroups it launches with
+// (8 waves, two per SIMD, factor a block 2.5% faster alone).
+constexpr int kPotrfWaves = kThreads / 64;
 
 __device__ __forceinline__ void potrf64_body(const TileMap &map, int k, int jb, double *winv_k, int *info, int64_t n) {
     constexpr int kW = kPotrfWaves;
@@ -576,11 +584,6 @@ __device__ __forceinline__ void potrf64_body(const TileMap &map, int k, int jb, 
     }
 }
 
-__global__ void __launch_bounds__(64 * kPotrfWaves) potrf64_kernel(TileMap map, int k, int jb, double *winv_k,
-                                                                   int *info, int64_t n) {
-    potrf64_body(map, k, jb, winv_k, info, n);
-}
-
 // acc = X (64 x 64, column-major, ldx) W^T, W the 64 x 64 lower inverse (ld 64).
 __device__ __forceinline__ void times_wt(Acc64 &acc, const double *X, int64_t ldx, const double *W, GemmSmem &sm) {
     acc_zero(acc);
@@ -648,10 +651,30 @@ __device__ __forceinline__ void diag_update_body(const TileMap &map, int k, int 
     }
 }
 
-__global__ void __launch_bounds__(kThreads) diag_update_kernel(TileMap map, int k, int jb, const double *winv_k,
-                                                               double *scratch) {
-    diag_update_body(map, k, jb, winv_k, scratch, blockIdx.x);
+// Launch jb of the diagonal tile (see above); rem (rem + 1) / 2 workgroups
+// for jb > 0, rem = nsb - jb, and one for jb = 0.
+__device__ __forceinline__ void diag_step_body(const TileMap &map, int k, int jb, double *winv_k, double *scratch,
+                                               int *info, int64_t n, int bx) {
+    if (jb > 0) {
+        // workgroup 0 takes the first pair, (jb, jb)
+        diag_update_body(map, k, jb - 1, winv_k, scratch, bx);
+        if (bx != 0) return;
+        __syncthreads();   // block (jb, jb) updated by every thread
+    }
+    potrf64_body(map, k, jb, winv_k, info, n);
 }
+
+__global__ void __launch_bounds__(kThreads) diag_step_kernel(TileMap map, int k, int jb, double *winv_k,
+                                                             double *scratch, int *info, int64_t n) {
+    diag_step_body(map, k, jb, winv_k, scratch, info, n, blockIdx.x);
+}
+
+namespace {
+unsigned diag_step_blocks(int nsb, int jb) {
+    const int rem = nsb - jb;
+    return jb == 0 ? 1u : (unsigned) (rem * (rem + 1) / 2);
+}
+}  // namespace
 
 // Blocks (r, c), r > c, of the scratch copy back into diagonal tile k.
 __device__ __forceinline__ void diag_copy_body(const TileMap &map, int k, const double *scratch, int bx) {
@@ -680,14 +703,9 @@ __global__ void __launch_bounds__(kThreads) diag_copy_kernel(TileMap map, int k,
 // same tile size, problem b described by jobs[b] (blockIdx.y, or blockIdx.z
 // for the update); workgroups beyond a problem's extent exit at once.
 // ------------------------------------------------------------------------
-__global__ void __launch_bounds__(64 * kPotrfWaves) potrf64_batch_kernel(const BatchJob *jobs, int jb) {
+__global__ void __launch_bounds__(kThreads) diag_step_batch_kernel(const BatchJob *jobs, int jb) {
     const BatchJob &J = jobs[blockIdx.y];
-    potrf64_body(J.map, J.k, jb, J.winv_k, J.info, J.n);
-}
-
-__global__ void __launch_bounds__(kThreads) diag_update_batch_kernel(const BatchJob *jobs, int jb) {
-    const BatchJob &J = jobs[blockIdx.y];
-    diag_update_body(J.map, J.k, jb, J.winv_k, J.scratch, blockIdx.x);
+    diag_step_body(J.map, J.k, jb, J.winv_k, J.scratch, J.info, J.n, blockIdx.x);
 }
 
 __global__ void __launch_bounds__(kThreads) diag_copy_batch_kernel(const BatchJob *jobs) {
@@ -713,13 +731,9 @@ __global__ void __launch_bounds__(kThreads) update_batch_kernel(const BatchJob *
 void launch_factor_diag_batch(const BatchJob *jobs, int njobs, int ts, hipStream_t s) {
     if (njobs <= 0) return;
     const int nsb = ts / kBlk;
-    for (int jb = 0; jb < nsb; ++jb) {
-        hipLaunchKernelGGL(potrf64_batch_kernel, dim3(1, njobs), dim3(64 * kPotrfWaves), 0, s, jobs, jb);
-        const int rem = nsb - jb - 1;
-        if (rem > 0)
-            hipLaunchKernelGGL(diag_update_batch_kernel, dim3((unsigned) (rem * (rem + 1) / 2), njobs), dim3(kThreads),
-                               0, s, jobs, jb);
-    }
+    for (int jb = 0; jb < nsb; ++jb)
+        hipLaunchKernelGGL(diag_step_batch_kernel, dim3(diag_step_blocks(nsb, jb), njobs), dim3(kThreads), 0, s, jobs,
+                           jb);
     if (nsb > 1)
         hipLaunchKernelGGL(diag_copy_batch_kernel, dim3((unsigned) (nsb * (nsb - 1) / 2), njobs), dim3(kThreads), 0, s,
                            jobs);
@@ -736,13 +750,9 @@ void launch_panel_update_batch(const BatchJob *jobs, int njobs, int ts, int rest
 void launch_factor_diag(const TileMap &map, int k, double *winv_k, double *scratch, int *info, int64_t n,
                         hipStream_t s) {
     const int nsb = map.ts / kBlk;
-    for (int jb = 0; jb < nsb; ++jb) {
-        hipLaunchKernelGGL(potrf64_kernel, dim3(1), dim3(64 * kPotrfWaves), 0, s, map, k, jb, winv_k, info, n);
-        const int rem = nsb - jb - 1;
-        if (rem > 0)
-            hipLaunchKernelGGL(diag_update_kernel, dim3((unsigned) (rem * (rem + 1) / 2)), dim3(kThreads), 0, s, map,
-                               k, jb, winv_k, scratch);
-    }
+    for (int jb = 0; jb < nsb; ++jb)
+        hipLaunchKernelGGL(diag_step_kernel, dim3(diag_step_blocks(nsb, jb)), dim3(kThreads), 0, s, map, k, jb, winv_k,
+                           scratch, info, n);
     if (nsb > 1)
         hipLaunchKernelGGL(diag_copy_kernel, dim3((unsigned) (nsb * (nsb - 1) / 2)), dim3(kThreads), 0, s, map, k,
                            scratch);
