@@ -181,6 +181,138 @@ __global__ void __launch_bounds__(kThreads) trsm_rows_kernel(TileMap map, int k,
 }
 
 // ------------------------------------------------------------------------
+// The same solve in 32-row blocks.  A row block's solve is a chain of ts / 64
+// dependent steps bound by its CU's FP64 MFMA rate; in 32-row blocks a short
+// panel (the end of a factorization, small problems) spreads over twice the
+// CUs, each with half the MFMA work (and the same loads of L_kk).
+// ------------------------------------------------------------------------
+namespace {
+constexpr int kHalf = 32;   // rows per block
+
+// acc.v[mi] holds C[mi*16 + lane%16][w*16 + lane/16 + 4*v], w = threadIdx.x / 64.
+struct AccHalf {
+    f64x4 v[2];
+};
+
+__device__ __forceinline__ void acc_zero(AccHalf &acc) {
+    acc.v[0] = f64x4{0.0, 0.0, 0.0, 0.0};
+    acc.v[1] = f64x4{0.0, 0.0, 0.0, 0.0};
+}
+
+template <typename F>
+__device__ __forceinline__ void acc_foreach(const AccHalf &acc, F f) {
+    const int lane = threadIdx.x & 63;
+    const int w = threadIdx.x >> 6;
+#pragma unroll
+    for (int mi = 0; mi < 2; ++mi)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) f(mi * 16 + (lane & 15), w * 16 + (lane >> 4) + 4 * v, acc.v[mi][v]);
+}
+
+// acc += A (32 x K, column-major, lda) * B (64 x K, column-major, ldb)^T,
+// staged like wg_gemm_nt (A in the first 32 rows of sm.a).  K must be a
+// multiple of kKc.
+template <typename TA>
+__device__ void wg_gemm_nt_half(AccHalf &acc, const TA *__restrict__ A, int64_t lda, const double *__restrict__ B,
+                                int64_t ldb, int K, GemmSmem &sm) {
+    constexpr int kStageA = kHalf * kKc / kThreads;   // 4
+    const int t = threadIdx.x;
+    const int lane = t & 63;
+    const int w = t >> 6;
+    const int lr = lane & 15, lk = lane >> 4;
+    // A chunk element (row t % 32, column t / 32 + 8 q); B chunk element
+    // (row t % 64, column t / 64 + 4 q)
+    const int arow = t & (kHalf - 1), acol = t / kHalf;
+    const int brow = t & 63, bcol = t >> 6;
+    double ra[kStageA], rb[kStage];
+#pragma unroll
+    for (int q = 0; q < kStageA; ++q) ra[q] = (double) A[arow + (int64_t) (acol + q * (kThreads / kHalf)) * lda];
+#pragma unroll
+    for (int q = 0; q < kStage; ++q) rb[q] = B[brow + (int64_t) (bcol + q * (kThreads / kBlk)) * ldb];
+#pragma unroll
+    for (int q = 0; q < kStageA; ++q) sm.a[0][acol + q * (kThreads / kHalf)][arow] = ra[q];
+#pragma unroll
+    for (int q = 0; q < kStage; ++q) sm.b[0][bcol + q * (kThreads / kBlk)][brow] = rb[q];
+    __syncthreads();
+    const int nch = K / kKc;
+    for (int c = 0; c < nch; ++c) {
+        const int buf = c & 1;
+        const bool more = c + 1 < nch;
+        if (more) {
+            const int64_t k0 = (int64_t) (c + 1) * kKc;
+#pragma unroll
+            for (int q = 0; q < kStageA; ++q) ra[q] = (double) A[arow + (k0 + acol + q * (kThreads / kHalf)) * lda];
+#pragma unroll
+            for (int q = 0; q < kStage; ++q) rb[q] = B[brow + (k0 + bcol + q * (kThreads / kBlk)) * ldb];
+        }
+#pragma unroll
+        for (int kk = 0; kk < kKc; kk += 4) {
+            const double a0 = sm.a[buf][kk + lk][lr];
+            const double a1 = sm.a[buf][kk + lk][16 + lr];
+            const double b0 = sm.b[buf][kk + lk][w * 16 + lr];
+            acc.v[0] = mfma_f64(b0, a0, acc.v[0]);
+            acc.v[1] = mfma_f64(b0, a1, acc.v[1]);
+        }
+        if (more) {
+#pragma unroll
+            for (int q = 0; q < kStageA; ++q) sm.a[buf ^ 1][acol + q * (kThreads / kHalf)][arow] = ra[q];
+#pragma unroll
+            for (int q = 0; q < kStage; ++q) sm.b[buf ^ 1][bcol + q * (kThreads / kBlk)][brow] = rb[q];
+        }
+        __syncthreads();
+    }
+}
+}  // namespace
+
+// Row block g0 + bx of 32 rows (ts / 32 per tile row), all column blocks.
+template <typename TX>
+__device__ __forceinline__ void trsm_rows_half_body(const TileMap &map, int k, int64_t g0, const double *winv_k,
+                                                    int bx, GemmSmem &sm) {
+    double (*sT)[kLdsPad] = reinterpret_cast<double (*)[kLdsPad]>(&sm.a[0][0][0]);
+    const int ts = map.ts;
+    const int nsb = ts / kBlk;
+    const int nrb = ts / kHalf;
+    const int64_t g = g0 + bx;
+    const int i = (int) (g / nrb);
+    const int rb = (int) (g % nrb);
+    TX *X = reinterpret_cast<TX *>(map.base + map.off[i * map.nt + k]) + rb * kHalf;
+    const double *L = tile_f64(map, k, k);
+    const int lane = threadIdx.x & 63;
+    const int w = threadIdx.x >> 6;
+    const int lr = lane & 15, lk = lane >> 4;
+
+    for (int jb = 0; jb < nsb; ++jb) {
+        AccHalf acc;
+        acc_zero(acc);
+        if (jb > 0) wg_gemm_nt_half(acc, X, ts, L + jb * kBlk, ts, jb * kBlk, sm);
+        TX *Xj = X + (int64_t) jb * kBlk * ts;
+        acc_foreach(acc, [&](int r, int c, double v) { sT[r][c] = (double) Xj[r + (int64_t) c * ts] - v; });
+        __syncthreads();
+        // X_j = T W_j^T
+        const double *W = winv_k + (int64_t) jb * kBlk * kBlk;
+        AccHalf acc2;
+        acc_zero(acc2);
+#pragma unroll 4
+        for (int kk = 0; kk < kBlk; kk += 4) {
+            const double a0 = sT[lr][kk + lk];
+            const double a1 = sT[16 + lr][kk + lk];
+            const double b0 = W[(w * 16 + lr) + (kk + lk) * kBlk];
+            acc2.v[0] = mfma_f64(b0, a0, acc2.v[0]);
+            acc2.v[1] = mfma_f64(b0, a1, acc2.v[1]);
+        }
+        acc_foreach(acc2, [&](int r, int c, double v) { Xj[r + (int64_t) c * ts] = (TX) v; });
+        __syncthreads();
+    }
+}
+
+template <typename TX>
+__global__ void __launch_bounds__(kThreads) trsm_rows_half_kernel(TileMap map, int k, int64_t g0,
+                                                                  const double *winv_k) {
+    __shared__ GemmSmem sm;
+    trsm_rows_half_body<TX>(map, k, g0, winv_k, blockIdx.x, sm);
+}
+
+// ------------------------------------------------------------------------
 // The same solve for FP32 panel rows with v_mfma_f32_16x16x4f32 (FP32
 // accumulation; L and W converted to FP32 as they are staged, 50 KB of LDS):
 // 1.8x the rate of the FP64 form, for the univariate kernel's mixed plans
@@ -337,6 +469,16 @@ __global__ void __launch_bounds__(kThreads) trsm_rows_mixed_kernel(TileMap map, 
         trsm_rows_body<float>(map, k, g0, 0, nsb, winv_k, blockIdx.x, sm);
 }
 
+// The same in 32-row blocks (g0 and count64 in 32-row blocks).
+__global__ void __launch_bounds__(kThreads) trsm_rows_half_mixed_kernel(TileMap map, int k, int64_t g0, int count64,
+                                                                        const double *winv_k) {
+    __shared__ GemmSmem sm;
+    if ((int) blockIdx.x < count64)
+        trsm_rows_half_body<double>(map, k, g0, winv_k, blockIdx.x, sm);
+    else
+        trsm_rows_half_body<float>(map, k, g0, winv_k, blockIdx.x, sm);
+}
+
 // ------------------------------------------------------------------------
 // Trailing update, FP64.
 // ------------------------------------------------------------------------
@@ -370,11 +512,23 @@ __global__ void __launch_bounds__(kThreads) update_kernel(TileMap map, const dou
     update_body(map, p64, pairs, k, blockIdx.x, blockIdx.y);
 }
 
+// A panel of fewer 64-row blocks than this is solved in 32-row blocks (with
+// more, every CU already holds a row block, and 64-row blocks load L_kk half
+// as often per flop).
+constexpr int kHalfRowsBelow = 256;
+
 void launch_trsm_rows(const TileMap &map, int k, int64_t g0, int count, int jb0, int jb1, const double *winv_k,
                       hipStream_t s, bool x_f32, bool f32mfma) {
     if (count <= 0 || jb1 <= jb0) return;
+    const bool halves = count < kHalfRowsBelow && jb0 == 0 && jb1 == map.ts / kBlk;
     if (x_f32 && f32mfma)
         hipLaunchKernelGGL(trsm_rows_f32mfma_kernel, dim3(count), dim3(kThreads), 0, s, map, k, g0, jb0, jb1, winv_k);
+    else if (halves && x_f32)
+        hipLaunchKernelGGL(trsm_rows_half_kernel<float>, dim3(2 * count), dim3(kThreads), 0, s, map, k, 2 * g0,
+                           winv_k);
+    else if (halves)
+        hipLaunchKernelGGL(trsm_rows_half_kernel<double>, dim3(2 * count), dim3(kThreads), 0, s, map, k, 2 * g0,
+                           winv_k);
     else if (x_f32)
         hipLaunchKernelGGL(trsm_rows_kernel<float>, dim3(count), dim3(kThreads), 0, s, map, k, g0, jb0, jb1, winv_k);
     else
@@ -384,8 +538,12 @@ void launch_trsm_rows(const TileMap &map, int k, int64_t g0, int count, int jb0,
 void launch_trsm_rows_mixed(const TileMap &map, int k, int64_t g0, int count64, int count32, const double *winv_k,
                             hipStream_t s) {
     if (count64 + count32 <= 0) return;
-    hipLaunchKernelGGL(trsm_rows_mixed_kernel, dim3((unsigned) (count64 + count32)), dim3(kThreads), 0, s, map, k,
-                       g0, count64, winv_k);
+    if (count64 + count32 < kHalfRowsBelow)
+        hipLaunchKernelGGL(trsm_rows_half_mixed_kernel, dim3((unsigned) (2 * (count64 + count32))), dim3(kThreads), 0,
+                           s, map, k, 2 * g0, 2 * count64, winv_k);
+    else
+        hipLaunchKernelGGL(trsm_rows_mixed_kernel, dim3((unsigned) (count64 + count32)), dim3(kThreads), 0, s, map, k,
+                           g0, count64, winv_k);
 }
 
 void launch_update(const TileMap &map, const int2 *pairs, int npairs, int k, hipStream_t s) {
@@ -721,6 +879,14 @@ __global__ void __launch_bounds__(kThreads) trsm_rows_batch_kernel(const BatchJo
     trsm_rows_body<double>(J.map, J.k, (int64_t) (J.k + 1) * nsb, 0, nsb, J.winv_k, blockIdx.x, sm);
 }
 
+__global__ void __launch_bounds__(kThreads) trsm_rows_half_batch_kernel(const BatchJob *jobs) {
+    const BatchJob &J = jobs[blockIdx.y];
+    const int nrb = J.map.ts / kHalf;
+    if ((int) blockIdx.x >= (J.map.nt - J.k - 1) * nrb) return;
+    __shared__ GemmSmem sm;
+    trsm_rows_half_body<double>(J.map, J.k, (int64_t) (J.k + 1) * nrb, J.winv_k, blockIdx.x, sm);
+}
+
 __global__ void __launch_bounds__(kThreads) update_batch_kernel(const BatchJob *jobs) {
     const BatchJob &J = jobs[blockIdx.z];
     const int rest = J.map.nt - J.k - 1;   // tile columns k+1 .. nt-1: the first rest (rest + 1) / 2 pairs
@@ -742,7 +908,11 @@ void launch_factor_diag_batch(const BatchJob *jobs, int njobs, int ts, hipStream
 void launch_panel_update_batch(const BatchJob *jobs, int njobs, int ts, int rest, hipStream_t s) {
     if (njobs <= 0 || rest <= 0) return;
     const int nsb = ts / kBlk;
-    hipLaunchKernelGGL(trsm_rows_batch_kernel, dim3((unsigned) (rest * nsb), njobs), dim3(kThreads), 0, s, jobs);
+    if (rest * nsb * njobs < kHalfRowsBelow)
+        hipLaunchKernelGGL(trsm_rows_half_batch_kernel, dim3((unsigned) (2 * rest * nsb), njobs), dim3(kThreads), 0, s,
+                           jobs);
+    else
+        hipLaunchKernelGGL(trsm_rows_batch_kernel, dim3((unsigned) (rest * nsb), njobs), dim3(kThreads), 0, s, jobs);
     hipLaunchKernelGGL(update_batch_kernel, dim3((unsigned) (nsb * nsb), (unsigned) (rest * (rest + 1) / 2), njobs),
                        dim3(kThreads), 0, s, jobs);
 }
