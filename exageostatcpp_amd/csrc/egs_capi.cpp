@@ -167,6 +167,7 @@ struct egs_problem_s {
     int64_t dev_bytes = 0;                   // device bytes held (size classes of egs_devmem)
     int refine_iters = 0;
     double refine_resid = 0.0;
+    bool refine_converged = true;            // refine_resid <= kRefineAccept
     // mixed plans: an evaluation whose refinement needs more than
     // fallback_iters iterations (0: never), or whose mixed factor is not
     // positive definite, is repeated under the FP64 plan
@@ -880,6 +881,10 @@ int sigma_times(egs_problem_s *p, const double *v, double *q, const double *base
 // tol ||b||.  x is left in d_cg[0 .. npad).
 constexpr double kRefineTol = 1e-12;
 constexpr int kRefineMaxIter = 60;
+// A refinement whose true residual ends above this has not recovered the
+// solve: the iteration broke down (p'Sigma p <= 0 in floating point, or a
+// preconditioner far from Sigma) or ran out of iterations.
+constexpr double kRefineAccept = 1e-10;
 
 int refine_solve(egs_problem_s *p, const double *b) {
     hipStream_t s = p->ctx->stream;
@@ -924,6 +929,7 @@ int refine_solve(egs_problem_s *p, const double *b) {
     prof_end(p, pi);
     p->refine_iters = it;
     p->refine_resid = bb > 0.0 ? std::sqrt(rr / bb) : 0.0;
+    p->refine_converged = p->refine_resid <= kRefineAccept;
     return EGS_SUCCESS;
 }
 
@@ -1703,8 +1709,10 @@ int egs_loglik(egs_problem_t p, const double *theta, double *loglik, double *log
     // The mixed factor gives log|Sigma| directly, and the refinement corrects
     // only z'Sigma^{-1}z; a refinement that needs many iterations means the
     // factor is far from Sigma's (smooth fields), so its log-determinant is
-    // not trusted.  Such an evaluation, and one whose mixed factor is not
-    // positive definite, is repeated under the FP64 plan.
+    // not trusted.  Such an evaluation, one whose refinement did not converge
+    // (its quadratic form is not recovered, and q0 - q then says nothing), and
+    // one whose mixed factor is not positive definite, is repeated under the
+    // FP64 plan.
     // With w = L^{-1} z and S = L^{-1} Sigma L^{-T} - I, the mixed factor's
     // log-determinant is off by log det(I + S) ~ tr S, and q0 - q = w'w -
     // w'(I + S)^{-1} w ~ w'S w, whose mean is tr S when w ~ N(0, I) (z drawn
@@ -1715,7 +1723,8 @@ int egs_loglik(egs_problem_t p, const double *theta, double *loglik, double *log
     else if (st == EGS_SUCCESS)
         p->error_estimate = 0.0;
     if (p->fallback_iters <= 0 && p->fallback_tol <= 0.0) return st;   // the guard is off
-    const bool poor = st == EGS_SUCCESS && ((p->fallback_iters > 0 && p->refine_iters > p->fallback_iters) ||
+    const bool poor = st == EGS_SUCCESS && (!p->refine_converged ||
+                                            (p->fallback_iters > 0 && p->refine_iters > p->fallback_iters) ||
                                             (p->fallback_tol > 0.0 && p->error_estimate > p->fallback_tol));
     if (!poor && st != EGS_ERR_NOT_SPD) return st;
     const int mode = p->mode, b64 = p->band64, b32 = p->band32;
@@ -1909,6 +1918,12 @@ static int generate_observations_fp64(egs_problem_s *p, const double *theta, int
     return EGS_SUCCESS;
 }
 
+// predict_once's status when a mixed plan's refinement did not converge and
+// the guard is on (egs_predict_mspe then repeats the prediction in FP64)
+constexpr int kRefineNotConverged = -1;
+
+static bool guard_on(const egs_problem_s *p) { return p->fallback_iters > 0 || p->fallback_tol > 0.0; }
+
 static int predict_once(egs_problem_s *p, const double *theta, int64_t n_miss, const double *xm, const double *ym,
                         const double *zm, double *zmiss, const double *z_actual, double *mspe3) {
     int st = EGS_SUCCESS;
@@ -1946,6 +1961,10 @@ static int predict_once(egs_problem_s *p, const double *theta, int64_t n_miss, c
     if (is_mixed(p)) {
         st = refine_solve(p, p->d_obs);
         if (st) return st;
+        if (!p->refine_converged && guard_on(p)) {
+            if (!reuse && (st = read_info(p))) return st;   // NOT_SPD takes the same fallback
+            return kRefineNotConverged;
+        }
         weights = p->d_cg;
     } else {
         egs::launch_copy_pad(p->d_work, p->d_obs, p->n, p->npad, s);
@@ -2000,9 +2019,10 @@ int egs_predict_mspe(egs_problem_t p, const double *theta, int64_t n_miss, const
     if (st) return st;
     p->fell_back = false;
     st = predict_once(p, theta, n_miss, xm, ym, zm, zmiss, z_actual, mspe3);
-    if (st != EGS_ERR_NOT_SPD || !is_mixed(p) || (p->fallback_iters <= 0 && p->fallback_tol <= 0.0)) return st;
-    // a mixed factor that is not positive definite: the prediction under the
-    // FP64 plan, then the plan again (unfactored), as egs_loglik does
+    if ((st != EGS_ERR_NOT_SPD && st != kRefineNotConverged) || !is_mixed(p) || !guard_on(p)) return st;
+    // a mixed factor that is not positive definite, or whose refinement did
+    // not converge: the prediction under the FP64 plan, then the plan again
+    // (unfactored), as egs_loglik does
     const int mode = p->mode, b64 = p->band64, b32 = p->band32;
     if ((st = egs_problem_set_precision(p, EGS_PRECISION_FP64, b64, b32))) return st;
     const int st_pred = predict_once(p, theta, n_miss, xm, ym, zm, zmiss, z_actual, mspe3);

@@ -453,6 +453,114 @@ __global__ void __launch_bounds__(kThreads) trsm_rows_f32mfma_kernel(TileMap map
     }
 }
 
+// The FP32-MFMA solve in 32-row blocks (as trsm_rows_half_body): wave w
+// holds rows 0-31 and columns 16 w .. 16 w + 15; acc.v[mi] element v of lane
+// l is C[mi*16 + l%16][16 w + 4 (l/16) + v].
+struct AccHalf32 {
+    f32x4 v[2];
+};
+
+__device__ void wg_gemm_nt_f32_half(AccHalf32 &acc, const float *__restrict__ A, int64_t lda,
+                                    const double *__restrict__ B, int64_t ldb, int K, GemmSmem32 &sm) {
+    constexpr int kStageA = kHalf * kKc / kThreads;   // 4
+    const int t = threadIdx.x;
+    const int lane = t & 63;
+    const int w = t >> 6;
+    const int lr = lane & 15, lk = lane >> 4;
+    const int arow = t & (kHalf - 1), acol = t / kHalf;
+    const int brow = t & 63, bcol = t >> 6;
+    float ra[kStageA], rb[kStage];
+#pragma unroll
+    for (int q = 0; q < kStageA; ++q) ra[q] = A[arow + (int64_t) (acol + q * (kThreads / kHalf)) * lda];
+#pragma unroll
+    for (int q = 0; q < kStage; ++q) rb[q] = (float) B[brow + (int64_t) (bcol + q * (kThreads / kBlk)) * ldb];
+#pragma unroll
+    for (int q = 0; q < kStageA; ++q) sm.a[0][acol + q * (kThreads / kHalf)][arow] = ra[q];
+#pragma unroll
+    for (int q = 0; q < kStage; ++q) sm.b[0][bcol + q * (kThreads / kBlk)][brow] = rb[q];
+    __syncthreads();
+    const int nch = K / kKc;
+    for (int c = 0; c < nch; ++c) {
+        const int buf = c & 1;
+        const bool more = c + 1 < nch;
+        if (more) {
+            const int64_t k0 = (int64_t) (c + 1) * kKc;
+#pragma unroll
+            for (int q = 0; q < kStageA; ++q) ra[q] = A[arow + (k0 + acol + q * (kThreads / kHalf)) * lda];
+#pragma unroll
+            for (int q = 0; q < kStage; ++q) rb[q] = (float) B[brow + (k0 + bcol + q * (kThreads / kBlk)) * ldb];
+        }
+#pragma unroll
+        for (int kk = 0; kk < kKc; kk += 4) {
+            const float a0 = sm.a[buf][kk + lk][lr];
+            const float a1 = sm.a[buf][kk + lk][16 + lr];
+            const float b0 = sm.b[buf][kk + lk][w * 16 + lr];
+            acc.v[0] = mfma_f32(b0, a0, acc.v[0]);
+            acc.v[1] = mfma_f32(b0, a1, acc.v[1]);
+        }
+        if (more) {
+#pragma unroll
+            for (int q = 0; q < kStageA; ++q) sm.a[buf ^ 1][acol + q * (kThreads / kHalf)][arow] = ra[q];
+#pragma unroll
+            for (int q = 0; q < kStage; ++q) sm.b[buf ^ 1][bcol + q * (kThreads / kBlk)][brow] = rb[q];
+        }
+        __syncthreads();
+    }
+}
+
+template <typename F>
+__device__ __forceinline__ void acc32_foreach(const AccHalf32 &acc, F f) {
+    const int lane = threadIdx.x & 63;
+    const int w = threadIdx.x >> 6;
+#pragma unroll
+    for (int mi = 0; mi < 2; ++mi)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) f(mi * 16 + (lane & 15), w * 16 + 4 * (lane >> 4) + v, acc.v[mi][v]);
+}
+
+// g0 in 32-row blocks; all column blocks.
+__global__ void __launch_bounds__(kThreads) trsm_rows_f32mfma_half_kernel(TileMap map, int k, int64_t g0,
+                                                                          const double *winv_k) {
+    __shared__ GemmSmem32 sm;
+    __shared__ float sW[kBlk][kLdsPad];
+    float (*sT)[kLdsPad] = reinterpret_cast<float (*)[kLdsPad]>(&sm.a[0][0][0]);
+    const int ts = map.ts;
+    const int nsb = ts / kBlk;
+    const int nrb = ts / kHalf;
+    const int64_t g = g0 + blockIdx.x;
+    const int i = (int) (g / nrb);
+    const int rb = (int) (g % nrb);
+    float *X = reinterpret_cast<float *>(map.base + map.off[i * map.nt + k]) + rb * kHalf;
+    const double *L = tile_f64(map, k, k);
+    const int lane = threadIdx.x & 63;
+    const int w = threadIdx.x >> 6;
+    const int lr = lane & 15, lk = lane >> 4;
+    for (int jb = 0; jb < nsb; ++jb) {
+        AccHalf32 acc;
+        acc.v[0] = f32x4{0.f, 0.f, 0.f, 0.f};
+        acc.v[1] = f32x4{0.f, 0.f, 0.f, 0.f};
+        if (jb > 0) wg_gemm_nt_f32_half(acc, X, ts, L + jb * kBlk, ts, jb * kBlk, sm);
+        float *Xj = X + (int64_t) jb * kBlk * ts;
+        acc32_foreach(acc, [&](int r, int c, float v) { sT[r][c] = Xj[r + (int64_t) c * ts] - v; });
+        const double *W = winv_k + (int64_t) jb * kBlk * kBlk;
+        for (int e = threadIdx.x; e < kBlk * kBlk; e += kThreads) sW[e & 63][e >> 6] = (float) W[e];
+        __syncthreads();
+        AccHalf32 acc2;
+        acc2.v[0] = f32x4{0.f, 0.f, 0.f, 0.f};
+        acc2.v[1] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll 4
+        for (int kk = 0; kk < kBlk; kk += 4) {
+            const float a0 = sT[lr][kk + lk];
+            const float a1 = sT[16 + lr][kk + lk];
+            const float b0 = sW[w * 16 + lr][kk + lk];
+            acc2.v[0] = mfma_f32(b0, a0, acc2.v[0]);
+            acc2.v[1] = mfma_f32(b0, a1, acc2.v[1]);
+        }
+        acc32_foreach(acc2, [&](int r, int c, float v) { Xj[r + (int64_t) c * ts] = v; });
+        __syncthreads();
+    }
+}
+
 // FP64 rows [g0, g0 + count64) and FP32 rows after them in one launch, both
 // on FP64 MFMA, so that the two panel solves of a mixed plan run side by side
 // rather than one after the other (each is a chain of ts / 64 dependent steps
@@ -521,7 +629,10 @@ void launch_trsm_rows(const TileMap &map, int k, int64_t g0, int count, int jb0,
                       hipStream_t s, bool x_f32, bool f32mfma) {
     if (count <= 0 || jb1 <= jb0) return;
     const bool halves = count < kHalfRowsBelow && jb0 == 0 && jb1 == map.ts / kBlk;
-    if (x_f32 && f32mfma)
+    if (halves && x_f32 && f32mfma)
+        hipLaunchKernelGGL(trsm_rows_f32mfma_half_kernel, dim3(2 * count), dim3(kThreads), 0, s, map, k, 2 * g0,
+                           winv_k);
+    else if (x_f32 && f32mfma)
         hipLaunchKernelGGL(trsm_rows_f32mfma_kernel, dim3(count), dim3(kThreads), 0, s, map, k, g0, jb0, jb1, winv_k);
     else if (halves && x_f32)
         hipLaunchKernelGGL(trsm_rows_half_kernel<float>, dim3(2 * count), dim3(kThreads), 0, s, map, k, 2 * g0,

@@ -172,7 +172,9 @@ int egs_problem_set_graph(egs_problem_t problem, int enable);
  * relative log-likelihood error.  An evaluation is repeated under the FP64
  * plan (the mixed plan restored afterwards, unfactored) when that estimate
  * exceeds `tolerance`, when the refinement needs more than max_iterations
- * iterations, or when the mixed factor is not positive definite (the last
+ * iterations, when it ends above a relative FP64 residual of 1e-10 (a
+ * breakdown: the quadratic form is not recovered and the estimate says
+ * nothing), or when the mixed factor is not positive definite (the last two
  * also for egs_predict / egs_predict_mspe).  Defaults:
  * 4 iterations and 1e-5 (the default plan's estimate at n = 100k on the
  * exponential field is of order 1e-7); 0 turns either test off.  No

@@ -196,6 +196,40 @@ def test_single_product_selection_pins_both_ends(ctx, data4k):
     p.close()
 
 
+@pytest.mark.parametrize("theta", [(1.0, 0.1, 0.5), (1.5, 0.2, 1.5), (0.8, 0.3, 2.5), (2.0, 0.02, 0.3)])
+def test_guarded_mixed_results_are_refined_or_fp64(ctx, theta):
+    """Under the default guard a mixed evaluation either is repeated in FP64 or
+    its refinement converged (relative FP64 residual <= 1e-10: a breakdown of
+    the refinement is a fallback, since the error estimate then says nothing),
+    and kriging agrees with the FP64 plan's to 1e-8 either way (rough to very
+    smooth fields, serving-sized tiles)."""
+    th = list(theta)
+    n = 3000
+    rng = np.random.default_rng(7)
+    x, y = rng.random(n), rng.random(n)
+    xm, ym = rng.random(64), rng.random(64)
+    p = egs.Problem(ctx, 0, x, y, dts=384)
+    p.generate_observations(th, seed=3)
+    l64 = p.loglik(th)[0]
+    z64, _ = p.predict(th, xm, ym)
+    p.set_precision(egs.PRECISION_MIXED)
+    lm = p.loglik(th)[0]
+    fell, _, est = p.fallback()
+    rf = p.refinement()
+    log("  theta %s: fp64 %.10f mixed %.10f rel %.2e fell back %s estimate %.2e refine %s" %
+        (th, l64, lm, abs(lm - l64) / abs(l64), fell, est, rf))
+    if not fell:
+        assert rf["residual"] <= 1e-10
+    zmx, _ = p.predict(th, xm, ym)
+    fell_p = p.fallback()[0]
+    if not fell_p:
+        assert p.refinement()["residual"] <= 1e-10
+    err = np.max(np.abs(zmx - z64))
+    log("  kriging max diff %.2e (fell back %s)" % (err, fell_p))
+    assert err <= 1e-8 * max(1.0, np.max(np.abs(z64)))
+    p.close()
+
+
 def test_default_mixed_plan_at_100k_within_1e6_of_fp64(ctx):
     """The benchmark workload at full size (n = 100,000, theta = (1, 0.05, 0.5),
     dts 960): the default mixed plan's log-likelihood within 1e-6 relative of
