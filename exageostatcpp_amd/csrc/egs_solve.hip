@@ -467,7 +467,8 @@ void launch_copy_pad(double *dst, const double *src, int64_t n, int64_t npad, hi
 // ------------------------------------------------------------------------
 // Sigma v from the FP64 lower tiles (refinement of mixed plans).  One
 // workgroup per tile (i, j); the tile is read once: wave w walks the columns
-// c = w, w + 4, ..., lane l holds rows l + 64 m of a 512-row chunk.  The row
+// in groups of four, c = 4 w + 16 p + u, lane l holds rows l + 64 m of a
+// 512-row chunk.  The row
 // products T v_j go to prow, the column products T' v_i (off the diagonal) to
 // pcol; both are summed in a fixed order, so results are reproducible.
 // ------------------------------------------------------------------------
@@ -499,7 +500,44 @@ __global__ void __launch_bounds__(kThreads) sigma_tile_matvec_kernel(const doubl
             vr[m] = r < ts ? vi[r] : 0.0;
             acc[m] = 0.0;
         }
-        for (int c = w; c < ts; c += kThreads / 64) {
+        // four consecutive columns per wave and pass, so that 32 loads per
+        // lane are in flight; their four column dot products are reduced
+        // across the wave together (7 shuffles instead of 24)
+        constexpr int kPass = 4;
+        constexpr int kStride = kPass * (kThreads / 64);
+        const int cmain = ts - ts % kStride;
+        for (int c0 = w * kPass; c0 < cmain; c0 += kStride) {
+            double vc[kPass], dot[kPass];
+#pragma unroll
+            for (int u = 0; u < kPass; ++u) {
+                vc[u] = vj[c0 + u];
+                dot[u] = 0.0;
+            }
+            const double *col = T + (int64_t) c0 * ts + r0 + lane;
+#pragma unroll
+            for (int m = 0; m < kRowsPerLane; ++m) {
+                if (r0 + lane + 64 * m < ts) {
+#pragma unroll
+                    for (int u = 0; u < kPass; ++u) {
+                        const double t = __builtin_nontemporal_load(col + (int64_t) u * ts + 64 * m);
+                        acc[m] += t * vc[u];
+                        dot[u] += t * vr[m];
+                    }
+                }
+            }
+            if (offdiag) {
+                // lanes with bit 5 clear keep columns 0, 1, set 2, 3; then bit
+                // 4 picks one of the two: lane 16 q ends with column q
+                const bool h32 = lane & 32, h16 = lane & 16;
+                const double k0 = (h32 ? dot[2] : dot[0]) + __shfl_xor(h32 ? dot[0] : dot[2], 32, 64);
+                const double k1 = (h32 ? dot[3] : dot[1]) + __shfl_xor(h32 ? dot[1] : dot[3], 32, 64);
+                double d = (h16 ? k1 : k0) + __shfl_xor(h16 ? k0 : k1, 16, 64);
+#pragma unroll
+                for (int off = 8; off > 0; off >>= 1) d += __shfl_xor(d, off, 64);
+                if ((lane & 15) == 0) s_col[c0 + (lane >> 4)] += d;
+            }
+        }
+        for (int c = cmain + w; c < ts; c += kThreads / 64) {
             const double vc = vj[c];
             const double *col = T + (int64_t) c * ts + r0 + lane;
             double dot = 0.0;
