@@ -758,8 +758,8 @@ __device__ __forceinline__ double readlane_f64(double v, int l) {
 
 // Block (jb, jb) of diagonal tile k: L = chol(A) in place and W = L^{-1} into
 // winv_k, both lower (zeros above the diagonal).  The block lives in registers:
-// thread (w, i) = (threadIdx.x / 64, lane) holds row i of columns c = w + 8 q,
-// q < 8, in y[q]: column c of A until step c, then column c of X, which
+// thread (w, i) = (threadIdx.x / 64, lane) holds row i of columns c = w + kW q,
+// q < 64 / kW, in y[q]: column c of A until step c, then column c of X, which
 // becomes L^{-1}.  Column step j (64 steps, one barrier each) is the
 // right-looking Cholesky step A_ic -= L_ij L_cj (c > j) fused with the
 // column-oriented inversion X_jc /= L_jj, X_ic -= L_ij X_jc (i > j, c <= j).
@@ -770,8 +770,8 @@ __device__ __forceinline__ double readlane_f64(double v, int l) {
 // buffered) for c > j, the unscaled X_jc read from lane j for c <= j.  At step
 // j the owner wave of column j stores L's column j and restarts its register
 // as column j of X.
-// One wave per SIMD, the block size of the update workgroups it launches with
-// (8 waves, two per SIMD, factor a block 2.5% faster alone).
+// kW = 4 waves, one per SIMD: the block size of the update workgroups it
+// launches with (8 waves, two per SIMD, factor a block 2.5% faster alone).
 constexpr int kPotrfWaves = kThreads / 64;
 
 __device__ __forceinline__ void potrf64_body(const TileMap &map, int k, int jb, double *winv_k, int *info, int64_t n) {
