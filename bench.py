@@ -123,6 +123,12 @@ def operands(args):
     return "bf16" if args.plain_bf16 else "fp16-split"
 
 
+def fp8_tau(args):
+    """The FP8-product threshold in effect for the univariate kernel: the
+    option, else the library default (2^-23; 0 turns the tier off)."""
+    return args.fp8_tau if args.fp8_tau is not None else 2.0 ** -23
+
+
 def product_band(args, bivariate):
     """The FP64-product band in effect: the option, else the library default."""
     if args.fp64_product_band is not None:
@@ -150,7 +156,7 @@ def measured_traffic(kernel, args):
             and t.get("band_fp64") == args.band_fp64 and t.get("band_fp32") == args.band_fp32
             and t.get("single_band") == args.single_band and t.get("single_tau") == tau_json(args)
             and t.get("fp64_product_band", -1) == product_band(args, bivariate=False)
-            and t.get("operands") == operands(args))
+            and t.get("operands") == operands(args) and t.get("fp8_tau", 0.0) == fp8_tau(args))
     if not same:
         return None, None
     return t["fetch_bytes_per_launch"] + t["write_bytes_per_launch"], t.get("algorithmic_bytes_per_launch")
@@ -202,7 +208,7 @@ def traffic_summary(args):
     out = {"kernel": "update_16_select", "kernel_name_regex": rx, "n": args.n, "dts": args.dts,
            "band_fp64": args.band_fp64, "band_fp32": args.band_fp32, "single_band": args.single_band,
            "single_tau": tau_json(args), "fp64_product_band": product_band(args, bivariate=False),
-           "operands": operands(args),
+           "operands": operands(args), "fp8_tau": fp8_tau(args),
            "fetch_bytes_per_launch": f_kib * 1024.0 * cal_f, "write_bytes_per_launch": w_kib * 1024.0 * cal_w,
            "algorithmic_bytes_per_launch": alg, "launches_per_factorization": launches,
            "tile_updates_per_factorization": updates, "calibration": {"fetch": cal_f, "write": cal_w},
@@ -572,6 +578,9 @@ def main():
                          "univariate and 2 for bivariate kernels)")
     ap.add_argument("--single-tau", type=float, default=2.0 ** -16,
                     help="... when their panel rows' scales multiply to <= tau times the panel's largest squared")
+    ap.add_argument("--fp8-tau", type=float, default=None,
+                    help="FP32-tile blocks whose panel rows' scales multiply to <= this times the panel's largest "
+                         "squared take one FP8 (e4m3) product (0: never; default: the library's, 2^-23)")
     ap.add_argument("--plain-bf16", action="store_true", help="single BF16 product instead of the split one")
     ap.add_argument("--cpu-sample", type=int, default=20000,
                     help="n of the host sample (each evaluation a few seconds on 16 host cores)")
@@ -662,6 +671,7 @@ def main():
     mode = egs.PRECISION_MIXED_BF16 if args.plain_bf16 else egs.PRECISION_MIXED
     prob.set_precision(mode, args.band_fp64, args.band_fp32)
     prob.set_single_product(args.single_band, args.single_tau)
+    prob.set_fp8_product(fp8_tau(args))
     if args.fp64_product_band is not None:
         prob.set_fp64_product_band(args.fp64_product_band)
     mixed = timed(egs, ctx, prob, theta, args.steps, args.warmup, "mixed")
@@ -683,9 +693,11 @@ def main():
     achieved = d["flops"] / (d["ms"] * 1e-3) / 1e12 if d["ms"] > 0 else 0.0
     prec, mfma_per_flop = peaks[dom]
     pc = mixed["product_counts"]
-    if dom == "update_16_select" and pc["split"] + pc["single"] > 0:
-        # MFMAs per product: 3 for split blocks, 1 for single-product ones
-        mfma_per_flop = (3.0 * pc["split"] + pc["single"]) / (pc["split"] + pc["single"])
+    nblk = pc["split"] + pc["single"] + pc.get("fp8", 0)
+    if dom == "update_16_select" and nblk > 0:
+        # FP16-MFMA time per product: 3 for split blocks, 1 for single-product
+        # ones, 1/2 for FP8 ones (e4m3 MFMA at twice the FP16 rate)
+        mfma_per_flop = (3.0 * pc["split"] + pc["single"] + 0.5 * pc.get("fp8", 0)) / nblk
     traffic, alg_bytes = measured_traffic(dom, args)
     dom64 = fp64["kernels"]["update_fp64"]
     ach64 = dom64["flops"] / (dom64["ms"] * 1e-3) / 1e12 if dom64["ms"] > 0 else 0.0
@@ -701,7 +713,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64+f32+bf16" if args.plain_bf16 else "f64+f32+f16",
+        "dtype": "f64+f32+bf16" if args.plain_bf16 else ("f64+f32+f16+fp8" if fp8_tau(args) > 0 else "f64+f32+f16"),
         "data": "synthetic: jittered grid locations (srand(0)), z = L(theta) e from the dlarnv stream, "
                 "generated on the GPU in FP64",
         "config": {"workload": "univariate Matern likelihood, n=%d, theta=(1.0, 0.05, 0.5), dts=%d, mixed plan "
@@ -710,11 +722,14 @@ def main():
                                    "BF16" if args.plain_bf16 else (
                                        "row-scaled split-FP16" if args.single_band < 0 else
                                        "row-scaled split-FP16, one FP16 product beyond band %d where the panel "
-                                       "rows' scales multiply to <= %g of the panel's largest squared"
-                                       % (args.single_band, args.single_tau))),
+                                       "rows' scales multiply to <= %g of the panel's largest squared%s"
+                                       % (args.single_band, args.single_tau,
+                                          ", one FP8 (e4m3) product for FP32 tiles where they multiply to <= %g"
+                                          % fp8_tau(args) if fp8_tau(args) > 0 else ""))),
                    "n": args.n, "dts": args.dts, "band_fp64": args.band_fp64, "band_fp32": args.band_fp32,
                    "single_band": args.single_band, "single_tau": tau_json(args),
                    "fp64_product_band": product_band(args, bivariate=False),
+                   "fp8_tau": fp8_tau(args),
                    "operands": operands(args)},
         "roofline": {
             "bound": "mfma",
@@ -732,9 +747,10 @@ def main():
             "mfma_per_product": mfma_per_flop,
             "frac_mfma_issued": achieved * mfma_per_flop / PEAK[prec],
             "note": "achieved and frac count algorithmic flops (2 ts^3 per tile update) over the launch time; "
-                    "frac_mfma_issued weights them by the MFMAs each product takes (3 per split product, 1 per "
-                    "single one, from the block counts of every per-block launch, FP32 tiles and the FP64 band's "
-                    "together); algorithmic_bytes is each updated tile read and written once",
+                    "frac_mfma_issued weights them by the FP16-MFMA time each product takes (3 per split product, "
+                    "1 per single one, 1/2 per FP8 one, from the block counts of every per-block launch, FP32 tiles "
+                    "and the FP64 band's together); peak is the FP16 dense peak; algorithmic_bytes is each "
+                    "updated tile read and written once",
         },
         "fp64": {
             "evals_per_s": fp64["evals_per_s"], "ms_per_step": fp64["ms_per_step"], "steps": fp64_steps,

@@ -76,6 +76,7 @@ EXPORTS = {
     "egs_memory_cache": ([_i64, ctypes.POINTER(_i64), ctypes.POINTER(_i64)], _int),
     "egs_problem_set_precision": ([_vp, _int, _int, _int], _int),
     "egs_problem_set_single_product": ([_vp, _int, ctypes.c_double], _int),
+    "egs_problem_set_fp8_product": ([_vp, ctypes.c_double], _int),
     "egs_problem_set_fp64_product_band": ([_vp, _int], _int),
     "egs_problem_set_graph": ([_vp, _int], _int),
     "egs_problem_set_observations": ([_vp, _dp, _i64], _int),
@@ -92,6 +93,7 @@ EXPORTS = {
     "egs_problem_set_fallback": ([_vp, _int, ctypes.c_double], _int),
     "egs_last_fallback": ([_vp, ctypes.POINTER(_int), ctypes.POINTER(_i64), _dp], _int),
     "egs_last_product_counts": ([_vp, ctypes.POINTER(_i64), ctypes.POINTER(_i64)], _int),
+    "egs_last_product_counts_fp8": ([_vp, ctypes.POINTER(_i64), ctypes.POINTER(_i64), ctypes.POINTER(_i64)], _int),
     "egs_problem_profile": ([_vp, _int], _int),
     "egs_problem_kernel_stats": ([_vp, _int, ctypes.POINTER(_i64), _dp, _dp], _int),
     "egs_generate_locations": ([_i64, _int, _dp, _dp, _dp], _int),
@@ -271,6 +273,12 @@ class Problem:
         times the panel's largest squared); band None or negative: never."""
         _check(lib().egs_problem_set_single_product(self.handle, -1 if band is None else int(band), float(tau)))
 
+    def set_fp8_product(self, tau):
+        """Blocks of FP32 tiles beyond the single-product band take one FP8
+        (e4m3) product where the row-scale product is <= tau times the
+        panel's largest squared; 0 turns it off."""
+        _check(lib().egs_problem_set_fp8_product(self.handle, float(tau)))
+
     def set_observations(self, z):
         z, pz = _arr(z)
         _check(lib().egs_problem_set_observations(self.handle, pz, len(z)))
@@ -343,11 +351,11 @@ class Problem:
         _check(lib().egs_problem_set_fp64_product_band(self.handle, -1 if band is None else int(band)))
 
     def product_counts(self):
-        """Block updates of the last factorization with the split and with the
-        single FP16 product."""
-        sp, si = _i64(), _i64()
-        _check(lib().egs_last_product_counts(self.handle, ctypes.byref(sp), ctypes.byref(si)))
-        return {"split": sp.value, "single": si.value}
+        """Block updates of the last factorization with the split FP16, the
+        single FP16 and the single FP8 product."""
+        sp, si, s8 = _i64(), _i64(), _i64()
+        _check(lib().egs_last_product_counts_fp8(self.handle, ctypes.byref(sp), ctypes.byref(si), ctypes.byref(s8)))
+        return {"split": sp.value, "single": si.value, "fp8": s8.value}
 
     def profile(self, enable=True):
         _check(lib().egs_problem_profile(self.handle, 1 if enable else 0))

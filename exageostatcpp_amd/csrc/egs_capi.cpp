@@ -137,6 +137,11 @@ struct egs_problem_s {
     // the bivariate kernel (its sweep's errors grew to 1.4-3.0e-6).
     bool f32_panel_solve = true;
     double tau_single = 0x1p-16;
+    // One FP8 (e4m3) product for FP32-tile blocks beyond band_single whose
+    // panel rows have bmax(A) bmax(B) <= tau_fp8 pmax^2 (0: never).  2^-23 =
+    // 2^-7 tau_single for univariate kernels: an e4m3 product's error bound
+    // (2^-4 relative) there equals the FP16 product's (2^-11) at tau_single.
+    double tau_fp8 = 0.0;
     // mixed plans: per-class update lists (descending column) and, for class c,
     // cls_count[c][j] = number of its tiles in columns >= j
     int2 *d_pairs_cls[kClasses] = {};
@@ -148,10 +153,11 @@ struct egs_problem_s {
     double *d_p64[kPanelSlots] = {};
     float *d_p32[kPanelSlots] = {};
     uint16_t *d_p16h[kPanelSlots] = {}, *d_p16l[kPanelSlots] = {};   // FP16 or BF16 bits
+    uint8_t *d_p8[kPanelSlots] = {};                                  // e4m3 of the row-scaled values
     float *d_pinv[kPanelSlots] = {};   // row scales of the FP16 parts (inverse, powers of two)
     float *d_bmax[kPanelSlots] = {};   // largest row scale per 192-row block of a panel
     unsigned *d_pmax = nullptr;              // largest row scale per panel (float bits), nt entries
-    unsigned long long *d_counts = nullptr;  // split / single-product block updates of the last factorization
+    unsigned long long *d_counts = nullptr;  // split / single / FP8-product block updates of the last factorization
     double *d_cg = nullptr;                  // refinement vectors x, r, p, q, w (npad each)
     // FP64 copy of the lower tiles of Sigma written with every covariance
     // generation of a mixed plan (the refinement's Sigma v reads it), and the
@@ -355,6 +361,8 @@ void free_problem_buffers(egs_problem_s *p) {
         pfree(p, p->d_p32[b]);
         pfree(p, p->d_p16h[b]);
         pfree(p, p->d_p16l[b]);
+        pfree(p, p->d_p8[b]);
+        p->d_p8[b] = nullptr;
         pfree(p, p->d_pinv[b]);
         pfree(p, p->d_bmax[b]);
         p->d_pinv[b] = nullptr;
@@ -495,12 +503,13 @@ int setup_mixed(egs_problem_s *p) {
                    pmalloc(p, &p->d_p32[b], panel * sizeof(float)) == hipSuccess &&
                    pmalloc(p, &p->d_p16h[b], panel * sizeof(uint16_t)) == hipSuccess &&
                    pmalloc(p, &p->d_p16l[b], panel * sizeof(uint16_t)) == hipSuccess &&
+                   pmalloc(p, &p->d_p8[b], panel) == hipSuccess &&
                    pmalloc(p, &p->d_pinv[b], sizeof(float) * (size_t) p->npad) == hipSuccess &&
                    pmalloc(p, &p->d_bmax[b], sizeof(float) * (size_t) (p->npad / 192)) == hipSuccess;
     if (!slots_ok) return fail(EGS_ERR_OUT_OF_MEMORY, "cannot allocate the mixed-precision panel copies");
     if (!p->d_cg) {
         if (pmalloc(p, &p->d_pmax, sizeof(unsigned) * (size_t) p->nt) != hipSuccess ||
-            pmalloc(p, &p->d_counts, 2 * sizeof(unsigned long long)) != hipSuccess ||
+            pmalloc(p, &p->d_counts, 3 * sizeof(unsigned long long)) != hipSuccess ||
             pmalloc(p, &p->d_cg, 5 * sizeof(double) * (size_t) p->npad) != hipSuccess ||
             pmalloc(p, &p->d_mv, sizeof(double) * (size_t) egs::matvec_chunks(p->n) * p->npad) != hipSuccess ||
             pmalloc(p, &p->d_dot, sizeof(double) * egs::dot_blocks()) != hipSuccess)
@@ -626,6 +635,7 @@ void factor_panel(egs_problem_s *p, int k) {
             const int sl = k % kPanelSlots;
             if (split) egs::launch_panel_rowscale(m, k, p->d_pinv[sl], p->d_bmax[sl], p->d_pmax, s);
             egs::launch_panel_convert(m, k, p->d_p64[sl], p->d_p32[sl], p->d_p16h[sl], p->d_p16l[sl],
+                                      split && p->tau_fp8 > 0.0 ? p->d_p8[sl] : nullptr,
                                       split ? p->d_pinv[sl] : nullptr, s);
         }
     }
@@ -677,7 +687,8 @@ void launch_mixed_update(egs_problem_s *p, int k, int jcol_lo, int jcol_hi, hipS
                 : c == 3 ? (all_single ? egs::kSingleF16 : egs::kAdaptiveF16)
                          : (p->mode == EGS_PRECISION_MIXED ? egs::kSplitF16 : egs::kSingleBF16);
             egs::launch_update192_16(m, p->d_p16h[slot], p->d_p16l[slot], p->d_pinv[slot], p->d_bmax[slot],
-                                     p->d_pmax, (float) p->tau_single, pr, end - beg, k, op, p->d_counts, st);
+                                     p->d_pmax, (float) p->tau_single, p->d_p8[slot], (float) p->tau_fp8, pr,
+                                     end - beg, k, op, p->d_counts, st);
         }
         prof_end(p, pi, st);
     }
@@ -711,7 +722,7 @@ int run_potrf(egs_problem_s *p) {
     egs::launch_zero_u32(reinterpret_cast<unsigned *>(p->d_info), 1, s);
     if (p->mode == EGS_PRECISION_MIXED) {
         egs::launch_zero_u32(p->d_pmax, nt, s);
-        egs::launch_zero_u32(reinterpret_cast<unsigned *>(p->d_counts), 4, s);
+        egs::launch_zero_u32(reinterpret_cast<unsigned *>(p->d_counts), 6, s);
     }
     factor_panel(p, 0);
     EGS_HIP(hipEventRecord(p->ev_panel[0], s));
@@ -1022,6 +1033,7 @@ void init_shape(egs_problem_s *p, int kernel, int64_t n_loc, int dim, int dts) {
     p->band_prod64 = kernel == EGS_KERNEL_BIVARIATE_MATERN_PARSIMONIOUS ? 2 : 0;
     p->single_fp64 = kernel != EGS_KERNEL_BIVARIATE_MATERN_PARSIMONIOUS;
     p->f32_panel_solve = kernel != EGS_KERNEL_BIVARIATE_MATERN_PARSIMONIOUS;
+    p->tau_fp8 = kernel == EGS_KERNEL_BIVARIATE_MATERN_PARSIMONIOUS ? 0.0 : 0x1p-23;
     p->p = kernel == EGS_KERNEL_BIVARIATE_MATERN_PARSIMONIOUS ? 2 : 1;
     p->dim = dim;
     p->dts = dts;
@@ -1058,8 +1070,8 @@ int64_t footprint(const egs_problem_s &q) {
     for (int c = 0; c < kClasses; ++c) b += dev_class_bytes(8 * std::max<int64_t>(1, count[c]));
     const int64_t panel = npad * ts;
     b += kPanelSlots * (dev_class_bytes(8 * panel) + dev_class_bytes(4 * panel) + 2 * dev_class_bytes(2 * panel) +
-                        dev_class_bytes(4 * npad) + dev_class_bytes(4 * (npad / 192)));
-    b += dev_class_bytes(4 * nt) + dev_class_bytes(16) + dev_class_bytes(5 * vb) +
+                        dev_class_bytes(panel) + dev_class_bytes(4 * npad) + dev_class_bytes(4 * (npad / 192)));
+    b += dev_class_bytes(4 * nt) + dev_class_bytes(24) + dev_class_bytes(5 * vb) +
          dev_class_bytes(8 * (int64_t) egs::matvec_chunks(q.n) * npad) + dev_class_bytes(8 * egs::dot_blocks());
     b += dev_class_bytes(8 * nt * ts * ts) + dev_class_bytes(vb);
     b += dev_class_bytes(8 * ntiles * ts * ts) + 2 * dev_class_bytes(8 * ntiles * ts);
@@ -1342,6 +1354,16 @@ int egs_problem_set_fp64_product_band(egs_problem_t p, int band) {
     invalidate_graph(p);
     p->band_prod64 = band < 0 ? (1 << 30) : band;
     return is_mixed(p) ? setup_mixed(p) : EGS_SUCCESS;
+}
+
+int egs_problem_set_fp8_product(egs_problem_t p, double tau) {
+    int st = check_problem(p);
+    if (st) return st;
+    if (std::isnan(tau) || tau < 0.0) return fail(EGS_ERR_INVALID_ARG, "tau must be >= 0 (got %g)", tau);
+    EGS_HIP(hipStreamSynchronize(p->ctx->stream));
+    invalidate_graph(p);
+    p->tau_fp8 = tau;
+    return EGS_SUCCESS;
 }
 
 int egs_problem_set_single_product(egs_problem_t p, int band, double tau) {
@@ -1786,15 +1808,20 @@ int egs_last_refinement(egs_problem_t p, int *iterations, double *residual) {
 }
 
 int egs_last_product_counts(egs_problem_t p, int64_t *split_blocks, int64_t *single_blocks) {
+    return egs_last_product_counts_fp8(p, split_blocks, single_blocks, nullptr);
+}
+
+int egs_last_product_counts_fp8(egs_problem_t p, int64_t *split_blocks, int64_t *single_blocks, int64_t *fp8_blocks) {
     int st = check_problem(p);
     if (st) return st;
-    unsigned long long c[2] = {0, 0};
+    unsigned long long c[3] = {0, 0, 0};
     if (p->mode == EGS_PRECISION_MIXED && p->d_counts) {
         EGS_HIP(hipStreamSynchronize(p->ctx->stream));
         EGS_HIP(hipMemcpy(c, p->d_counts, sizeof(c), hipMemcpyDeviceToHost));
     }
     if (split_blocks) *split_blocks = (int64_t) c[0];
     if (single_blocks) *single_blocks = (int64_t) c[1];
+    if (fp8_blocks) *fp8_blocks = (int64_t) c[2];
     return EGS_SUCCESS;
 }
 

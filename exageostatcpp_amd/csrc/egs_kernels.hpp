@@ -129,8 +129,10 @@ void launch_panel_rowscale(const TileMap &map, int k, float *inv_scale, float *b
 void launch_zero_u32(unsigned *p, int n, hipStream_t s);
 // Panel k to FP64 and FP32 column-major copies and 16-bit hi/lo row-major parts:
 // FP16 of the row-scaled values when inv_scale is given, BF16 of the values otherwise.
+// With inv_scale, p8 (optional) receives the OCP e4m3 byte of each row-scaled
+// value, row-major like the 16-bit parts.
 void launch_panel_convert(const TileMap &map, int k, double *p64, float *p32, uint16_t *p16h, uint16_t *p16l,
-                          const float *inv_scale, hipStream_t s);
+                          uint8_t *p8, const float *inv_scale, hipStream_t s);
 void launch_update192_f32(const TileMap &map, const float *p32, const int2 *pairs, int npairs, int k, hipStream_t s);
 // Operands of the 16-bit update: split FP16 (hi*hi + hi*lo + lo*hi of the
 // row-scaled parts), one FP16 product of the row-scaled hi parts, the choice
@@ -145,10 +147,13 @@ enum Operands {
 };
 // C -= A B^T from the 16-bit parts of panel k.  The single and per-block forms
 // add the number of 192 x 192 blocks they update with the split product to
-// counts[0] and with one product to counts[1].
+// counts[0], with one product to counts[1] and with one FP8 product to
+// counts[2].  kAdaptiveF16 with tau8 > 0 takes one e4m3 product from p8 (the
+// panel's FP8 copy) for blocks with bmax(A) bmax(B) <= tau8 pmax[k]^2.
 void launch_update192_16(const TileMap &map, const uint16_t *p16h, const uint16_t *p16l, const float *inv_scale,
-                         const float *bmax, const unsigned *pmax, float tau, const int2 *pairs, int npairs, int k,
-                         Operands op, unsigned long long *counts, hipStream_t s);
+                         const float *bmax, const unsigned *pmax, float tau, const uint8_t *p8, float tau8,
+                         const int2 *pairs, int npairs, int k, Operands op, unsigned long long *counts,
+                         hipStream_t s);
 // FP64 update reading panel k from a contiguous FP64 copy (tile i at p64 + (i-k-1) ts^2).
 void launch_update192_panel(const TileMap &map, const double *p64, const int2 *pairs, int npairs, int k,
                             hipStream_t s);

@@ -114,7 +114,7 @@ __global__ void zero_u32_kernel(unsigned *p, int n) {
 // workgroup per 64 x 64 block.
 // ------------------------------------------------------------------------
 __global__ void __launch_bounds__(kThreads) panel_convert_kernel(TileMap map, int k, double *p64, float *p32,
-                                                                 uint16_t *p16h, uint16_t *p16l,
+                                                                 uint16_t *p16h, uint16_t *p16l, uint8_t *p8,
                                                                  const float *inv_scale) {
     __shared__ float sT[kBlk][kBlk + 1];
     const int ts = map.ts;
@@ -149,6 +149,8 @@ __global__ void __launch_bounds__(kThreads) panel_convert_kernel(TileMap map, in
             const _Float16 lo = (_Float16) (xs - (float) hi);
             p16h[o] = __builtin_bit_cast(uint16_t, hi);
             p16l[o] = __builtin_bit_cast(uint16_t, lo);
+            // OCP e4m3 of the same row-scaled value (|xs| < 1, nearest even)
+            if (p8) p8[o] = (uint8_t) (__builtin_amdgcn_cvt_pk_fp8_f32(xs, 0.0f, 0, false) & 0xff);
         } else {
             const __bf16 hi = (__bf16) x;
             const __bf16 lo = (__bf16) (x - (float) hi);
@@ -304,15 +306,33 @@ __device__ __forceinline__ f32x16 mfma_32(const u16x8 &x, const u16x8 &y, f32x16
                                                        acc, 0, 0, 0);
 }
 
+typedef int i32x8 __attribute__((ext_vector_type(8)));
+
+// One e4m3 x e4m3 product over 64 K values, FP32 accumulation (scales 2^0).
+// x and y each hold 32 one-byte values per lane: lanes 0-31 K 0-31 of the
+// chunk, lanes 32-63 K 32-63, from the two 16-byte planes p, p + 1.
+__device__ __forceinline__ f32x16 mfma_fp8(const u16x8 &x0, const u16x8 &x1, const u16x8 &y0, const u16x8 &y1,
+                                           f32x16 acc) {
+    struct Pair {
+        u16x8 a, b;
+    };
+    const i32x8 x = __builtin_bit_cast(i32x8, Pair{x0, x1});
+    const i32x8 y = __builtin_bit_cast(i32x8, Pair{y0, y1});
+    return __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(x, y, acc, 0, 0, 0, 127, 0, 127);
+}
+
 // kMode: 0 split product, 1 single product, 2 chosen per block: single when
-// bmax(A rows) * bmax(B rows) <= tau * pmax(panel k)^2, split otherwise.
+// bmax(A rows) * bmax(B rows) <= tau * pmax(panel k)^2, split otherwise; with
+// an FP8 copy of the panel (p8, FP32 tiles only), one e4m3 product where that
+// ratio is also <= tau8.
 enum { kModeSplit = 0, kModeSingle = 1, kModeAdaptive = 2 };
 
 template <typename T16, int kMode, int kDepth, bool kC64 = false>
 __global__ void __launch_bounds__(kNTw, kMode == kModeSplit ? 1 : 2) update192_w_kernel(TileMap map, const uint16_t *p16h, const uint16_t *p16l,
                                                               const float *inv_scale, const float *bmax,
-                                                              const unsigned *pmax, float tau, const int2 *pairs,
-                                                              int k, unsigned long long *counts) {
+                                                              const unsigned *pmax, float tau, const uint16_t *p8,
+                                                              float tau8, const int2 *pairs, int k,
+                                                              unsigned long long *counts) {
     // [buffer][part]: the split kernel double-buffers hi and lo; the others
     // double-buffer one part, and the per-block kernel's split path uses the
     // two buffers for hi and lo, single-buffered (50 KB: two workgroups per CU)
@@ -428,6 +448,71 @@ __global__ void __launch_bounds__(kNTw, kMode == kModeSplit ? 1 : 2) update192_w
             }
         }
     };
+    // One FP8 product: the FP8 copy of the panel has one byte per value, so a
+    // chunk of the same 64 bytes per row (four 16-byte planes, staged as in
+    // run) holds 64 K values, and ts / 64 chunks cover K.  The panel is read
+    // as 16-bit words: row stride ts / 2, chunk stride 32 words.
+    auto run_fp8 = [&]() {
+        const int64_t a8 = ((int64_t) (ij.x - k - 1) * tsz + (int64_t) sbm * kBM * ts) / 2;
+        const int64_t b8 = ((int64_t) (ij.y - k - 1) * tsz + (int64_t) sbn * kBM * ts) / 2;
+        const uint16_t *s8a = p8 + a8, *s8b = p8 + b8;
+        const int nch8 = ts / (2 * kKCw);
+        // one chunk of loads in flight (a chunk is twice the FP16 path's K):
+        // two would not fit beside the accumulators without spilling
+        constexpr int kD8 = 1;
+        uint32_t g8[kPerThreadW];
+#pragma unroll
+        for (int q = 0; q < kPerThreadW; ++q) {
+            const int e = t + kNTw * q;
+            g8[q] = (uint32_t) ((e >> 2) * (ts / 2) + (e & 3) * 8);
+        }
+        u16x8 rg[kD8][kPerThreadW][2];
+        auto issue = [&](int set, int chunk) {
+            const uint32_t kofs = (uint32_t) (chunk * kKCw);
+#pragma unroll
+            for (int q = 0; q < kPerThreadW; ++q) {
+                rg[set][q][0] = *reinterpret_cast<const u16x8 *>(s8a + g8[q] + kofs);
+                rg[set][q][1] = *reinterpret_cast<const u16x8 *>(s8b + g8[q] + kofs);
+            }
+        };
+        auto stage = [&](int set, int buf) {
+#pragma unroll
+            for (int q = 0; q < kPerThreadW; ++q) {
+                const int e = t + kNTw * q;
+                sA[buf][0][e & 3][e >> 2] = rg[set][q][0];
+                sB[buf][0][e & 3][e >> 2] = rg[set][q][1];
+            }
+        };
+#pragma unroll
+        for (int u = 0; u < kD8; ++u)
+            if (u < nch8) issue(u, u);
+        stage(0, 0);
+        __syncthreads();
+        for (int c0 = 0; c0 < nch8; c0 += kD8) {
+#pragma unroll
+            for (int u = 0; u < kD8; ++u) {
+                const int c = c0 + u;
+                if (c >= nch8) break;
+                const int buf = c & 1;
+                if (c + kD8 < nch8) issue(u, c + kD8);
+                u16x8 a0[3], a1[3];
+#pragma unroll
+                for (int mi = 0; mi < 3; ++mi) {
+                    a0[mi] = sA[buf][0][2 * lk][wm * 96 + mi * 32 + lr];
+                    a1[mi] = sA[buf][0][2 * lk + 1][wm * 96 + mi * 32 + lr];
+                }
+#pragma unroll
+                for (int ni = 0; ni < 3; ++ni) {
+                    const u16x8 b0 = sB[buf][0][2 * lk][wn * 96 + ni * 32 + lr];
+                    const u16x8 b1 = sB[buf][0][2 * lk + 1][wn * 96 + ni * 32 + lr];
+#pragma unroll
+                    for (int mi = 0; mi < 3; ++mi) acc[mi][ni] = mfma_fp8(b0, b1, a0[mi], a1[mi], acc[mi][ni]);
+                }
+                if (c + 1 < nch8) stage((u + 1) % kD8, buf ^ 1);
+                __syncthreads();
+            }
+        }
+    };
     // Split product with hi and lo in the two LDS buffers, one chunk at a time
     // (the per-block kernel): loads of chunk c+1 are in flight while chunk c
     // is multiplied.
@@ -481,20 +566,24 @@ __global__ void __launch_bounds__(kNTw, kMode == kModeSplit ? 1 : 2) update192_w
         }
     };
     bool single = kMode == kModeSingle;
+    bool fp8 = false;
     if constexpr (kMode == kModeAdaptive) {
         // uniform over the workgroup
         const float a = bmax[(ij.x - k - 1) * nsb + sbm];
         const float b = bmax[(ij.y - k - 1) * nsb + sbn];
         const float pm = __uint_as_float(pmax[k]);
         single = a * b <= tau * pm * pm;
+        if constexpr (!kC64) fp8 = single && p8 != nullptr && a * b <= tau8 * pm * pm;
     }
-    if (counts && t == 0) atomicAdd(counts + (single ? 1 : 0), 1ull);
+    if (counts && t == 0) atomicAdd(counts + (fp8 ? 2 : single ? 1 : 0), 1ull);
     if constexpr (kMode == kModeSplit) {
         run(std::true_type{});
     } else if constexpr (kMode == kModeSingle) {
         run(std::false_type{});
     } else {
-        if (single)
+        if (fp8)
+            run_fp8();
+        else if (single)
             run(std::false_type{});
         else
             run_split_1buf();
@@ -555,12 +644,12 @@ void launch_zero_u32(unsigned *p, int n, hipStream_t s) {
 }
 
 void launch_panel_convert(const TileMap &map, int k, double *p64, float *p32, uint16_t *p16h, uint16_t *p16l,
-                          const float *inv_scale, hipStream_t s) {
+                          uint8_t *p8, const float *inv_scale, hipStream_t s) {
     const int rows = map.nt - k - 1;
     if (rows <= 0) return;
     const int nsb = map.ts / kBlk;
     hipLaunchKernelGGL(panel_convert_kernel, dim3((unsigned) (rows * nsb * nsb)), dim3(kThreads), 0, s, map, k, p64,
-                       p32, p16h, p16l, inv_scale);
+                       p32, p16h, p16l, p8, inv_scale);
 }
 
 void launch_update192_f32(const TileMap &map, const float *p32, const int2 *pairs, int npairs, int k, hipStream_t s) {
@@ -571,40 +660,42 @@ void launch_update192_f32(const TileMap &map, const float *p32, const int2 *pair
 }
 
 void launch_update192_16(const TileMap &map, const uint16_t *p16h, const uint16_t *p16l, const float *inv_scale,
-                         const float *bmax, const unsigned *pmax, float tau, const int2 *pairs, int npairs, int k,
-                         Operands op, unsigned long long *counts, hipStream_t s) {
+                         const float *bmax, const unsigned *pmax, float tau, const uint8_t *p8, float tau8,
+                         const int2 *pairs, int npairs, int k, Operands op, unsigned long long *counts,
+                         hipStream_t s) {
     if (npairs <= 0) return;
     const int nsb = map.ts / kBM;
     const dim3 grid((unsigned) npairs * (unsigned) (nsb * nsb));
+    const uint16_t *w8 = reinterpret_cast<const uint16_t *>(p8);
     // two chunks of loads in flight: a third measured no faster
     switch (op) {
     case kSplitF16:
         hipLaunchKernelGGL((update192_w_kernel<_Float16, kModeSplit, 2>), grid, dim3(kNTw), 0, s, map, p16h, p16l,
-                           inv_scale, bmax, pmax, tau, pairs, k, nullptr);
+                           inv_scale, bmax, pmax, tau, nullptr, 0.0f, pairs, k, nullptr);
         break;
     case kSingleF16:
         hipLaunchKernelGGL((update192_w_kernel<_Float16, kModeSingle, 2>), grid, dim3(kNTw), 0, s, map, p16h, p16l,
-                           inv_scale, bmax, pmax, tau, pairs, k, counts);
+                           inv_scale, bmax, pmax, tau, nullptr, 0.0f, pairs, k, counts);
         break;
     case kAdaptiveF16:
         hipLaunchKernelGGL((update192_w_kernel<_Float16, kModeAdaptive, 2>), grid, dim3(kNTw), 0, s, map, p16h,
-                           p16l, inv_scale, bmax, pmax, tau, pairs, k, counts);
+                           p16l, inv_scale, bmax, pmax, tau, tau8 > 0.0f ? w8 : nullptr, tau8, pairs, k, counts);
         break;
     case kSplitF16C64:
         hipLaunchKernelGGL((update192_w_kernel<_Float16, kModeSplit, 2, true>), grid, dim3(kNTw), 0, s, map, p16h,
-                           p16l, inv_scale, bmax, pmax, tau, pairs, k, nullptr);
+                           p16l, inv_scale, bmax, pmax, tau, nullptr, 0.0f, pairs, k, nullptr);
         break;
     case kSingleF16C64:
         hipLaunchKernelGGL((update192_w_kernel<_Float16, kModeSingle, 2, true>), grid, dim3(kNTw), 0, s, map, p16h,
-                           p16l, inv_scale, bmax, pmax, tau, pairs, k, counts);
+                           p16l, inv_scale, bmax, pmax, tau, nullptr, 0.0f, pairs, k, counts);
         break;
     case kAdaptiveF16C64:
         hipLaunchKernelGGL((update192_w_kernel<_Float16, kModeAdaptive, 2, true>), grid, dim3(kNTw), 0, s, map, p16h,
-                           p16l, inv_scale, bmax, pmax, tau, pairs, k, counts);
+                           p16l, inv_scale, bmax, pmax, tau, nullptr, 0.0f, pairs, k, counts);
         break;
     default:
         hipLaunchKernelGGL((update192_w_kernel<__bf16, kModeSingle, 2>), grid, dim3(kNTw), 0, s, map, p16h, p16l,
-                           nullptr, nullptr, nullptr, 0.0f, pairs, k, nullptr);
+                           nullptr, nullptr, nullptr, 0.0f, nullptr, 0.0f, pairs, k, nullptr);
     }
 }
 

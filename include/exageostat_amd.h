@@ -147,6 +147,19 @@ int egs_problem_set_precision(egs_problem_t problem, int mode, int band_fp64, in
  * (2^-24 K smax^2).
  * Kept across egs_problem_set_precision calls. */
 int egs_problem_set_single_product(egs_problem_t problem, int band, double tau);
+/* Under EGS_PRECISION_MIXED, a 192 x 192 block of an FP32 tile beyond the
+ * single-product band that passes the single-product test and also has
+ * smax(A rows) * smax(B rows) <= tau * smax(panel)^2 takes one product of
+ * OCP e4m3 values (3 significant bits) of the row-scaled panel on FP8 MFMA,
+ * accumulated in FP32, instead of an FP16 one.  Its error bound, about
+ * 2^-4 K smax(A) smax(B), equals the FP16 product's at the single-product
+ * threshold when tau = 2^-7 times that threshold.  0 turns it off, and so
+ * does a single-product tau of infinity (every block then takes the FP16
+ * product without a per-block test).
+ * Default: 2^-23 for univariate kernels (2^-7 of the default single-product
+ * tau), 0 for the bivariate one.  Kept across egs_problem_set_precision
+ * calls.  No counterpart in the reference. */
+int egs_problem_set_fp8_product(egs_problem_t problem, double tau);
 /* Under EGS_PRECISION_MIXED, tiles of the FP64 band (|i - j| <= band_fp64)
  * with |i - j| > band keep FP64 storage but are updated with split FP16
  * products accumulated in FP32 and added in FP64 (for univariate kernels one
@@ -190,6 +203,9 @@ int egs_last_refinement(egs_problem_t problem, int *iterations, double *residual
  * last factorization under EGS_PRECISION_MIXED: those done with the split FP16
  * product and those done with one FP16 product (0 and 0 otherwise). */
 int egs_last_product_counts(egs_problem_t problem, int64_t *split_blocks, int64_t *single_blocks);
+/* The same with the blocks done with one FP8 product (egs_problem_set_fp8_product). */
+int egs_last_product_counts_fp8(egs_problem_t problem, int64_t *split_blocks, int64_t *single_blocks,
+                                int64_t *fp8_blocks);
 
 /* Observations Z (length n_loc * P): the measurements descriptor Z and its
  * restore copy Z_COPY of ChameleonImplementation.cpp:113-125. */

@@ -53,7 +53,7 @@ def test_committed_traffic_summary_matches_the_default_workload():
     import bench
 
     a = argparse.Namespace(n=100000, dts=960, band_fp64=4, band_fp32=4, single_band=0, single_tau=2.0 ** -16,
-                           fp64_product_band=None, plain_bf16=False)
+                           fp64_product_band=None, plain_bf16=False, fp8_tau=None)
     traffic, alg = bench.measured_traffic("update_16_select", a)
     assert traffic is not None and 1.0 < traffic / alg < 3.0
     # another plan must not reuse it

@@ -176,7 +176,7 @@ def test_single_product_selection_pins_both_ends(ctx, data4k):
     p.set_fallback(0, 0.0)
     p.set_single_product(-1)
     split_only = p.loglik(th)
-    assert p.product_counts() == {"split": 0, "single": 0}
+    assert p.product_counts() == {"split": 0, "single": 0, "fp8": 0}
     p.set_single_product(0, 0.0)
     tau0 = p.loglik(th)
     c0 = p.product_counts()
@@ -190,8 +190,45 @@ def test_single_product_selection_pins_both_ends(ctx, data4k):
                                                                           dflt[0], cd))
     assert tau0 == split_only
     assert c0["split"] > 0 and c0["split"] + c0["single"] == cinf["single"] and cinf["split"] == 0
-    assert cd["split"] + cd["single"] == cinf["single"] and cd["single"] > 0 and cd["split"] > 0
+    # the default plan's smallest single-product blocks take the FP8 tier
+    assert cd["split"] + cd["single"] + cd["fp8"] == cinf["single"] and cd["single"] > 0 and cd["split"] > 0
     assert tau_inf[1] != split_only[1]
+    assert abs(dflt[0] - ref["llh"]) <= 1e-6 * abs(ref["llh"])
+    p.close()
+
+
+def test_fp8_tier_moves_single_blocks_and_keeps_accuracy(ctx, data4k):
+    """The FP8 tier of the per-block update (FP32 tiles beyond the band): with
+    tau8 = 0 no block takes it; with tau8 equal to the single-product tau it
+    takes over single-product blocks and leaves the split ones alone (block
+    counts); with tau8 = infinity every block of the per-block FP32 class takes
+    it (the result changes but stays finite); the default tau8 = 2^-23 keeps
+    the plan within 1e-6 of the oracle."""
+    x, y, th, z, ref = data4k
+    p = egs.Problem(ctx, 0, x, y, dts=192)
+    p.set_observations(z)
+    p.set_precision(egs.PRECISION_MIXED, 1, 3)
+    p.set_fallback(0, 0.0)
+    p.set_fp8_product(0.0)
+    off = p.loglik(th)
+    c_off = p.product_counts()
+    p.set_fp8_product(2.0 ** -16)
+    eq = p.loglik(th)
+    c_eq = p.product_counts()
+    p.set_fp8_product(float("inf"))
+    allf = p.loglik(th)
+    c_all = p.product_counts()
+    p.set_fp8_product(2.0 ** -23)
+    dflt = p.loglik(th)
+    c_d = p.product_counts()
+    log("  off %r %s; tau8 = tau %r %s; all %r %s; default %r %s (oracle %r)" %
+        (off[0], c_off, eq[0], c_eq, allf[0], c_all, dflt[0], c_d, ref["llh"]))
+    assert c_off["fp8"] == 0
+    assert c_eq["fp8"] > 0 and c_eq["split"] == c_off["split"] and c_eq["single"] + c_eq["fp8"] == c_off["single"]
+    assert c_all["fp8"] >= c_eq["fp8"] and c_all["split"] + c_all["single"] + c_all["fp8"] == \
+        c_off["split"] + c_off["single"]
+    assert np.isfinite(allf[0]) and allf[1] != off[1]
+    assert c_d["split"] + c_d["single"] + c_d["fp8"] == c_off["split"] + c_off["single"]
     assert abs(dflt[0] - ref["llh"]) <= 1e-6 * abs(ref["llh"])
     p.close()
 
