@@ -125,8 +125,8 @@ def operands(args):
 
 def fp8_tau(args):
     """The FP8-product threshold in effect for the univariate kernel: the
-    option, else the library default (2^-23; 0 turns the tier off)."""
-    return args.fp8_tau if args.fp8_tau is not None else 2.0 ** -23
+    option, else the library default (2^-26; 0 turns the tier off)."""
+    return args.fp8_tau if args.fp8_tau is not None else 2.0 ** -26
 
 
 def product_band(args, bivariate):
@@ -521,6 +521,7 @@ def mle_compare(egs, args):
             fp64_prob.set_observations(prob.get_observations())
             prob.set_precision(mode, args.band_fp64, args.band_fp32)
             prob.set_single_product(args.single_band, args.single_tau)
+            prob.set_fp8_product(fp8_tau(args))
             if args.fp64_product_band is not None:
                 prob.set_fp64_product_band(args.fp64_product_band)
         count, fallback = [0], [0]
@@ -580,7 +581,7 @@ def main():
                     help="... when their panel rows' scales multiply to <= tau times the panel's largest squared")
     ap.add_argument("--fp8-tau", type=float, default=None,
                     help="FP32-tile blocks whose panel rows' scales multiply to <= this times the panel's largest "
-                         "squared take one FP8 (e4m3) product (0: never; default: the library's, 2^-23)")
+                         "squared take one FP8 (e4m3) product (0: never; default: the library's, 2^-26)")
     ap.add_argument("--plain-bf16", action="store_true", help="single BF16 product instead of the split one")
     ap.add_argument("--cpu-sample", type=int, default=20000,
                     help="n of the host sample (each evaluation a few seconds on 16 host cores)")

@@ -138,9 +138,9 @@ struct egs_problem_s {
     bool f32_panel_solve = true;
     double tau_single = 0x1p-16;
     // One FP8 (e4m3) product for FP32-tile blocks beyond band_single whose
-    // panel rows have bmax(A) bmax(B) <= tau_fp8 pmax^2 (0: never).  2^-23 =
-    // 2^-7 tau_single for univariate kernels: an e4m3 product's error bound
-    // (2^-4 relative) there equals the FP16 product's (2^-11) at tau_single.
+    // panel rows have bmax(A) bmax(B) <= tau_fp8 pmax^2 (0: never).  2^-26 for
+    // univariate kernels: an e4m3 product's error bound (2^-4 relative) there
+    // is 8x under the FP16 product's (2^-11) at tau_single = 2^-16.
     double tau_fp8 = 0.0;
     // mixed plans: per-class update lists (descending column) and, for class c,
     // cls_count[c][j] = number of its tiles in columns >= j
@@ -1033,7 +1033,7 @@ void init_shape(egs_problem_s *p, int kernel, int64_t n_loc, int dim, int dts) {
     p->band_prod64 = kernel == EGS_KERNEL_BIVARIATE_MATERN_PARSIMONIOUS ? 2 : 0;
     p->single_fp64 = kernel != EGS_KERNEL_BIVARIATE_MATERN_PARSIMONIOUS;
     p->f32_panel_solve = kernel != EGS_KERNEL_BIVARIATE_MATERN_PARSIMONIOUS;
-    p->tau_fp8 = kernel == EGS_KERNEL_BIVARIATE_MATERN_PARSIMONIOUS ? 0.0 : 0x1p-23;
+    p->tau_fp8 = kernel == EGS_KERNEL_BIVARIATE_MATERN_PARSIMONIOUS ? 0.0 : 0x1p-26;
     p->p = kernel == EGS_KERNEL_BIVARIATE_MATERN_PARSIMONIOUS ? 2 : 1;
     p->dim = dim;
     p->dts = dts;

@@ -156,8 +156,9 @@ int egs_problem_set_single_product(egs_problem_t problem, int band, double tau);
  * threshold when tau = 2^-7 times that threshold.  0 turns it off, and so
  * does a single-product tau of infinity (every block then takes the FP16
  * product without a per-block test).
- * Default: 2^-23 for univariate kernels (2^-7 of the default single-product
- * tau), 0 for the bivariate one.  Kept across egs_problem_set_precision
+ * Default: 2^-26 for univariate kernels (2^-10 of the default single-product
+ * tau: an 8x smaller bound than the FP16 product's there; at 2^-23 a 100k fit
+ * drifted 1.7e-3 from the FP64 estimate), 0 for the bivariate one.  Kept across egs_problem_set_precision
  * calls.  No counterpart in the reference. */
 int egs_problem_set_fp8_product(egs_problem_t problem, double tau);
 /* Under EGS_PRECISION_MIXED, tiles of the FP64 band (|i - j| <= band_fp64)

@@ -202,7 +202,7 @@ def test_fp8_tier_moves_single_blocks_and_keeps_accuracy(ctx, data4k):
     tau8 = 0 no block takes it; with tau8 equal to the single-product tau it
     takes over single-product blocks and leaves the split ones alone (block
     counts); with tau8 = infinity every block of the per-block FP32 class takes
-    it (the result changes but stays finite); the default tau8 = 2^-23 keeps
+    it (the result changes but stays finite); the default tau8 = 2^-26 keeps
     the plan within 1e-6 of the oracle."""
     x, y, th, z, ref = data4k
     p = egs.Problem(ctx, 0, x, y, dts=192)
@@ -218,7 +218,7 @@ def test_fp8_tier_moves_single_blocks_and_keeps_accuracy(ctx, data4k):
     p.set_fp8_product(float("inf"))
     allf = p.loglik(th)
     c_all = p.product_counts()
-    p.set_fp8_product(2.0 ** -23)
+    p.set_fp8_product(2.0 ** -26)
     dflt = p.loglik(th)
     c_d = p.product_counts()
     log("  off %r %s; tau8 = tau %r %s; all %r %s; default %r %s (oracle %r)" %
