@@ -388,8 +388,8 @@ __global__ void __launch_bounds__(kNTw, kMode == kModeSplit ? 1 : 2) update192_w
             for (int q = 0; q < kPerThreadW; ++q)
 #pragma unroll
                 for (int p = 0; p < P; ++p) {
-                    rg[set][q][p] = *reinterpret_cast<const u16x8 *>(src[p] + goff[q] + kofs);
-                    rg[set][q][P + p] = *reinterpret_cast<const u16x8 *>(src[2 + p] + goff[q] + kofs);
+                    rg[set][q][p] = *reinterpret_cast<const u16x8 *>(src[p] + (goff[q] + kofs));
+                    rg[set][q][P + p] = *reinterpret_cast<const u16x8 *>(src[2 + p] + (goff[q] + kofs));
                 }
         };
         auto stage = [&](int set, int buf) {
@@ -471,8 +471,8 @@ __global__ void __launch_bounds__(kNTw, kMode == kModeSplit ? 1 : 2) update192_w
             const uint32_t kofs = (uint32_t) (chunk * kKCw);
 #pragma unroll
             for (int q = 0; q < kPerThreadW; ++q) {
-                rg[set][q][0] = *reinterpret_cast<const u16x8 *>(s8a + g8[q] + kofs);
-                rg[set][q][1] = *reinterpret_cast<const u16x8 *>(s8b + g8[q] + kofs);
+                rg[set][q][0] = *reinterpret_cast<const u16x8 *>(s8a + (g8[q] + kofs));
+                rg[set][q][1] = *reinterpret_cast<const u16x8 *>(s8b + (g8[q] + kofs));
             }
         };
         auto stage = [&](int set, int buf) {
@@ -524,8 +524,8 @@ __global__ void __launch_bounds__(kNTw, kMode == kModeSplit ? 1 : 2) update192_w
             for (int q = 0; q < kPerThreadW; ++q)
 #pragma unroll
                 for (int p = 0; p < 2; ++p) {
-                    rg[q][p] = *reinterpret_cast<const u16x8 *>(src[p] + goff[q] + kofs);
-                    rg[q][2 + p] = *reinterpret_cast<const u16x8 *>(src[2 + p] + goff[q] + kofs);
+                    rg[q][p] = *reinterpret_cast<const u16x8 *>(src[p] + (goff[q] + kofs));
+                    rg[q][2 + p] = *reinterpret_cast<const u16x8 *>(src[2 + p] + (goff[q] + kofs));
                 }
         };
         issue(0);
