@@ -56,7 +56,10 @@ def test_committed_traffic_summary_matches_the_default_workload():
                            fp64_product_band=None, plain_bf16=False, fp8_tau=None)
     traffic, alg = bench.measured_traffic("update_16_select", a)
     assert traffic is not None and 1.0 < traffic / alg < 3.0
-    # another plan must not reuse it
+    # another plan must not reuse it: other bands, or the FP8 tier off
+    a.fp8_tau = 0.0
+    assert bench.measured_traffic("update_16_select", a) == (None, None)
+    a.fp8_tau = None
     a.band_fp64 = 2
     assert bench.measured_traffic("update_16_select", a) == (None, None)
 
